@@ -1,0 +1,253 @@
+/*
+ * tenet_hip.h -- MI355X (gfx950) dense backend for TeNeT's block-sparse
+ * symmetric contraction path: tree-transform (recoupling) replay and
+ * per-coupled-sector block GEMM, behind a plain C ABI.
+ *
+ * This is the drop-in boundary.  Each entry point replaces one seam of the
+ * reference (paths relative to the TeNeT repository root):
+ *
+ *   tnt_gemm_batch        DenseExecutor::matmul_batch_axpby_into /
+ *                         matmul_batch_axpby_with_ops_into
+ *                         (tenet-dense/src/executor.rs:217-312); also the
+ *                         per-job StorageGemm::matmul_range_into seam
+ *                         (tenet-operations/src/fusion_replay.rs:1302-1345,
+ *                         tenet-operations/src/cuda.rs:53-113)
+ *   tnt_plan_upload       compiled TreeTransformStructure handed to a
+ *                         storage-aware backend
+ *                         (tenet-operations/src/transform_structure.rs:82,
+ *                         :1343-1360, :1753-1759)
+ *   tnt_tree_transform    TreeTransformBackend::tree_transform_structure_into_raw
+ *                         and ..._overwrite_into_raw
+ *                         (tenet-operations/src/replay_backend.rs:28-176, :547-623)
+ *   tnt_contract_upload / FusionContractPlan + FusionBlockContractPlan replay:
+ *   tnt_contract          execute_dynamic_tree_execution_artifact_impl
+ *                         (tenet-tensors/src/contract/dynamic.rs:1241-1530) and
+ *                         FusionBlockContractPlan::execute_direct_on_storage_prezeroed
+ *                         (tenet-operations/src/fusion_replay.rs:1195-1266)
+ *   tnt_weighted_dot      weighted_inner_cuda (tenet/src/typed.rs:11210-11262)
+ *   tnt_buf_*             CudaDenseStorage upload/download/zero
+ *                         (tenet-dense/src/cuda_adapter.rs:80-218)
+ *   tnt_ctx_*             CudaDenseContext (tenet-dense/src/cuda_adapter.rs:80-107);
+ *                         one context = one leased executor (tenet/src/runtime.rs:802-840)
+ *   tnt_graph_*           no reference counterpart (the reference launches one
+ *                         GEMM per sector); batches a whole plan into a hipGraph.
+ *
+ * Conventions (kept from the reference):
+ *   - all payloads are flat column-major element buffers; offsets are element
+ *     offsets; leading dimension of a job operand = its row count
+ *     (tenet-dense/src/executor.rs:8-13, :510-515);
+ *   - destinations of one batch are pairwise disjoint, so jobs run in any order;
+ *   - validate before mutate: every shape/offset check runs before any kernel;
+ *   - device mismatch is an error, never a transfer; there is no host fallback.
+ *
+ * Every function returns 0 on success or a negative TNT_E* code; the message
+ * of the last error of a context is returned by tnt_last_error().
+ * A context is single-threaded (like a leased DenseExecutor); use one context
+ * per host thread.
+ */
+#ifndef TENET_HIP_H
+#define TENET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TNT_MAX_RANK 8
+
+enum tnt_status {
+  TNT_OK = 0,
+  TNT_E_INVALID = -1,
+  TNT_E_UNSUPPORTED = -2,
+  TNT_E_SHAPE = -3,
+  TNT_E_OOB = -4,
+  TNT_E_DEVICE_MISMATCH = -5,
+  TNT_E_HIP = -6,
+  TNT_E_OOM = -7
+};
+
+/* Payload element types.  Buffers are FP64 (like the reference's device
+ * subset, tenet-dense/src/cuda_adapter.rs:108-140); TNT_F32 is reserved. */
+enum tnt_dtype { TNT_F64 = 0, TNT_F32 = 1 };
+
+/* MatrixOp (tenet-dense/src/executor.rs:24-31); Adjoint == transpose for real data. */
+enum tnt_op { TNT_OP_IDENTITY = 0, TNT_OP_ADJOINT = 1 };
+
+/* Payload arithmetic of the block GEMM. */
+enum tnt_precision {
+  TNT_PREC_F64 = 0,        /* FP64 MFMA, FP64 accumulate */
+  TNT_PREC_F32 = 1,        /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
+  TNT_PREC_SPLIT_BF16 = 2  /* Ozaki split into BF16 slices, BF16 MFMA, exact FP32 chunk sums, FP64 accumulate */
+};
+
+typedef struct tnt_ctx tnt_ctx;
+typedef struct tnt_buf tnt_buf;
+typedef struct tnt_plan tnt_plan;
+typedef struct tnt_artifact tnt_artifact;
+typedef struct tnt_graph tnt_graph;
+
+/* DenseGemmBatchJob (tenet-dense/src/executor.rs:15-22): column-major
+ * dst[rows x cols] (+)= op(lhs)[rows x contracted] * op(rhs)[contracted x cols]. */
+typedef struct {
+  size_t dst_offset;
+  size_t lhs_offset;
+  size_t rhs_offset;
+  size_t rows;
+  size_t contracted;
+  size_t cols;
+} tnt_gemm_job;
+
+/* One grouped-GEMM job over explicit device pointers (several contractions in
+ * one launch).  Column-major with explicit leading dimensions; op applies to
+ * the stored matrices. */
+typedef struct {
+  double* dst;
+  const double* lhs;
+  const double* rhs;
+  int64_t rows, contracted, cols;
+  int64_t ldd, ldl, ldr;
+  int32_t lhs_op, rhs_op;
+} tnt_gemm_ptr_job;
+
+/* One strided layout of a TreeTransformLayout table
+ * (tenet-operations/src/transform_structure.rs:1753-1759): `shape` is given in
+ * DESTINATION axis order for both roles, `strides` are the element strides of
+ * the block it addresses in its own payload. */
+typedef struct {
+  int64_t offset;
+  int32_t rank;
+  int32_t reserved;
+  int64_t shape[TNT_MAX_RANK];
+  int64_t strides[TNT_MAX_RANK];
+} tnt_layout;
+
+enum tnt_block_kind { TNT_BLOCK_SINGLE = 0, TNT_BLOCK_MULTI = 1 };
+
+/* TreeTransformBlock (transform_structure.rs:1343-1360).
+ * Single: dst layout `dst_layout`, src layout `src_layout`, coefficient
+ *         coeffs[coeff].
+ * Multi:  dst layouts dst_layout .. +dst_count, src layouts src_layout ..
+ *         +src_count, row-major U[dst, src] at coeffs[coeff]; all layouts of
+ *         one Multi block have the same element count. */
+typedef struct {
+  int32_t kind;
+  uint32_t dst_layout;
+  uint32_t dst_count;
+  uint32_t src_layout;
+  uint32_t src_count;
+  uint32_t coeff;
+} tnt_transform_block;
+
+typedef struct {
+  int64_t dst_len;                 /* required destination payload length */
+  int64_t src_len;                 /* required source payload length */
+  size_t n_layouts;
+  const tnt_layout* layouts;
+  size_t n_blocks;
+  const tnt_transform_block* blocks;
+  size_t n_coeffs;
+  const double* coeffs;
+  size_t n_inactive;               /* destination layouts written by no block */
+  const uint32_t* inactive;        /* (scaled by beta / zeroed on overwrite) */
+} tnt_transform_desc;
+
+/* A contiguous region of a payload. */
+typedef struct {
+  int64_t offset;
+  int64_t len;
+} tnt_region;
+
+/* A DynamicTree contraction artifact (contract/dynamic.rs:807-817):
+ * dst = alpha * OUT( CORE( LHS(lhs), RHS(rhs) ) ) + beta * dst.
+ * A NULL transform means that operand (or the core result) is used as is.
+ * Core jobs address the transformed operands and the core destination. */
+typedef struct {
+  const tnt_transform_desc* lhs;
+  const tnt_transform_desc* rhs;
+  const tnt_transform_desc* out;
+  size_t n_jobs;
+  const tnt_gemm_job* jobs;
+  int64_t lhs_len, rhs_len;        /* lengths of the (transformed) core operands */
+  int64_t core_len;                /* length of the core destination */
+  size_t n_uncovered;              /* core-destination regions written by no job */
+  const tnt_region* uncovered;
+  int32_t lhs_op, rhs_op;
+} tnt_contract_desc;
+
+/* ---- context -------------------------------------------------------------- */
+int tnt_ctx_create(int device, tnt_ctx** out);
+int tnt_ctx_destroy(tnt_ctx* ctx);
+const char* tnt_last_error(const tnt_ctx* ctx);
+int tnt_ctx_synchronize(tnt_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for interop and event timing. */
+int tnt_ctx_stream(tnt_ctx* ctx, void** stream);
+/* Device HBM bytes held by the context's pools. */
+int tnt_ctx_pool_bytes(const tnt_ctx* ctx, size_t* bytes);
+
+/* ---- device buffers (flat payloads) --------------------------------------- */
+int tnt_buf_alloc(tnt_ctx* ctx, size_t len, int dtype, tnt_buf** out);
+/* Wrap caller-owned device memory (not freed by tnt_buf_free). */
+int tnt_buf_wrap(tnt_ctx* ctx, void* device_ptr, size_t len, int dtype, tnt_buf** out);
+int tnt_buf_free(tnt_ctx* ctx, tnt_buf* buf);
+int tnt_buf_upload(tnt_ctx* ctx, tnt_buf* buf, const void* host, size_t len);
+int tnt_buf_download(tnt_ctx* ctx, const tnt_buf* buf, void* host, size_t len);
+int tnt_buf_zero(tnt_ctx* ctx, tnt_buf* buf);
+/* Seeded fill over the flat payload: normal == 0 gives the reference's
+ * uniform draw in [-1, 1) (Fill::Rand, tenet/src/typed.rs:2086-2092, bitwise
+ * identical); normal != 0 gives N(0,1) by Box-Muller on the same stream. */
+int tnt_buf_fill_random(tnt_ctx* ctx, tnt_buf* buf, uint64_t seed, int normal);
+int tnt_buf_info(const tnt_buf* buf, void** device_ptr, size_t* len, int* dtype, int* device);
+
+/* ---- batched block GEMM ---------------------------------------------------- */
+/* runs: optional strided-batch run partition (executor.rs:473-488); it is a
+ * scheduling hint only and may be NULL. */
+int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                   const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
+                   int lhs_op, int rhs_op, double alpha, double beta, int precision);
+/* Grouped GEMM over explicit pointers: all jobs in one persistent launch. */
+int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, double alpha,
+                     double beta, int precision);
+
+/* ---- tree-transform (recoupling) replay ----------------------------------- */
+int tnt_plan_upload(tnt_ctx* ctx, const tnt_transform_desc* desc, tnt_plan** out);
+int tnt_plan_destroy(tnt_ctx* ctx, tnt_plan* plan);
+/* dst = alpha * T(src) + beta * dst, or dst = alpha * T(src) when overwrite != 0
+ * (inactive destination layouts are then zeroed, never read). */
+int tnt_tree_transform(tnt_ctx* ctx, const tnt_plan* plan, tnt_buf* dst, const tnt_buf* src,
+                       double alpha, double beta, int overwrite);
+
+/* ---- contraction artifact -------------------------------------------------- */
+int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* desc, tnt_artifact** out);
+int tnt_contract_destroy(tnt_ctx* ctx, tnt_artifact* art);
+int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_buf* lhs,
+                 const tnt_buf* rhs, double alpha, double beta, int precision);
+
+/* ---- reductions ------------------------------------------------------------ */
+/* out = sum_r weights[r] * sum_{i in region r} a[i] * b[i]  (FP64, deterministic). */
+int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                     const double* weights, size_t n_regions, double* out);
+/* y = alpha * x + beta * y over whole payloads. */
+int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta);
+
+/* ---- tile-shape autotuner --------------------------------------------------- */
+/* Force one GEMM tile configuration for tables built after this call
+ * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128; -1: per-job choice from the
+ * tuned table).  Used by the autotuner sweep; no reference counterpart. */
+int tnt_set_tile_config(int cfg);
+/* Install the tuned per-shape table: entries (m_max, n_max, cfg) scanned in
+ * order, first match wins; n = 0 clears it. */
+int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n);
+
+/* ---- hipGraph batching ----------------------------------------------------- */
+int tnt_graph_begin(tnt_ctx* ctx);
+int tnt_graph_end(tnt_ctx* ctx, tnt_graph** out);
+int tnt_graph_launch(tnt_ctx* ctx, const tnt_graph* graph);
+int tnt_graph_destroy(tnt_ctx* ctx, tnt_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENET_HIP_H */

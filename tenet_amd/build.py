@@ -1,0 +1,68 @@
+"""Build the native libraries of tenet_amd in-tree.
+
+  lib/libtenet_host.so  host structure compiler (g++, no GPU code)
+  lib/libtenet_hip.so   gfx950 backend (hipcc, --offload-arch=gfx950)
+
+Usage: python -m tenet_amd.build   (or tenet_amd.build.build())
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/host_capi.cpp"]
+HIP_SRC = ["csrc/hip/backend.hip", "csrc/hip/xform.hip", "csrc/hip/gemm.hip",
+           "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip"]
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+             # no packed FP32 instructions anywhere in device code
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+
+
+def build_host():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libtenet_host.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", out] + HOST_SRC)
+    return out
+
+
+def build_hip(jobs=8):
+    os.makedirs(LIB, exist_ok=True)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    procs = []
+    for src in HIP_SRC:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, cwd=HERE))
+        if len(procs) >= jobs:
+            if procs.pop(0).wait() != 0:
+                raise RuntimeError("hipcc failed")
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed")
+    out = os.path.join(LIB, "libtenet_hip.so")
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out] + objs)
+    return out
+
+
+def build():
+    build_host()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,1064 @@
+// C-ABI of the gfx950 backend (include/tenet_hip.h): contexts, device
+// buffers, plan/artifact upload (compiled layouts -> device work lists),
+// dispatch of the recoupling and GEMM kernels, reductions, hipGraph capture.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/tenet_hip.h"
+#include "kernels.hpp"
+#include "split.hpp"
+
+using namespace tnt_dev;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct TableCacheEntry {
+  void* dev = nullptr;
+  size_t bytes = 0;
+};
+
+struct tnt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::multimap<size_t, void*> free_blocks;  // device pool (size -> ptr)
+  std::unordered_map<void*, size_t> live;    // pool-owned allocations
+  size_t pool_bytes = 0;
+  bool capturing = false;
+  // content-addressed device copies of small per-call tables
+  std::unordered_map<std::string, TableCacheEntry> tables;
+  std::list<std::string> table_lru;
+  // pinned host scalar for reductions
+  double* host_scalar = nullptr;
+  double* dev_scalar = nullptr;
+  std::unique_ptr<SplitWorkspace, SplitWorkspaceDeleter> split;
+};
+
+struct tnt_buf {
+  void* ptr = nullptr;
+  size_t len = 0;
+  int dtype = TNT_F64;
+  int device = 0;
+  bool owned = false;
+};
+
+struct DevArrays {  // device copy of a normalised transform plan
+  XBlock* blocks = nullptr;
+  XLayout* layouts = nullptr;
+  XItem* items = nullptr;
+  double* coeffs = nullptr;
+  uint32_t n_items = 0;
+};
+
+struct tnt_plan {
+  int device = 0;
+  int64_t dst_len = 0, src_len = 0;
+  DevArrays d;
+  std::vector<void*> allocs;
+  double bytes_moved = 0;  // algorithmic bytes of one replay (read src + write dst)
+};
+
+struct GemmTable {
+  GJob* jobs = nullptr;
+  GTile* tiles = nullptr;
+  uint32_t n_tiles_cfg[TILE_CFG_COUNT] = {0, 0, 0, 0};
+  uint32_t tile_start_cfg[TILE_CFG_COUNT] = {0, 0, 0, 0};
+  std::vector<GJob> host_jobs;
+  std::vector<GTile> host_tiles;
+};
+
+struct tnt_artifact {
+  int device = 0;
+  tnt_plan* lhs = nullptr;
+  tnt_plan* rhs = nullptr;
+  tnt_plan* out = nullptr;
+  GemmTable gemm;
+  std::vector<void*> allocs;
+  int64_t lhs_len = 0, rhs_len = 0, core_len = 0;
+  double* scratch_l = nullptr;
+  double* scratch_r = nullptr;
+  double* scratch_c = nullptr;
+  int64_t* unc_off = nullptr;
+  int64_t* unc_len = nullptr;
+  uint32_t n_unc = 0;
+  int lhs_op = 0, rhs_op = 0;
+};
+
+struct tnt_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+#define FAIL(ctx, code, msg)        \
+  do {                              \
+    if (ctx) (ctx)->err = (msg);    \
+    return (code);                  \
+  } while (0)
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      if (ctx) (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);         \
+      return (_e == hipErrorOutOfMemory) ? TNT_E_OOM : TNT_E_HIP;                      \
+    }                                                                                  \
+  } while (0)
+
+static int set_device(tnt_ctx* ctx) {
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return TNT_OK;
+}
+
+// Pooled device allocation: reuse a free block of at least `bytes` and at
+// most 2x that; never frees while a graph is being captured.
+static int pool_alloc(tnt_ctx* ctx, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  bytes = (bytes + 255) & ~size_t(255);
+  auto it = ctx->free_blocks.lower_bound(bytes);
+  if (it != ctx->free_blocks.end() && it->first <= 2 * bytes) {
+    *out = it->second;
+    ctx->live[it->second] = it->first;
+    ctx->free_blocks.erase(it);
+    return TNT_OK;
+  }
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "device allocation requested during graph capture");
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // release cached blocks and retry once
+    for (auto& kv : ctx->free_blocks) {
+      (void)hipFree(kv.second);
+      ctx->pool_bytes -= kv.first;
+    }
+    ctx->free_blocks.clear();
+    (void)hipGetLastError();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      (void)hipGetLastError();
+      return TNT_E_OOM;
+    }
+  }
+  ctx->pool_bytes += bytes;
+  ctx->live[p] = bytes;
+  *out = p;
+  return TNT_OK;
+}
+
+static void pool_free(tnt_ctx* ctx, void* p) {
+  if (!p) return;
+  auto it = ctx->live.find(p);
+  if (it == ctx->live.end()) return;
+  ctx->free_blocks.emplace(it->second, p);
+  ctx->live.erase(it);
+}
+
+template <class T>
+static int upload_vec(tnt_ctx* ctx, const std::vector<T>& v, T** dev, std::vector<void*>& allocs) {
+  void* p = nullptr;
+  int rc = pool_alloc(ctx, sizeof(T) * std::max<size_t>(v.size(), 1), &p);
+  if (rc) return rc;
+  allocs.push_back(p);
+  if (!v.empty()) {
+    // stream-ordered: a recycled pool block may still be read by queued work
+    HIPCHK(ctx, hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  *dev = static_cast<T*>(p);
+  return TNT_OK;
+}
+
+// Content-addressed device copy of a small table (job lists, region lists).
+static int cached_table(tnt_ctx* ctx, const void* data, size_t bytes, const char* tag, void** dev) {
+  std::string key(tag);
+  key.append(static_cast<const char*>(data), bytes);
+  auto it = ctx->tables.find(key);
+  if (it != ctx->tables.end()) {
+    *dev = it->second.dev;
+    return TNT_OK;
+  }
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "new job table requested during graph capture; run the call once before capturing");
+  void* p = nullptr;
+  int rc = pool_alloc(ctx, bytes, &p);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(p, data, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->tables[key] = {p, bytes};
+  ctx->table_lru.push_back(key);
+  if (ctx->table_lru.size() > 512) {  // bounded cache; the stream is drained before reuse
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const std::string& old = ctx->table_lru.front();
+    auto o = ctx->tables.find(old);
+    if (o != ctx->tables.end()) {
+      pool_free(ctx, o->second.dev);
+      ctx->tables.erase(o);
+    }
+    ctx->table_lru.pop_front();
+  }
+  *dev = p;
+  return TNT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Layout normalisation (host, at upload)
+// ---------------------------------------------------------------------------
+struct NLayout {
+  int64_t offset;
+  std::vector<int64_t> dims, strides;
+};
+
+// Common refinement of the column-major shapes of several layouts with equal
+// element counts, then drop extent-1 axes, sort by the first layout's
+// strides, and fuse axes contiguous in every layout.
+static bool normalize(std::vector<NLayout>& ls, std::vector<int64_t>& dims_out, std::string& err) {
+  int64_t count = -1;
+  std::vector<int64_t> bps;  // prefix-product breakpoints
+  for (auto& l : ls) {
+    int64_t p = 1;
+    for (auto d : l.dims) {
+      if (d <= 0) {
+        p = 0;
+        break;
+      }
+      p *= d;
+      bps.push_back(p);
+    }
+    if (count < 0) count = p;
+    if (p != count) {
+      err = "layouts of one transform block differ in element count";
+      return false;
+    }
+  }
+  if (count == 0) {
+    dims_out.clear();
+    return true;
+  }
+  std::sort(bps.begin(), bps.end());
+  bps.erase(std::unique(bps.begin(), bps.end()), bps.end());
+  for (size_t i = 1; i < bps.size(); ++i)
+    if (bps[i] % bps[i - 1]) {
+      err = "layout shapes have no common refinement";
+      return false;
+    }
+  std::vector<int64_t> ref;  // refined dims
+  int64_t prev = 1;
+  for (auto b : bps) {
+    if (b / prev > 1) ref.push_back(b / prev);
+    prev = b;
+  }
+  // express each layout on the refined dims
+  for (auto& l : ls) {
+    std::vector<int64_t> st;
+    size_t r = 0;
+    for (size_t a = 0; a < l.dims.size(); ++a) {
+      int64_t rem = l.dims[a], s = l.strides[a];
+      while (rem > 1) {
+        if (r >= ref.size() || rem % ref[r]) {
+          err = "layout refinement failed";
+          return false;
+        }
+        st.push_back(s);
+        s *= ref[r];
+        rem /= ref[r];
+        ++r;
+      }
+    }
+    l.dims = ref;
+    l.strides = st;
+  }
+  size_t n = ref.size();
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return ls[0].strides[x] < ls[0].strides[y]; });
+  std::vector<int64_t> dims;
+  std::vector<std::vector<int64_t>> strides(ls.size());
+  for (size_t oi = 0; oi < n; ++oi) {
+    size_t a = order[oi];
+    bool fuse = !dims.empty();
+    for (size_t li = 0; fuse && li < ls.size(); ++li)
+      fuse = strides[li].back() * dims.back() == ls[li].strides[a];
+    if (fuse) {
+      dims.back() *= ref[a];
+    } else {
+      dims.push_back(ref[a]);
+      for (size_t li = 0; li < ls.size(); ++li) strides[li].push_back(ls[li].strides[a]);
+    }
+  }
+  if (dims.empty()) {
+    dims.push_back(1);
+    for (auto& s : strides) s.push_back(1);
+  }
+  for (size_t li = 0; li < ls.size(); ++li) ls[li].strides = strides[li];
+  dims_out = dims;
+  return true;
+}
+
+static bool layout_in_bounds(const tnt_layout& l, int64_t len, std::string& err) {
+  if (l.rank < 0 || l.rank > TNT_MAX_RANK) {
+    err = "layout rank out of range";
+    return false;
+  }
+  int64_t hi = l.offset, n = 1;
+  for (int i = 0; i < l.rank; ++i) {
+    if (l.shape[i] < 0 || l.strides[i] < 0) {
+      err = "negative extent or stride";
+      return false;
+    }
+    n *= l.shape[i];
+    if (l.shape[i] > 0) hi += (l.shape[i] - 1) * l.strides[i];
+  }
+  if (n > 0 && (l.offset < 0 || hi >= len)) {
+    err = "layout addresses elements outside its payload";
+    return false;
+  }
+  return true;
+}
+
+static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out) {
+  if (!d) FAIL(ctx, TNT_E_INVALID, "null transform descriptor");
+  std::vector<XBlock> blocks;
+  std::vector<XLayout> layouts;
+  std::vector<XItem> items;
+  std::string err;
+  double bytes = 0;
+  auto add_block = [&](int kind, const std::vector<uint32_t>& dl, const std::vector<uint32_t>& sl,
+                       uint32_t coeff) -> int {
+    std::vector<NLayout> ls;
+    for (auto i : dl) {
+      if (i >= d->n_layouts) FAIL(ctx, TNT_E_OOB, "layout index out of range");
+      const tnt_layout& L = d->layouts[i];
+      if (!layout_in_bounds(L, d->dst_len, err)) FAIL(ctx, TNT_E_OOB, "destination " + err);
+      ls.push_back({L.offset, std::vector<int64_t>(L.shape, L.shape + L.rank),
+                    std::vector<int64_t>(L.strides, L.strides + L.rank)});
+    }
+    for (auto i : sl) {
+      if (i >= d->n_layouts) FAIL(ctx, TNT_E_OOB, "layout index out of range");
+      const tnt_layout& L = d->layouts[i];
+      if (!layout_in_bounds(L, d->src_len, err)) FAIL(ctx, TNT_E_OOB, "source " + err);
+      ls.push_back({L.offset, std::vector<int64_t>(L.shape, L.shape + L.rank),
+                    std::vector<int64_t>(L.strides, L.strides + L.rank)});
+    }
+    std::vector<int64_t> dims;
+    if (!normalize(ls, dims, err)) FAIL(ctx, TNT_E_SHAPE, err);
+    if (dims.empty()) return TNT_OK;  // zero-size block
+    if (dims.size() > (size_t)kMaxRank) FAIL(ctx, TNT_E_UNSUPPORTED, "normalised rank exceeds 8");
+    XBlock b{};
+    b.kind = kind;
+    b.rank = (int32_t)dims.size();
+    b.ndst = (int32_t)dl.size();
+    b.nsrc = (int32_t)sl.size();
+    b.lay = (int32_t)layouts.size();
+    b.coeff = (int32_t)coeff;
+    for (size_t i = 0; i < dims.size(); ++i) b.dims[i] = dims[i];
+    for (auto& l : ls) {
+      XLayout x{};
+      x.offset = l.offset;
+      for (size_t i = 0; i < l.strides.size(); ++i) x.strides[i] = l.strides[i];
+      layouts.push_back(x);
+    }
+    uint32_t bi = (uint32_t)blocks.size();
+    blocks.push_back(b);
+    int64_t plane = dims[0] * (dims.size() > 1 ? dims[1] : 1);
+    int64_t outer = 1;
+    for (size_t i = 2; i < dims.size(); ++i) outer *= dims[i];
+    int64_t elems = plane * outer;
+    bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
+    // ~8K element-updates per work item (Multi weighted by its layouts)
+    int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
+    int64_t target = std::max<int64_t>(256, 8192 / weight);
+    if (plane >= target) {
+      int64_t chunk = ((target + 63) / 64) * 64;
+      for (int64_t o = 0; o < outer; ++o)
+        for (int64_t e = 0; e < plane; e += chunk)
+          items.push_back({bi, 1u, (uint64_t)o, (uint32_t)e, (uint32_t)std::min(chunk, plane - e)});
+    } else {
+      int64_t per = std::max<int64_t>(1, target / plane);
+      for (int64_t o = 0; o < outer; o += per)
+        items.push_back({bi, (uint32_t)std::min(per, outer - o), (uint64_t)o, 0u, (uint32_t)plane});
+    }
+    return TNT_OK;
+  };
+  for (size_t bi = 0; bi < d->n_blocks; ++bi) {
+    const tnt_transform_block& tb = d->blocks[bi];
+    std::vector<uint32_t> dl, sl;
+    for (uint32_t i = 0; i < tb.dst_count; ++i) dl.push_back(tb.dst_layout + i);
+    for (uint32_t i = 0; i < tb.src_count; ++i) sl.push_back(tb.src_layout + i);
+    int rc;
+    if (tb.kind == TNT_BLOCK_SINGLE) {
+      if (tb.dst_count != 1 || tb.src_count != 1) FAIL(ctx, TNT_E_INVALID, "Single block needs one source and one destination");
+      if (tb.coeff >= d->n_coeffs) FAIL(ctx, TNT_E_OOB, "coefficient index out of range");
+      rc = add_block(XK_SINGLE, dl, sl, tb.coeff);
+    } else if (tb.kind == TNT_BLOCK_MULTI) {
+      if (tb.src_count == 0 || tb.dst_count == 0) FAIL(ctx, TNT_E_INVALID, "empty Multi block");
+      if (tb.src_count > (uint32_t)kMaxMultiSrc)
+        FAIL(ctx, TNT_E_UNSUPPORTED, "Multi block with more than 32 source blocks");
+      if ((size_t)tb.coeff + (size_t)tb.src_count * tb.dst_count > d->n_coeffs)
+        FAIL(ctx, TNT_E_OOB, "coefficient range out of bounds");
+      rc = add_block(XK_MULTI, dl, sl, tb.coeff);
+    } else {
+      FAIL(ctx, TNT_E_INVALID, "unknown transform block kind");
+    }
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < d->n_inactive; ++i) {
+    int rc = add_block(XK_SCALE, {d->inactive[i]}, {}, 0);
+    if (rc) return rc;
+  }
+  auto* p = new tnt_plan;
+  p->device = ctx->device;
+  p->dst_len = d->dst_len;
+  p->src_len = d->src_len;
+  p->bytes_moved = bytes;
+  std::vector<double> coeffs(d->coeffs, d->coeffs + d->n_coeffs);
+  int rc = upload_vec(ctx, blocks, &p->d.blocks, p->allocs);
+  if (!rc) rc = upload_vec(ctx, layouts, &p->d.layouts, p->allocs);
+  if (!rc) rc = upload_vec(ctx, items, &p->d.items, p->allocs);
+  if (!rc) rc = upload_vec(ctx, coeffs, &p->d.coeffs, p->allocs);
+  if (rc) {
+    for (auto a : p->allocs) pool_free(ctx, a);
+    delete p;
+    return rc;
+  }
+  p->d.n_items = (uint32_t)items.size();
+  *out = p;
+  return TNT_OK;
+}
+
+static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double* src, double alpha,
+                     double beta, int overwrite) {
+  XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.coeffs, dst, src, alpha, beta, overwrite};
+  launch_xform(a, p->d.n_items, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tables + tile-shape autotuner
+// ---------------------------------------------------------------------------
+// Tile shape per job: measured preference table (filled by tnt_autotune via
+// the environment-independent heuristic below when no measurement exists).
+struct TileRule {
+  int64_t m_max, n_max;
+  int32_t cfg;
+};
+static std::vector<TileRule> tile_table;  // tuned table (tnt_set_tile_table)
+
+static int choose_tile(int64_t m, int64_t n, int64_t k) {
+  (void)k;
+  for (const auto& r : tile_table)
+    if (m <= r.m_max && n <= r.n_max) return r.cfg;
+  if (m >= 128 && n >= 128) return TILE_128x128;
+  if (m >= 128 && n < 128) return TILE_128x64;
+  if (m < 128 && n >= 128) return TILE_64x128;
+  return TILE_64x64;
+}
+
+static int tile_override = -1;  // set by tnt_set_tile_config (autotuner sweeps)
+
+static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs) {
+  t.host_jobs = jobs;
+  std::vector<std::vector<GTile>> per(TILE_CFG_COUNT);
+  // larger jobs first for load balance
+  std::vector<size_t> order(jobs.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    double ca = double(jobs[a].m) * jobs[a].n * jobs[a].k, cb = double(jobs[b].m) * jobs[b].n * jobs[b].k;
+    return ca > cb;
+  });
+  for (size_t oi : order) {
+    const GJob& j = jobs[oi];
+    if (j.m <= 0 || j.n <= 0) continue;
+    int cfg = tile_override >= 0 ? tile_override : choose_tile(j.m, j.n, j.k);
+    int bm = tile_rows(cfg), bn = tile_cols(cfg);
+    int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
+    for (int a = 0; a < tm; ++a)
+      for (int b = 0; b < tn; ++b) per[cfg].push_back({(int32_t)oi, a, b, cfg});
+  }
+  t.host_tiles.clear();
+  for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+    t.tile_start_cfg[c] = (uint32_t)t.host_tiles.size();
+    t.n_tiles_cfg[c] = (uint32_t)per[c].size();
+    t.host_tiles.insert(t.host_tiles.end(), per[c].begin(), per[c].end());
+  }
+}
+
+static int upload_gemm_table(tnt_ctx* ctx, GemmTable& t, std::vector<void*>& allocs) {
+  int rc = upload_vec(ctx, t.host_jobs, &t.jobs, allocs);
+  if (!rc) rc = upload_vec(ctx, t.host_tiles, &t.tiles, allocs);
+  return rc;
+}
+
+static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
+                          double alpha, double beta, int precision) {
+  if (precision == TNT_PREC_SPLIT_BF16) {
+    if (!ctx->split) ctx->split.reset(split_workspace_create());
+    return split_gemm_run(ctx->split.get(), t.host_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
+                          alpha, beta, ctx->stream, ctx->capturing, ctx->err);
+  }
+  for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+    if (!t.n_tiles_cfg[c]) continue;
+    GemmArgs g{t.jobs, t.tiles + t.tile_start_cfg[c], d, a, b, alpha, beta};
+    if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
+    else launch_gemm_f64(g, c, t.n_tiles_cfg[c], ctx->stream);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+static bool job_in_bounds(const GJob& j, size_t dlen, size_t alen, size_t blen) {
+  auto span = [](int64_t off, int64_t rows, int64_t cols, int64_t ld) -> int64_t {
+    if (rows == 0 || cols == 0) return off;
+    return off + (cols - 1) * ld + rows;  // one past last element
+  };
+  bool a_adj = j.ops & 1, b_adj = j.ops & 2;
+  int64_t ar = a_adj ? j.k : j.m, ac = a_adj ? j.m : j.k;
+  int64_t br = b_adj ? j.n : j.k, bc = b_adj ? j.k : j.n;
+  if (j.lda < std::max<int64_t>(1, ar) || j.ldb < std::max<int64_t>(1, br) || j.ldd < std::max<int64_t>(1, j.m))
+    return false;
+  if (j.d_off < 0 || j.a_off < 0 || j.b_off < 0) return false;
+  return span(j.d_off, j.m, j.n, j.ldd) <= (int64_t)dlen && span(j.a_off, ar, ac, j.lda) <= (int64_t)alen &&
+         span(j.b_off, br, bc, j.ldb) <= (int64_t)blen;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tnt_ctx_create(int device, tnt_ctx** out) {
+  if (!out) return TNT_E_INVALID;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return TNT_E_HIP;
+  if (device < 0 || device >= n) return TNT_E_DEVICE_MISMATCH;
+  auto* c = new tnt_ctx;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->host_scalar), sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&c->dev_scalar), sizeof(double)) != hipSuccess) {
+    delete c;
+    return TNT_E_HIP;
+  }
+  *out = c;
+  return TNT_OK;
+}
+
+int tnt_ctx_destroy(tnt_ctx* ctx) {
+  if (!ctx) return TNT_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->split.reset();
+  for (auto& kv : ctx->tables) pool_free(ctx, kv.second.dev);
+  for (auto& kv : ctx->live) (void)hipFree(kv.first);
+  for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
+  (void)hipFree(ctx->dev_scalar);
+  (void)hipHostFree(ctx->host_scalar);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TNT_OK;
+}
+
+const char* tnt_last_error(const tnt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tnt_ctx_synchronize(tnt_ctx* ctx) {
+  if (!ctx) return TNT_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return TNT_OK;
+}
+
+int tnt_ctx_stream(tnt_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return TNT_E_INVALID;
+  *stream = ctx->stream;
+  return TNT_OK;
+}
+
+int tnt_ctx_pool_bytes(const tnt_ctx* ctx, size_t* bytes) {
+  if (!ctx || !bytes) return TNT_E_INVALID;
+  *bytes = ctx->pool_bytes;
+  return TNT_OK;
+}
+
+int tnt_buf_alloc(tnt_ctx* ctx, size_t len, int dtype, tnt_buf** out) {
+  if (!ctx || !out) return TNT_E_INVALID;
+  if (dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, "payload buffers are FP64");
+  if (int rc = set_device(ctx)) return rc;
+  auto* b = new tnt_buf;
+  b->len = len;
+  b->dtype = dtype;
+  b->device = ctx->device;
+  b->owned = true;
+  size_t es = dtype == TNT_F64 ? 8 : 4;
+  int rc = pool_alloc(ctx, len * es, &b->ptr);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return TNT_OK;
+}
+
+int tnt_buf_wrap(tnt_ctx* ctx, void* ptr, size_t len, int dtype, tnt_buf** out) {
+  if (!ctx || !out || (!ptr && len)) return TNT_E_INVALID;
+  if (dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, "payload buffers are FP64");
+  hipPointerAttribute_t attr;
+  if (ptr && hipPointerGetAttributes(&attr, ptr) == hipSuccess) {
+    if (attr.type == hipMemoryTypeDevice && attr.device != ctx->device)
+      FAIL(ctx, TNT_E_DEVICE_MISMATCH, "wrapped pointer lives on another device");
+  }
+  (void)hipGetLastError();
+  auto* b = new tnt_buf;
+  b->ptr = ptr;
+  b->len = len;
+  b->dtype = dtype;
+  b->device = ctx->device;
+  b->owned = false;
+  *out = b;
+  return TNT_OK;
+}
+
+int tnt_buf_free(tnt_ctx* ctx, tnt_buf* buf) {
+  if (!ctx || !buf) return TNT_E_INVALID;
+  if (buf->owned) {
+    if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "buffer freed during graph capture");
+    // stream-ordered reuse: later allocations are used on the same stream
+    pool_free(ctx, buf->ptr);
+  }
+  delete buf;
+  return TNT_OK;
+}
+
+static int check_buf(tnt_ctx* ctx, const tnt_buf* b, const char* what) {
+  if (!b) FAIL(ctx, TNT_E_INVALID, std::string("null buffer: ") + what);
+  if (b->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, std::string(what) + " lives on another device");
+  if (b->dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, std::string(what) + " must be an FP64 payload");
+  return TNT_OK;
+}
+
+int tnt_buf_upload(tnt_ctx* ctx, tnt_buf* buf, const void* host, size_t len) {
+  if (!ctx || !buf || (!host && len)) return TNT_E_INVALID;
+  if (len > buf->len) FAIL(ctx, TNT_E_SHAPE, "upload longer than buffer");
+  if (int rc = set_device(ctx)) return rc;
+  size_t es = buf->dtype == TNT_F64 ? 8 : 4;
+  HIPCHK(ctx, hipMemcpyAsync(buf->ptr, host, len * es, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return TNT_OK;
+}
+
+int tnt_buf_download(tnt_ctx* ctx, const tnt_buf* buf, void* host, size_t len) {
+  if (!ctx || !buf || (!host && len)) return TNT_E_INVALID;
+  if (len > buf->len) FAIL(ctx, TNT_E_SHAPE, "download longer than buffer");
+  if (int rc = set_device(ctx)) return rc;
+  size_t es = buf->dtype == TNT_F64 ? 8 : 4;
+  HIPCHK(ctx, hipMemcpyAsync(host, buf->ptr, len * es, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return TNT_OK;
+}
+
+int tnt_buf_zero(tnt_ctx* ctx, tnt_buf* buf) {
+  if (!ctx || !buf) return TNT_E_INVALID;
+  if (int rc = set_device(ctx)) return rc;
+  launch_fill(static_cast<double*>(buf->ptr), (int64_t)buf->len, 0.0, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+int tnt_buf_fill_random(tnt_ctx* ctx, tnt_buf* buf, uint64_t seed, int normal) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, buf, "buf")) return rc;
+  if (int rc = set_device(ctx)) return rc;
+  if (normal) launch_fill_normal(static_cast<double*>(buf->ptr), (int64_t)buf->len, seed, ctx->stream);
+  else launch_fill_unit(static_cast<double*>(buf->ptr), (int64_t)buf->len, seed, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+int tnt_buf_info(const tnt_buf* buf, void** ptr, size_t* len, int* dtype, int* device) {
+  if (!buf) return TNT_E_INVALID;
+  if (ptr) *ptr = buf->ptr;
+  if (len) *len = buf->len;
+  if (dtype) *dtype = buf->dtype;
+  if (device) *device = buf->device;
+  return TNT_OK;
+}
+
+int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                   const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
+                   int lhs_op, int rhs_op, double alpha, double beta, int precision) {
+  (void)runs;
+  (void)n_runs;
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, dst, "dst")) return rc;
+  if (int rc = check_buf(ctx, lhs, "lhs")) return rc;
+  if (int rc = check_buf(ctx, rhs, "rhs")) return rc;
+  if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
+  if ((lhs_op != TNT_OP_IDENTITY && lhs_op != TNT_OP_ADJOINT) || (rhs_op != TNT_OP_IDENTITY && rhs_op != TNT_OP_ADJOINT))
+    FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  std::vector<GJob> gj(n_jobs);
+  for (size_t i = 0; i < n_jobs; ++i) {
+    const tnt_gemm_job& j = jobs[i];
+    if (j.rows > INT32_MAX || j.cols > INT32_MAX || j.contracted > INT32_MAX)
+      FAIL(ctx, TNT_E_UNSUPPORTED, "job dimension exceeds 2^31");
+    GJob g{};
+    g.d_off = (int64_t)j.dst_offset;
+    g.a_off = (int64_t)j.lhs_offset;
+    g.b_off = (int64_t)j.rhs_offset;
+    g.m = (int32_t)j.rows;
+    g.k = (int32_t)j.contracted;
+    g.n = (int32_t)j.cols;
+    g.lda = lhs_op == TNT_OP_ADJOINT ? std::max(g.k, 1) : std::max(g.m, 1);
+    g.ldb = rhs_op == TNT_OP_ADJOINT ? std::max(g.n, 1) : std::max(g.k, 1);
+    g.ldd = std::max(g.m, 1);
+    g.ops = (lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (rhs_op == TNT_OP_ADJOINT ? 2 : 0);
+    if (!job_in_bounds(g, dst->len, lhs->len, rhs->len)) FAIL(ctx, TNT_E_OOB, "GEMM job addresses elements outside its buffers");
+    gj[i] = g;
+  }
+  if (int rc = set_device(ctx)) return rc;
+  GemmTable t;
+  build_gemm_table(t, gj);
+  void* dj = nullptr;
+  void* dt = nullptr;
+  int rc = cached_table(ctx, t.host_jobs.data(), sizeof(GJob) * std::max<size_t>(t.host_jobs.size(), 1), "J", &dj);
+  if (!rc) rc = cached_table(ctx, t.host_tiles.data(), sizeof(GTile) * std::max<size_t>(t.host_tiles.size(), 1), "T", &dt);
+  if (rc) return rc;
+  t.jobs = static_cast<GJob*>(dj);
+  t.tiles = static_cast<GTile*>(dt);
+  return run_gemm_table(ctx, t, (uintptr_t)dst->ptr, (uintptr_t)lhs->ptr, (uintptr_t)rhs->ptr, alpha, beta, precision);
+}
+
+int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, double alpha,
+                     double beta, int precision) {
+  if (!ctx) return TNT_E_INVALID;
+  if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  std::vector<GJob> gj(n_jobs);
+  for (size_t i = 0; i < n_jobs; ++i) {
+    const tnt_gemm_ptr_job& j = jobs[i];
+    if (!j.dst || !j.lhs || !j.rhs || j.rows < 0 || j.cols < 0 || j.contracted < 0)
+      FAIL(ctx, TNT_E_INVALID, "invalid grouped job");
+    if (((uintptr_t)j.dst | (uintptr_t)j.lhs | (uintptr_t)j.rhs) & 7)
+      FAIL(ctx, TNT_E_INVALID, "grouped job pointers must be 8-byte aligned");
+    GJob g{};
+    g.d_off = (int64_t)((uintptr_t)j.dst / 8);
+    g.a_off = (int64_t)((uintptr_t)j.lhs / 8);
+    g.b_off = (int64_t)((uintptr_t)j.rhs / 8);
+    g.m = (int32_t)j.rows;
+    g.k = (int32_t)j.contracted;
+    g.n = (int32_t)j.cols;
+    g.lda = (int32_t)j.ldl;
+    g.ldb = (int32_t)j.ldr;
+    g.ldd = (int32_t)j.ldd;
+    g.ops = (j.lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (j.rhs_op == TNT_OP_ADJOINT ? 2 : 0);
+    int64_t ar = (g.ops & 1) ? g.k : g.m, br = (g.ops & 2) ? g.n : g.k;
+    if (g.lda < std::max<int64_t>(1, ar) || g.ldb < std::max<int64_t>(1, br) || g.ldd < std::max<int32_t>(1, g.m))
+      FAIL(ctx, TNT_E_SHAPE, "leading dimension smaller than the row count");
+    gj[i] = g;
+  }
+  if (int rc = set_device(ctx)) return rc;
+  GemmTable t;
+  build_gemm_table(t, gj);
+  void* dj = nullptr;
+  void* dt = nullptr;
+  int rc = cached_table(ctx, t.host_jobs.data(), sizeof(GJob) * std::max<size_t>(t.host_jobs.size(), 1), "PJ", &dj);
+  if (!rc) rc = cached_table(ctx, t.host_tiles.data(), sizeof(GTile) * std::max<size_t>(t.host_tiles.size(), 1), "PT", &dt);
+  if (rc) return rc;
+  t.jobs = static_cast<GJob*>(dj);
+  t.tiles = static_cast<GTile*>(dt);
+  return run_gemm_table(ctx, t, 0, 0, 0, alpha, beta, precision);
+}
+
+int tnt_plan_upload(tnt_ctx* ctx, const tnt_transform_desc* desc, tnt_plan** out) {
+  if (!ctx || !out) return TNT_E_INVALID;
+  if (int rc = set_device(ctx)) return rc;
+  return build_plan(ctx, desc, out);
+}
+
+int tnt_plan_destroy(tnt_ctx* ctx, tnt_plan* plan) {
+  if (!ctx || !plan) return TNT_E_INVALID;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto a : plan->allocs) pool_free(ctx, a);
+  delete plan;
+  return TNT_OK;
+}
+
+int tnt_tree_transform(tnt_ctx* ctx, const tnt_plan* plan, tnt_buf* dst, const tnt_buf* src,
+                       double alpha, double beta, int overwrite) {
+  if (!ctx || !plan) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, dst, "dst")) return rc;
+  if (int rc = check_buf(ctx, src, "src")) return rc;
+  if (plan->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, "plan uploaded to another device");
+  if ((int64_t)dst->len < plan->dst_len) FAIL(ctx, TNT_E_SHAPE, "destination shorter than the plan's destination payload");
+  if ((int64_t)src->len < plan->src_len) FAIL(ctx, TNT_E_SHAPE, "source shorter than the plan's source payload");
+  if (dst->ptr == src->ptr && plan->dst_len > 0) FAIL(ctx, TNT_E_INVALID, "in-place tree transform is not supported");
+  if (int rc = set_device(ctx)) return rc;
+  run_plan(ctx, plan, static_cast<double*>(dst->ptr), static_cast<const double*>(src->ptr), alpha, beta, overwrite);
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact** out) {
+  if (!ctx || !d || !out) return TNT_E_INVALID;
+  if (int rc = set_device(ctx)) return rc;
+  auto* a = new tnt_artifact;
+  a->device = ctx->device;
+  a->lhs_len = d->lhs_len;
+  a->rhs_len = d->rhs_len;
+  a->core_len = d->core_len;
+  a->lhs_op = d->lhs_op;
+  a->rhs_op = d->rhs_op;
+  auto fail = [&](int rc) {
+    tnt_contract_destroy(ctx, a);
+    return rc;
+  };
+  if (d->lhs) {
+    if (int rc = build_plan(ctx, d->lhs, &a->lhs)) return fail(rc);
+    if (a->lhs->dst_len != d->lhs_len) return fail((ctx->err = "lhs transform length mismatch", TNT_E_SHAPE));
+  }
+  if (d->rhs) {
+    if (int rc = build_plan(ctx, d->rhs, &a->rhs)) return fail(rc);
+    if (a->rhs->dst_len != d->rhs_len) return fail((ctx->err = "rhs transform length mismatch", TNT_E_SHAPE));
+  }
+  if (d->out) {
+    if (int rc = build_plan(ctx, d->out, &a->out)) return fail(rc);
+    if (a->out->src_len != d->core_len) return fail((ctx->err = "output transform length mismatch", TNT_E_SHAPE));
+  }
+  std::vector<GJob> gj(d->n_jobs);
+  for (size_t i = 0; i < d->n_jobs; ++i) {
+    const tnt_gemm_job& j = d->jobs[i];
+    GJob g{};
+    g.d_off = (int64_t)j.dst_offset;
+    g.a_off = (int64_t)j.lhs_offset;
+    g.b_off = (int64_t)j.rhs_offset;
+    g.m = (int32_t)j.rows;
+    g.k = (int32_t)j.contracted;
+    g.n = (int32_t)j.cols;
+    g.lda = d->lhs_op == TNT_OP_ADJOINT ? std::max(g.k, 1) : std::max(g.m, 1);
+    g.ldb = d->rhs_op == TNT_OP_ADJOINT ? std::max(g.n, 1) : std::max(g.k, 1);
+    g.ldd = std::max(g.m, 1);
+    g.ops = (d->lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (d->rhs_op == TNT_OP_ADJOINT ? 2 : 0);
+    if (!job_in_bounds(g, d->core_len, d->lhs_len, d->rhs_len)) {
+      ctx->err = "core GEMM job addresses elements outside its operands";
+      return fail(TNT_E_OOB);
+    }
+    gj[i] = g;
+  }
+  build_gemm_table(a->gemm, gj);
+  if (int rc = upload_gemm_table(ctx, a->gemm, a->allocs)) return fail(rc);
+  std::vector<int64_t> uo, ul;
+  for (size_t i = 0; i < d->n_uncovered; ++i) {
+    if (d->uncovered[i].offset < 0 || d->uncovered[i].offset + d->uncovered[i].len > d->core_len) {
+      ctx->err = "uncovered region out of range";
+      return fail(TNT_E_OOB);
+    }
+    uo.push_back(d->uncovered[i].offset);
+    ul.push_back(d->uncovered[i].len);
+  }
+  a->n_unc = (uint32_t)uo.size();
+  if (int rc = upload_vec(ctx, uo, &a->unc_off, a->allocs)) return fail(rc);
+  if (int rc = upload_vec(ctx, ul, &a->unc_len, a->allocs)) return fail(rc);
+  void* p = nullptr;
+  if (a->lhs) {
+    if (int rc = pool_alloc(ctx, 8 * (size_t)std::max<int64_t>(a->lhs_len, 1), &p)) return fail(rc);
+    a->allocs.push_back(p);
+    a->scratch_l = static_cast<double*>(p);
+  }
+  if (a->rhs) {
+    if (int rc = pool_alloc(ctx, 8 * (size_t)std::max<int64_t>(a->rhs_len, 1), &p)) return fail(rc);
+    a->allocs.push_back(p);
+    a->scratch_r = static_cast<double*>(p);
+  }
+  if (a->out) {
+    if (int rc = pool_alloc(ctx, 8 * (size_t)std::max<int64_t>(a->core_len, 1), &p)) return fail(rc);
+    a->allocs.push_back(p);
+    a->scratch_c = static_cast<double*>(p);
+  }
+  *out = a;
+  return TNT_OK;
+}
+
+int tnt_contract_destroy(tnt_ctx* ctx, tnt_artifact* a) {
+  if (!ctx || !a) return TNT_E_INVALID;
+  (void)hipStreamSynchronize(ctx->stream);
+  if (a->lhs) tnt_plan_destroy(ctx, a->lhs);
+  if (a->rhs) tnt_plan_destroy(ctx, a->rhs);
+  if (a->out) tnt_plan_destroy(ctx, a->out);
+  for (auto p : a->allocs) pool_free(ctx, p);
+  delete a;
+  return TNT_OK;
+}
+
+int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_buf* lhs,
+                 const tnt_buf* rhs, double alpha, double beta, int precision) {
+  if (!ctx || !a) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, dst, "dst")) return rc;
+  if (int rc = check_buf(ctx, lhs, "lhs")) return rc;
+  if (int rc = check_buf(ctx, rhs, "rhs")) return rc;
+  if (a->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, "artifact uploaded to another device");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  int64_t need_l = a->lhs ? a->lhs->src_len : a->lhs_len;
+  int64_t need_r = a->rhs ? a->rhs->src_len : a->rhs_len;
+  int64_t need_d = a->out ? a->out->dst_len : a->core_len;
+  if ((int64_t)lhs->len < need_l || (int64_t)rhs->len < need_r || (int64_t)dst->len < need_d)
+    FAIL(ctx, TNT_E_SHAPE, "operand payload shorter than the artifact expects");
+  if (dst->ptr == lhs->ptr || dst->ptr == rhs->ptr) FAIL(ctx, TNT_E_INVALID, "destination aliases an operand");
+  if (int rc = set_device(ctx)) return rc;
+  const double* L = static_cast<const double*>(lhs->ptr);
+  const double* R = static_cast<const double*>(rhs->ptr);
+  if (a->lhs) {
+    run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
+    L = a->scratch_l;
+  }
+  if (a->rhs) {
+    run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
+    R = a->scratch_r;
+  }
+  int rc;
+  if (a->out) {
+    launch_scale_regions(a->scratch_c, a->unc_off, a->unc_len, a->n_unc, 0.0, ctx->stream);
+    rc = run_gemm_table(ctx, a->gemm, (uintptr_t)a->scratch_c, (uintptr_t)L, (uintptr_t)R, 1.0, 0.0, precision);
+    if (rc) return rc;
+    run_plan(ctx, a->out, static_cast<double*>(dst->ptr), a->scratch_c, alpha, beta, 0);
+  } else {
+    launch_scale_regions(static_cast<double*>(dst->ptr), a->unc_off, a->unc_len, a->n_unc, beta, ctx->stream);
+    rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                     const double* weights, size_t n_regions, double* out) {
+  if (!ctx || !out) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, a, "a")) return rc;
+  if (int rc = check_buf(ctx, b, "b")) return rc;
+  if (n_regions && (!regions || !weights)) FAIL(ctx, TNT_E_INVALID, "null region list");
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "tnt_weighted_dot returns a host value and cannot be captured");
+  std::vector<int64_t> cb, ce;
+  std::vector<int32_t> cr;
+  std::vector<double> w(weights, weights + n_regions);
+  const int64_t chunk = 1 << 16;
+  for (size_t r = 0; r < n_regions; ++r) {
+    const tnt_region& g = regions[r];
+    if (g.offset < 0 || g.len < 0 || (size_t)(g.offset + g.len) > a->len || (size_t)(g.offset + g.len) > b->len)
+      FAIL(ctx, TNT_E_OOB, "region outside the payloads");
+    for (int64_t s = 0; s < g.len; s += chunk) {
+      cb.push_back(g.offset + s);
+      ce.push_back(g.offset + std::min(g.len, s + chunk));
+      cr.push_back((int32_t)r);
+    }
+  }
+  if (int rc = set_device(ctx)) return rc;
+  if (cb.empty()) {
+    *out = 0.0;
+    return TNT_OK;
+  }
+  // one packed table: begins | ends | partials, regions and weights separately
+  void *dcb, *dce, *dcr, *dw;
+  int rc = cached_table(ctx, cb.data(), 8 * cb.size(), "DB", &dcb);
+  if (!rc) rc = cached_table(ctx, ce.data(), 8 * ce.size(), "DE", &dce);
+  if (!rc) rc = cached_table(ctx, cr.data(), 4 * cr.size(), "DR", &dcr);
+  if (!rc) rc = cached_table(ctx, w.data(), 8 * w.size(), "DW", &dw);
+  if (rc) return rc;
+  void* parts = nullptr;
+  if ((rc = pool_alloc(ctx, 8 * cb.size(), &parts))) return rc;
+  launch_weighted_dot_partials(static_cast<const double*>(a->ptr), static_cast<const double*>(b->ptr),
+                               static_cast<int64_t*>(dcb), static_cast<int64_t*>(dce),
+                               static_cast<int32_t*>(dcr), static_cast<double*>(dw),
+                               static_cast<double*>(parts), (uint32_t)cb.size(), ctx->stream);
+  launch_sum_partials(static_cast<double*>(parts), (uint32_t)cb.size(), ctx->dev_scalar, ctx->stream);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->host_scalar, ctx->dev_scalar, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  pool_free(ctx, parts);
+  *out = *ctx->host_scalar;
+  return TNT_OK;
+}
+
+int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, y, "y")) return rc;
+  if (int rc = check_buf(ctx, x, "x")) return rc;
+  if (x->len != y->len) FAIL(ctx, TNT_E_SHAPE, "axpby payload lengths differ");
+  if (int rc = set_device(ctx)) return rc;
+  launch_axpby(static_cast<double*>(y->ptr), static_cast<const double*>(x->ptr), (int64_t)y->len, alpha, beta, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+int tnt_graph_begin(tnt_ctx* ctx) {
+  if (!ctx) return TNT_E_INVALID;
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "graph capture already active");
+  if (int rc = set_device(ctx)) return rc;
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  return TNT_OK;
+}
+
+int tnt_graph_end(tnt_ctx* ctx, tnt_graph** out) {
+  if (!ctx || !out) return TNT_E_INVALID;
+  if (!ctx->capturing) FAIL(ctx, TNT_E_INVALID, "no graph capture active");
+  ctx->capturing = false;
+  auto* g = new tnt_graph;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+  if (e != hipSuccess) {
+    delete g;
+    ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+    return TNT_E_HIP;
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+    return TNT_E_HIP;
+  }
+  *out = g;
+  return TNT_OK;
+}
+
+int tnt_graph_launch(tnt_ctx* ctx, const tnt_graph* g) {
+  if (!ctx || !g) return TNT_E_INVALID;
+  if (int rc = set_device(ctx)) return rc;
+  HIPCHK(ctx, hipGraphLaunch(g->exec, ctx->stream));
+  return TNT_OK;
+}
+
+int tnt_graph_destroy(tnt_ctx* ctx, tnt_graph* g) {
+  if (!ctx || !g) return TNT_E_INVALID;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return TNT_OK;
+}
+
+// Tile-shape override used by the autotuner sweep (-1 restores the table).
+int tnt_set_tile_config(int cfg) {
+  if (cfg < -1 || cfg >= TILE_CFG_COUNT) return TNT_E_INVALID;
+  tile_override = cfg;
+  return TNT_OK;
+}
+
+int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n) {
+  if (n && (!m_max || !n_max || !cfg)) return TNT_E_INVALID;
+  std::vector<TileRule> t;
+  for (size_t i = 0; i < n; ++i) {
+    if (cfg[i] < 0 || cfg[i] >= TILE_CFG_COUNT) return TNT_E_INVALID;
+    t.push_back({m_max[i], n_max[i], cfg[i]});
+  }
+  tile_table.swap(t);
+  return TNT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// Grouped block GEMM for gfx950 on the matrix cores.
+//
+// One launch covers every (job, tile) of a batch of uneven coupled-sector
+// GEMMs (the reference issues one GEMM per sector:
+// tenet-operations/src/fusion_replay.rs:627-734, tenet-dense/src/executor.rs:217-312).
+// Column-major operands; op = identity or adjoint (transpose) per operand.
+//
+//   FP64: v_mfma_f64_16x16x4f64, FP64 accumulate.
+//   FP32: operands rounded to FP32 while staging into LDS,
+//         v_mfma_f32_16x16x4f32, FP32 accumulate, FP64 epilogue.
+//
+// Workgroup = 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
+// (BM/2)x(BN/2) sub-tile made of 16x16 MFMA tiles.  K advances in steps of
+// BK=16 through LDS with a register prefetch of the next K slab, so global
+// loads of slab k+1 overlap the MFMAs of slab k.
+//
+// MFMA 16x16x4 lane mapping (one value per lane for A and B, four for D):
+//   A[i][k]: i = lane % 16, k = lane / 16
+//   B[k][j]: j = lane % 16, k = lane / 16
+//   D[i][j]: j = lane % 16, i = 4 * (lane / 16) + r,  r = 0..3
+#include "kernels.hpp"
+
+namespace tnt_dev {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaOp;
+
+template <>
+struct MfmaOp<double> {
+  using acc_t = f64x4;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct MfmaOp<float> {
+  using acc_t = f32x4;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BK = 16;
+
+// T = LDS/MFMA element type; global operands are always FP64.
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int MT = WM / 16, NT = WN / 16;
+  constexpr int PA = 2, PB = 2;  // LDS row padding (elements)
+  constexpr int A_LOADS = BM * BK / 256, B_LOADS = BN * BK / 256;
+  __shared__ T As[BK][BM + PA];
+  __shared__ T Bs[BK][BN + PB];
+
+  const GTile tile = g.tiles[blockIdx.x];
+  const GJob job = g.jobs[tile.job];
+  const int m = job.m, n = job.n, k = job.k;
+  const int m0 = tile.tm * BM, n0 = tile.tn * BN;
+  const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
+  const double* A = reinterpret_cast<const double*>(g.abase) + job.a_off;
+  const double* B = reinterpret_cast<const double*>(g.bbase) + job.b_off;
+  double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const int64_t lda = job.lda, ldb = job.ldb, ldd = job.ldd;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+
+  using acc_t = typename MfmaOp<T>::acc_t;
+  acc_t acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  double ra[A_LOADS], rb[B_LOADS];
+
+  // Global -> registers for the K slab starting at k0.
+  auto load = [&](int k0) {
+    if (!a_adj) {  // A[i][kk] = A[i + kk*lda]; contiguous along i
+#pragma unroll
+      for (int r = 0; r < A_LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int i = e % BM, kk = e / BM;
+        const int gi = m0 + i, gk = k0 + kk;
+        ra[r] = (gi < m && gk < k) ? A[gi + (int64_t)gk * lda] : 0.0;
+      }
+    } else {  // op(A)[i][kk] = X[kk + i*lda]; contiguous along kk
+#pragma unroll
+      for (int r = 0; r < A_LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int kk = e % BK, i = e / BK;
+        const int gi = m0 + i, gk = k0 + kk;
+        ra[r] = (gi < m && gk < k) ? A[gk + (int64_t)gi * lda] : 0.0;
+      }
+    }
+    if (!b_adj) {  // B[kk][j] = B[kk + j*ldb]; contiguous along kk
+#pragma unroll
+      for (int r = 0; r < B_LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int kk = e % BK, j = e / BK;
+        const int gj = n0 + j, gk = k0 + kk;
+        rb[r] = (gj < n && gk < k) ? B[gk + (int64_t)gj * ldb] : 0.0;
+      }
+    } else {  // op(B)[kk][j] = Y[j + kk*ldb]; contiguous along j
+#pragma unroll
+      for (int r = 0; r < B_LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int j = e % BN, kk = e / BN;
+        const int gj = n0 + j, gk = k0 + kk;
+        rb[r] = (gj < n && gk < k) ? B[gj + (int64_t)gk * ldb] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < A_LOADS; ++r) {
+      const int e = tid + 256 * r;
+      if (!a_adj) As[e / BM][e % BM] = (T)ra[r];
+      else As[e % BK][e / BK] = (T)ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_LOADS; ++r) {
+      const int e = tid + 256 * r;
+      if (!b_adj) Bs[e % BK][e / BK] = (T)rb[r];
+      else Bs[e / BN][e % BN] = (T)rb[r];
+    }
+  };
+
+  const int nk = (k + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store();
+  }
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    if (t + 1 < nk) load((t + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kk = ks * 4 + (lane >> 4);
+      T af[MT], bf[NT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) af[i] = As[kk][wm + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bf[j] = Bs[kk][wn + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+    }
+    __syncthreads();
+    if (t + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  // Epilogue: D = alpha * acc + beta * D  (D not read when beta == 0).
+  const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+  for (int i = 0; i < MT; ++i) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int col = n0 + wn + j * 16 + (lane & 15);
+      if (col >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+        if (row < m) {
+          double* p = D + row + (int64_t)col * ldd;
+          const double v = alpha * (double)acc[i][j][r];
+          *p = (beta != 0.0) ? v + beta * *p : v;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  if (n_tiles == 0) return;
+  switch (cfg) {
+    case TILE_128x128:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_128x64:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 64>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_64x128:
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 128>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+  }
+}
+
+void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<double>(a, cfg, n_tiles, s);
+}
+
+void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<float>(a, cfg, n_tiles, s);
+}
+
+}  // namespace tnt_dev

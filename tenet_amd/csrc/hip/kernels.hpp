@@ -1,0 +1,119 @@
+// Internal device-side formats and kernel launchers of the gfx950 backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tnt_dev {
+
+constexpr int kMaxRank = 8;
+constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in registers
+
+enum XKind : int32_t { XK_SINGLE = 0, XK_MULTI = 1, XK_SCALE = 2 };
+
+// One normalised transform block: dims in destination order (extent-1 axes
+// dropped, axes sorted by destination stride, contiguous runs fused).
+struct XBlock {
+  int32_t kind;
+  int32_t rank;     // >= 1; dims[0] is the destination-fastest axis
+  int32_t ndst;     // destination layouts (Single: 1)
+  int32_t nsrc;     // source layouts (Single: 1, Scale: 0)
+  int32_t lay;      // first layout in the layout table: ndst destination layouts, then nsrc source
+  int32_t coeff;    // index of the coefficient (Single) or of row-major U[dst, src] (Multi)
+  int64_t dims[kMaxRank];
+};
+
+struct XLayout {
+  int64_t offset;
+  int64_t strides[kMaxRank];
+};
+
+// A work item: plane elements [e_begin, e_begin + e_count) (plane = dims 0
+// and 1) for outer indices [o_begin, o_begin + o_count) over dims 2..rank-1.
+struct XItem {
+  uint32_t block;
+  uint32_t o_count;
+  uint64_t o_begin;
+  uint32_t e_begin;
+  uint32_t e_count;
+};
+
+struct XformArgs {
+  const XBlock* blocks;
+  const XLayout* layouts;
+  const XItem* items;
+  const double* coeffs;
+  double* dst;
+  const double* src;
+  double alpha;
+  double beta;
+  int overwrite;
+};
+
+void launch_xform(const XformArgs& a, uint32_t n_items, hipStream_t s);
+
+// Device GEMM job: element offsets from the kernel's base pointers.
+struct GJob {
+  int64_t d_off, a_off, b_off;
+  int32_t m, k, n;
+  int32_t lda, ldb, ldd;
+  int32_t ops;  // bit0: lhs adjoint, bit1: rhs adjoint
+  int32_t pad;
+};
+
+struct GTile {
+  int32_t job;
+  int32_t tm;  // tile row index
+  int32_t tn;  // tile column index
+  int32_t cfg; // tile-shape configuration of this tile's job
+};
+
+struct GemmArgs {
+  const GJob* jobs;
+  const GTile* tiles;
+  uintptr_t dbase, abase, bbase;  // byte addresses; job offsets are in elements
+  double alpha, beta;
+};
+
+// Tile shapes (rows x cols per workgroup) available to the autotuner.
+enum TileCfg : int32_t { TILE_128x128 = 0, TILE_64x64 = 1, TILE_128x64 = 2, TILE_64x128 = 3, TILE_CFG_COUNT = 4 };
+inline int tile_rows(int cfg) { return (cfg == TILE_128x128 || cfg == TILE_128x64) ? 128 : 64; }
+inline int tile_cols(int cfg) { return (cfg == TILE_128x128 || cfg == TILE_64x128) ? 128 : 64; }
+
+// Launch one grouped GEMM over the tile table.  All tiles of one launch must
+// share `cfg`; the caller splits the table by configuration.
+void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+
+// Split-BF16 (Ozaki) GEMM: operands are pre-split into `slices` BF16 planes
+// with per-row (lhs) / per-column (rhs) power-of-two scales.
+struct SplitArgs {
+  const GJob* jobs;
+  const GTile* tiles;
+  uintptr_t dbase;
+  const uint16_t* a_planes;  // [slices][a_elems] bf16, per job packed row-major tiles (see gemm_split.hip)
+  const uint16_t* b_planes;
+  const int32_t* a_exp;      // per-row exponents
+  const int32_t* b_exp;      // per-column exponents
+  const int64_t* a_base;     // per job: start of its rows in a_planes / a_exp
+  const int64_t* b_base;
+  int64_t a_plane_stride, b_plane_stride;
+  int slices;
+  double alpha, beta;
+};
+
+// Reductions / elementwise
+void launch_weighted_dot_partials(const double* a, const double* b, const int64_t* chunk_begin,
+                                  const int64_t* chunk_end, const int32_t* chunk_region,
+                                  const double* weights, double* partials, uint32_t n_chunks,
+                                  hipStream_t s);
+void launch_sum_partials(const double* partials, uint32_t n, double* out, hipStream_t s);
+void launch_axpby(double* y, const double* x, int64_t n, double alpha, double beta, hipStream_t s);
+void launch_fill(double* y, int64_t n, double value, hipStream_t s);
+void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s);
+void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s);
+void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,
+                          double beta, hipStream_t s);
+
+}  // namespace tnt_dev

@@ -1,0 +1,149 @@
+// FP64 reductions and elementwise helpers.
+//
+// tnt_weighted_dot restates weighted_inner_cuda (tenet/src/typed.rs:11210-11262):
+// <A,B> = sum_c dim(c) * sum_ij A_c[ij] B_c[ij].  The reference computes one
+// partial per sector on the device and weights them on the host; here both
+// passes run on the device with a fixed chunking and a fixed tree order, so
+// the result is bitwise reproducible for a given region list (the property
+// the serving loop relies on: a request's inner products do not depend on
+// what else is batched with it).
+#include "kernels.hpp"
+
+namespace tnt_dev {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void wdot_partials(const double* __restrict__ a,
+                                                      const double* __restrict__ b,
+                                                      const int64_t* __restrict__ cb,
+                                                      const int64_t* __restrict__ ce,
+                                                      const int32_t* __restrict__ cr,
+                                                      const double* __restrict__ w,
+                                                      double* __restrict__ partials) {
+  __shared__ double red[4];
+  const int64_t beg = cb[blockIdx.x], end = ce[blockIdx.x];
+  double s = 0.0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += 256) s = fma(a[i], b[i], s);
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = w[cr[blockIdx.x]] * ((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+__global__ __launch_bounds__(256) void sum_partials(const double* __restrict__ p, uint32_t n,
+                                                     double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) s += p[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_weighted_dot_partials(const double* a, const double* b, const int64_t* cb,
+                                  const int64_t* ce, const int32_t* cr, const double* w,
+                                  double* partials, uint32_t n_chunks, hipStream_t s) {
+  if (n_chunks == 0) return;
+  hipLaunchKernelGGL(wdot_partials, dim3(n_chunks), dim3(256), 0, s, a, b, cb, ce, cr, w, partials);
+}
+
+void launch_sum_partials(const double* partials, uint32_t n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, s, partials, n, out);
+}
+
+__global__ __launch_bounds__(256) void axpby_kernel(double* __restrict__ y,
+                                                     const double* __restrict__ x, int64_t n,
+                                                     double alpha, double beta) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    y[i] = (beta != 0.0) ? alpha * x[i] + beta * y[i] : alpha * x[i];
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(double* __restrict__ y, int64_t n, double v) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) y[i] = v;
+}
+
+__global__ __launch_bounds__(256) void scale_regions_kernel(double* __restrict__ y,
+                                                             const int64_t* __restrict__ off,
+                                                             const int64_t* __restrict__ len,
+                                                             double beta) {
+  const int64_t o = off[blockIdx.y], l = len[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < l; i += (int64_t)gridDim.x * 256)
+    y[o + i] = (beta != 0.0) ? beta * y[o + i] : 0.0;
+}
+
+static uint32_t grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (uint32_t)g;
+}
+
+void launch_axpby(double* y, const double* x, int64_t n, double alpha, double beta, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, x, n, alpha, beta);
+}
+
+void launch_fill(double* y, int64_t n, double value, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n, value);
+}
+
+void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,
+                          double beta, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(scale_regions_kernel, dim3(64, n), dim3(256), 0, s, y, off, len, beta);
+}
+
+// Seeded fill of the reference (tenet/src/typed.rs:2086-2092): element k of
+// the flat payload receives draw k+1 of splitmix64 started at `seed`,
+// mapped to ((z >> 11) / 2^52) - 1 in [-1, 1).  Every step is exact in FP64,
+// so the device fill is bitwise identical to the host one.
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void fill_unit_kernel(double* __restrict__ y, int64_t n,
+                                                         uint64_t seed) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    y[i] = (double)(splitmix_at(seed, (uint64_t)i) >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// normal(0,1) by Box-Muller on consecutive pairs of the same stream: element
+// 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0,1].
+__global__ __launch_bounds__(256) void fill_normal_kernel(double* __restrict__ y, int64_t n,
+                                                           uint64_t seed) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  const int64_t pairs = (n + 1) / 2;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < pairs; p += stride) {
+    double u1 = ((double)(splitmix_at(seed, 2 * p) >> 11) + 1.0) * 0x1.0p-53;
+    double u2 = ((double)(splitmix_at(seed, 2 * p + 1) >> 11) + 1.0) * 0x1.0p-53;
+    double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    y[2 * p] = r * c;
+    if (2 * p + 1 < n) y[2 * p + 1] = r * s;
+  }
+}
+
+void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_unit_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n, seed);
+}
+
+void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_normal_kernel, dim3(grid_for((n + 1) / 2)), dim3(256), 0, s, y, n, seed);
+}
+
+}  // namespace tnt_dev

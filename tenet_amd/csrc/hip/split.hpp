@@ -1,0 +1,26 @@
+// Split-BF16 (Ozaki-style) GEMM path: declarations shared with backend.hip.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace tnt_dev {
+
+struct SplitWorkspace;
+
+// Runs dst = alpha * A B + beta * dst for every job of the table with the
+// split-BF16 scheme.  Returns a TNT status; `err` receives the message.
+int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
+                   const std::vector<GTile>& host_tiles, const GTile* dev_tiles, uintptr_t d,
+                   uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s,
+                   bool capturing, std::string& err);
+SplitWorkspace* split_workspace_create();
+void split_workspace_destroy(SplitWorkspace* ws);
+
+struct SplitWorkspaceDeleter {
+  void operator()(SplitWorkspace* w) const { split_workspace_destroy(w); }
+};
+
+}  // namespace tnt_dev

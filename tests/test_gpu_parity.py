@@ -1,0 +1,339 @@
+"""GPU parity of the gfx950 backend against the CPU oracle (through the C ABI).
+
+Bars: integer/index/layout work and U(1) permutations bit-exact; FP64 GEMM
+results within 1e-13 relative Frobenius error of the FP64 oracle (accumulation
+order differs from numpy's); FP32 GEMM within 1e-5.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from descs import spaces
+from oracle.fusion import HomSpace as OHomSpace, Leg as OLeg
+from oracle.symmetry import make_rule
+from oracle.tensor import OTensor, splitmix_unit
+from tenet_amd import _native as N
+from tenet_amd.tensor import GradedSpace, HomSpace, Rule, Runtime, TensorMap
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+KAT = json.load(open(os.path.join(HERE, "golden", "kat.json")))
+GOLD = json.load(open(os.path.join(HERE, "golden", "tensorkit_oracle.json")))
+
+
+@pytest.fixture(scope="module")
+def rt():
+    r = Runtime(0)
+    yield r
+    r.close()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb else 1.0)
+
+
+def _buf(rt, arr):
+    b = C.c_void_p()
+    rt._check(N.hip().tnt_buf_alloc(rt._ctx, max(1, len(arr)), 0, C.byref(b)))
+    if len(arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        rt._check(N.hip().tnt_buf_upload(rt._ctx, b, a.ctypes.data, len(a)))
+    return b
+
+
+def _get(rt, b, n):
+    out = np.empty(n)
+    rt._check(N.hip().tnt_buf_download(rt._ctx, b, out.ctypes.data, n))
+    return out
+
+
+def _free(rt, *bs):
+    for b in bs:
+        N.hip().tnt_buf_free(rt._ctx, b)
+
+
+# ---------------------------------------------------------------------------
+# GEMM seam (DenseExecutor::matmul_batch_axpby_with_ops_into)
+# ---------------------------------------------------------------------------
+GEMM_SHAPES = [(16, 4, 16), (1, 1, 1), (17, 5, 33), (64, 64, 64), (130, 70, 129), (300, 17, 5),
+               (5, 300, 7), (256, 256, 256), (129, 1000, 65)]
+
+
+@pytest.mark.parametrize("lop,rop", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
+    rng = np.random.default_rng(7 + 2 * lop + rop)
+    jobs, lhs, rhs, dst_parts, want = [], [], [], [], []
+    lo = ro = do = 0
+    for m, k, n in GEMM_SHAPES:
+        A = rng.standard_normal((m, k))
+        B = rng.standard_normal((k, n))
+        D0 = rng.standard_normal((m, n))
+        sa = A.T if lop else A  # stored matrix
+        sb = B.T if rop else B
+        lhs.append(sa.reshape(-1, order="F"))
+        rhs.append(sb.reshape(-1, order="F"))
+        dst_parts.append(D0.reshape(-1, order="F"))
+        want.append((0.5 * A @ B + 2.0 * D0).reshape(-1, order="F"))
+        jobs.append(N.GemmJob(do, lo, ro, m, k, n))
+        lo += m * k
+        ro += k * n
+        do += m * n
+    L, R, D = (np.concatenate(x) for x in (lhs, rhs, dst_parts))
+    bl, br, bd = _buf(rt, L), _buf(rt, R), _buf(rt, D)
+    arr = (N.GemmJob * len(jobs))(*jobs)
+    rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, arr, len(jobs), None, 0, lop, rop, 0.5, 2.0, prec))
+    got = _get(rt, bd, len(D))
+    _free(rt, bl, br, bd)
+    w = np.concatenate(want)
+    tol = 1e-13 if prec == 0 else 1e-5
+    o = 0
+    for (m, k, n), part in zip(GEMM_SHAPES, want):
+        assert rel(got[o:o + m * n], part) < tol, ((m, k, n), lop, rop, prec)
+        o += m * n
+    assert rel(got, w) < tol
+
+
+def test_gemm_batch_rejects_out_of_bounds_before_mutating(rt):
+    D = np.arange(16.0)
+    bd, bl, br = _buf(rt, D), _buf(rt, np.ones(16)), _buf(rt, np.ones(16))
+    jobs = (N.GemmJob * 2)(N.GemmJob(0, 0, 0, 2, 2, 2), N.GemmJob(10, 0, 0, 4, 2, 2))
+    rc = N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 2, None, 0, 0, 0, 1.0, 0.0, 0)
+    assert rc == -4
+    assert np.array_equal(_get(rt, bd, 16), D)
+    _free(rt, bd, bl, br)
+
+
+def test_gemm_grouped_pointer_jobs(rt):
+    rng = np.random.default_rng(3)
+    tensors = []
+    jobs = []
+    wants = []
+    for m, k, n in [(70, 33, 90), (128, 64, 128), (5, 200, 9), (257, 129, 66)]:
+        A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+        ba, bb, bd = _buf(rt, A.reshape(-1, order="F")), _buf(rt, B.reshape(-1, order="F")), _buf(rt, np.zeros(m * n))
+        pa, pb, pd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        for b, p in ((ba, pa), (bb, pb), (bd, pd)):
+            N.hip().tnt_buf_info(b, C.byref(p), None, None, None)
+        jobs.append(N.GemmPtrJob(pd.value, pa.value, pb.value, m, k, n, m, m, k, 0, 0))
+        tensors.append((ba, bb, bd, m, n))
+        wants.append(A @ B)
+    arr = (N.GemmPtrJob * len(jobs))(*jobs)
+    rt._check(N.hip().tnt_gemm_grouped(rt._ctx, arr, len(jobs), 1.0, 0.0, 0))
+    for (ba, bb, bd, m, n), w in zip(tensors, wants):
+        got = _get(rt, bd, m * n).reshape(n, m).T
+        assert rel(got, w) < 1e-13
+        _free(rt, ba, bb, bd)
+
+
+# ---------------------------------------------------------------------------
+# Replay known-answer tests through tnt_plan_upload / tnt_tree_transform
+# ---------------------------------------------------------------------------
+def _desc_from(dst_blocks, src_blocks, multis, dst_len, src_len):
+    from oracle.replay import packed_column_major
+    dl = packed_column_major(dst_blocks)
+    sl = packed_column_major(src_blocks)
+    layouts, blocks, coeffs = [], [], []
+
+    def lay(off, shape, strides):
+        L = N.Layout()
+        L.offset, L.rank = off, len(shape)
+        for i, (s, t) in enumerate(zip(shape, strides)):
+            L.shape[i], L.strides[i] = s, t
+        return L
+    for dsts, srcs, U in multis:
+        b = N.TransformBlock(1, len(layouts), len(dsts), 0, len(srcs), len(coeffs))
+        for d in dsts:
+            layouts.append(lay(*dl[d]))
+        b.src_layout = len(layouts)
+        for s in srcs:
+            # source layouts are read in destination axis order: reshape to the
+            # destination block's shape (packed column = column-major index)
+            layouts.append(lay(*sl[s]))
+        coeffs.extend(U)
+        blocks.append(b)
+    keep = (layouts, blocks, coeffs)
+    desc = N.TransformDesc()
+    desc.dst_len, desc.src_len = dst_len, src_len
+    desc.n_layouts = len(layouts)
+    desc.layouts = (N.Layout * len(layouts))(*layouts)
+    desc.n_blocks = len(blocks)
+    desc.blocks = (N.TransformBlock * len(blocks))(*blocks)
+    desc.n_coeffs = len(coeffs)
+    desc.coeffs = (C.c_double * len(coeffs))(*coeffs)
+    desc.n_inactive = 0
+    return desc, keep
+
+
+@pytest.mark.parametrize("name", ["multi_2x2", "multi_orientation", "multi_sorted_offsets"])
+def test_replay_kats_exact(rt, name):
+    k = KAT[name]
+    if name == "multi_sorted_offsets":
+        dstb = srcb = k["blocks"]
+        multis = [(m["dst_idx"], m["src_idx"], m["U_dst_src"]) for m in k["multis"]]
+        n = len(k["src"])
+        dlen = slen = n
+    else:
+        dstb, srcb = k["dst_blocks"], k["src_blocks"]
+        multis = [(k["dst_idx"], k["src_idx"], k["U_dst_src"])]
+        dlen = sum(int(np.prod(s)) for s in dstb)
+        slen = len(k["src"])
+    desc, keep = _desc_from(dstb, srcb, multis, dlen, slen)
+    plan = C.c_void_p()
+    rt._check(N.hip().tnt_plan_upload(rt._ctx, C.byref(desc), C.byref(plan)))
+    bs = _buf(rt, np.array(k["src"], dtype=float))
+    bd = _buf(rt, np.full(dlen, k["fill"], dtype=float))
+    ow = 1 if k["beta"] == 0.0 else 0
+    rt._check(N.hip().tnt_tree_transform(rt._ctx, plan, bd, bs, k["alpha"], k["beta"], ow))
+    got = _get(rt, bd, dlen)
+    N.hip().tnt_plan_destroy(rt._ctx, plan)
+    _free(rt, bs, bd)
+    assert list(got) == k["expected"]
+
+
+# ---------------------------------------------------------------------------
+# Layout-level parity: seeded fill, permute, contraction
+# ---------------------------------------------------------------------------
+U1V = [(-2, 3), (-1, 5), (0, 7), (1, 5), (2, 3)]
+SU2V = [(0, 3), (1, 2), (2, 2), (3, 1)]
+
+
+def test_seeded_fill_bitwise(rt):
+    V = GradedSpace(Rule("U1"), U1V)
+    t = TensorMap.rand(rt, [V, V], [V, V], seed=123456789)
+    assert np.array_equal(t.to_numpy(), splitmix_unit(123456789, t.space.len))
+
+
+@pytest.mark.parametrize("rule_name,pairs", [("U1", U1V), ("SU2", SU2V),
+                                             ("U1xZ2", [((0, 0), 3), ((1, 1), 2), ((-1, 1), 2), ((2, 0), 1)])])
+@pytest.mark.parametrize("p1,p2", [([1, 0], [2, 3]), ([0, 2], [1, 3]), ([2, 0, 1], [3]), ([3], [0, 1, 2]),
+                                   ([1, 3], [2, 0]), ([3, 2, 1, 0], [])])
+def test_permute_matches_oracle(rt, rule_name, pairs, p1, p2):
+    ps, os_ = spaces(rule_name, [(pairs, False), (pairs, True)], [(pairs, False), (pairs, False)])
+    t = TensorMap.rand(rt, ps, seed=11)
+    src = t.to_numpy()
+    got = t.permute(p1, p2).to_numpy()
+    want = OTensor(os_, src).permute(p1, p2).data
+    if rule_name == "SU2":
+        assert np.abs(got - want).max() <= 1e-14 * max(1.0, np.abs(want).max())
+    else:
+        assert np.array_equal(got, want), "abelian permutation must be bit-exact"
+
+
+def test_permute_axpby_into_existing(rt):
+    ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, False)], [(SU2V, False), (SU2V, False)])
+    t = TensorMap.rand(rt, ps, seed=5)
+    out = t.permute([0, 2], [1, 3])
+    before = out.to_numpy()
+    t.permute([0, 2], [1, 3], out=out, alpha=2.0, beta=-0.5)
+    want = 2.0 * OTensor(os_, t.to_numpy()).permute([0, 2], [1, 3]).data - 0.5 * before
+    assert np.abs(out.to_numpy() - want).max() < 1e-13
+
+
+CONTRACTS = [([2, 3], [0, 1], [0, 1, 2, 3], 2), ([3, 2], [0, 1], [0, 1, 2, 3], 2),
+             ([3, 2], [0, 1], [1, 0, 2, 3], 2), ([1, 3], [2, 0], [0, 2, 1, 3], 2),
+             ([0], [2], [0, 1, 2, 3, 4, 5], 3)]
+
+
+@pytest.mark.parametrize("rule_name,pairs", [("U1", U1V), ("SU2", SU2V)])
+@pytest.mark.parametrize("ci", range(len(CONTRACTS)))
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
+    axa, axb, perm, nout = CONTRACTS[ci]
+    ps, os_ = spaces(rule_name, [(pairs, False), (pairs, False)], [(pairs, False), (pairs, False)])
+    A = TensorMap.rand(rt, ps, seed=1)
+    B = TensorMap.rand(rt, ps, seed=2)
+    got = A.contract(axa, B, axb, perm, nout, precision=prec).to_numpy()
+    want = OTensor(os_, A.to_numpy()).contract(axa, OTensor(os_, B.to_numpy()), axb, perm, nout).data
+    assert rel(got, want) < (1e-13 if prec == "f64" else 2e-6), (rule_name, ci, prec)
+
+
+def _oracle_fill_fn(rule, c0):
+    def fn(unc, coupled, idx):
+        lab = [rule.label(s) for s in unc] + [rule.label(coupled)]
+        l1, l2, m1, m2, lc = lab
+        w = (c0 + 7 * l1 + 11 * l2 + 13 * m1 + 17 * m2 + 19 * lc + 23 * (idx[0] + 1)
+             + 29 * (idx[1] + 1) + 31 * (idx[2] + 1) + 37 * (idx[3] + 1))
+        return float(w % 41 - 20)
+    return fn
+
+
+@pytest.mark.parametrize("name,pairs", [("U1", [(-1, 2), (0, 3), (1, 2)]), ("SU2", [(0, 2), (1, 2), (2, 1)])])
+def test_tensorkit_invariant_stream_on_gpu(rt, name, pairs):
+    """benchmarks/tensorkit_semantic_oracle.out section 3 (semantic_suite.rs:839-1030)."""
+    rule = Rule(name)
+    V = GradedSpace(rule, pairs)
+    A = TensorMap.from_block_fn(rt, [V, V], [V, V], _oracle_fill_fn(rule, 3))
+    B = TensorMap.from_block_fn(rt, [V, V], [V, V], _oracle_fill_fn(rule, 5))
+    Cm = A.compose(B)
+    D = A.permute([1, 0], [3, 2])
+    E = D.compose(Cm)
+    G = A.adjoint().compose(A)
+    H = E.add(A, 1.0, 0.5)
+    got = {n: (t.norm(), t.tr()) for n, t in [("s1a", A), ("s1b", B), ("s2", Cm), ("s3", D), ("s4", E),
+                                               ("s5", G), ("s7", H)]}
+    s8 = H.compose(H).tr()
+    want = GOLD["stream"][name]
+    for step, (nv, tv) in got.items():
+        assert abs(nv - want[step][0]) <= 1e-12 * abs(want[step][0]), step
+        assert abs(tv - want[step][1]) <= 1e-12 * max(1.0, abs(want[step][1])), step
+    assert abs(s8 - want["s8"]) <= 1e-12 * abs(want["s8"])
+
+
+@pytest.mark.parametrize("d", [4, 16])
+def test_su2_microbench_checksums_on_gpu(rt, d):
+    """docs/su2_authority.md:81-83 on the workloads of microbench_fusion.rs:24-50."""
+    rule = Rule("SU2")
+    V = GradedSpace(rule, [(0, d), (1, d), (2, d)])
+    sp = HomSpace(rule, [V, V], [V, V])
+    n = sp.len
+    A = TensorMap.from_numpy(rt, sp, (np.arange(n) % 17) * 0.25 - 2.0)
+    B = TensorMap.from_numpy(rt, sp, (np.arange(n) % 13) * 0.5 - 3.0)
+    want = KAT["microbench_checksums"]["SU2"][str(d)]
+    for (la, lb, perm), w in zip([([2, 3], [0, 1], [0, 1, 2, 3]), ([3, 2], [0, 1], [0, 1, 2, 3]),
+                                  ([3, 2], [0, 1], [1, 0, 2, 3])], want):
+        out = A.contract(la, B, lb, perm, 2).to_numpy()
+        assert abs(out.sum() - w) <= 1e-13 * np.abs(out).sum() + 1e-12
+
+
+def test_quickstart_inner_50(rt):
+    rule = Rule("U1")
+    V = GradedSpace(rule, [(-1, 1), (0, 2), (1, 1)])
+    a = TensorMap.from_block_fn(rt, [V], [V], lambda u, c, i: float(2 + rule.label(c)) if i[0] == i[1] else 0.0)
+    b = TensorMap.from_block_fn(rt, [V], [V], lambda u, c, i: float(2 - rule.label(c)) if i[0] == i[1] else 0.0)
+    c = a.contract([1], b, [0])
+    assert c.inner(c) == 50.0
+
+
+def test_weighted_inner_matches_oracle(rt):
+    ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, False)], [(SU2V, False), (SU2V, False)])
+    A, B = TensorMap.rand(rt, ps, seed=3), TensorMap.rand(rt, ps, seed=4)
+    want = OTensor(os_, A.to_numpy()).inner(OTensor(os_, B.to_numpy()))
+    got = A.inner(B)
+    assert abs(got - want) <= 1e-12 * abs(want)
+    assert A.inner(B) == got  # deterministic reduction order
+
+
+def test_graph_capture_replays_match_eager(rt):
+    ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, False)], [(SU2V, False), (SU2V, False)])
+    A, B = TensorMap.rand(rt, ps, seed=8), TensorMap.rand(rt, ps, seed=9)
+    eager = A.contract([1, 3], B, [2, 0], [0, 2, 1, 3], 2)
+    ref = eager.to_numpy()
+    out = TensorMap.zeros(rt, eager.space)
+    h = N.hip()
+    rt._check(h.tnt_graph_begin(rt._ctx))
+    A.contract([1, 3], B, [2, 0], [0, 2, 1, 3], 2, out=out)
+    g = C.c_void_p()
+    rt._check(h.tnt_graph_end(rt._ctx, C.byref(g)))
+    for rep in range(4):
+        out.scale_(0.0)
+        rt.synchronize()
+        rt._check(h.tnt_graph_launch(rt._ctx, g))
+        rt.synchronize()
+        assert np.array_equal(out.to_numpy(), ref), rep
+    h.tnt_graph_destroy(rt._ctx, g)
