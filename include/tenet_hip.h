@@ -231,6 +231,16 @@ int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt
 /* y = alpha * x + beta * y over whole payloads. */
 int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta);
 
+/* ---- kernel timing ---------------------------------------------------------- */
+/* When on, every recoupling replay (kind 0) and block-GEMM launch (kind 1 + precision)
+ * is bracketed by HIP events on the context stream; stats report launches, total
+ * milliseconds and total algorithmic work (bytes for kind 0, flops otherwise).
+ * No reference counterpart (the reference's profiles are host wall-clock,
+ * tenet-operations/src/profile.rs:16-59). */
+int tnt_ctx_set_timing(tnt_ctx* ctx, int on);
+int tnt_ctx_kernel_stats(tnt_ctx* ctx, int kind, size_t* launches, double* total_ms, double* total_work);
+int tnt_ctx_reset_stats(tnt_ctx* ctx);
+
 /* ---- tile-shape autotuner --------------------------------------------------- */
 /* Force one GEMM tile configuration for tables built after this call
  * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128; -1: per-job choice from the

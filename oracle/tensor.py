@@ -27,6 +27,29 @@ def splitmix_unit(seed, n):
     return (z >> np.uint64(11)).astype(np.float64) / float(1 << 52) - 1.0
 
 
+def splitmix_normal(seed, n):
+    """normal(0,1) by Box-Muller on consecutive draw pairs of the same stream.
+
+    Elements 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0, 1]:
+    r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).  (The reference has no
+    normal fill; this is the build's definition for the synthetic workloads.)
+    """
+    m = 2 * ((n + 1) // 2)
+    k = np.arange(1, m + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK64) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = ((z >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0 ** -53
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(m)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    return out[:n]
+
+
 class OTensor:
     def __init__(self, space: HomSpace, data=None):
         self.space = space
@@ -44,6 +67,10 @@ class OTensor:
     @classmethod
     def rand(cls, space, seed):
         return cls(space, splitmix_unit(seed, space.required_len))
+
+    @classmethod
+    def randn(cls, space, seed):
+        return cls(space, splitmix_normal(seed, space.required_len))
 
     @classmethod
     def from_block_fn(cls, space, fn):

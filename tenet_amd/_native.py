@@ -102,6 +102,9 @@ HIP_API = {
     "tnt_weighted_dot": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ,
                                    C.POINTER(C.c_double)]),
     "tnt_axpby": (C.c_int, [VP, VP, VP, C.c_double, C.c_double]),
+    "tnt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
+    "tnt_ctx_kernel_stats": (C.c_int, [VP, C.c_int, SZP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tnt_ctx_reset_stats": (C.c_int, [VP]),
     "tnt_set_tile_config": (C.c_int, [C.c_int]),
     "tnt_set_tile_table": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32), SZ]),

@@ -44,6 +44,47 @@ struct tnt_ctx {
   double* host_scalar = nullptr;
   double* dev_scalar = nullptr;
   std::unique_ptr<SplitWorkspace, SplitWorkspaceDeleter> split;
+  // optional per-kernel-class event timing (tnt_ctx_set_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  struct Timed {
+    int kind;
+    hipEvent_t a, b;
+    double work;  // flops (GEMM) or bytes (recoupling)
+  };
+  std::vector<Timed> timed;
+};
+
+static hipEvent_t take_event(tnt_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Brackets one kernel-class launch with events on the context stream.
+struct TimedScope {
+  tnt_ctx* ctx;
+  int kind;
+  double work;
+  hipEvent_t a = nullptr;
+  TimedScope(tnt_ctx* c, int k, double w) : ctx(c), kind(k), work(w) {
+    if (ctx->timing && !ctx->capturing) {
+      a = take_event(ctx);
+      (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~TimedScope() {
+    if (a) {
+      hipEvent_t b = take_event(ctx);
+      (void)hipEventRecord(b, ctx->stream);
+      ctx->timed.push_back({kind, a, b, work});
+    }
+  }
 };
 
 struct tnt_buf {
@@ -440,6 +481,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
 static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double* src, double alpha,
                      double beta, int overwrite) {
   XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.coeffs, dst, src, alpha, beta, overwrite};
+  TimedScope ts(ctx, 0, p->bytes_moved);
   launch_xform(a, p->d.n_items, ctx->stream);
 }
 
@@ -499,8 +541,15 @@ static int upload_gemm_table(tnt_ctx* ctx, GemmTable& t, std::vector<void*>& all
   return rc;
 }
 
+static double table_flops(const GemmTable& t) {
+  double f = 0;
+  for (const auto& j : t.host_jobs) f += 2.0 * double(j.m) * double(j.n) * double(j.k);
+  return f;
+}
+
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
                           double alpha, double beta, int precision) {
+  TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
   if (precision == TNT_PREC_SPLIT_BF16) {
     if (!ctx->split) ctx->split.reset(split_workspace_create());
     return split_gemm_run(ctx->split.get(), t.host_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
@@ -562,6 +611,11 @@ int tnt_ctx_destroy(tnt_ctx* ctx) {
   for (auto& kv : ctx->tables) pool_free(ctx, kv.second.dev);
   for (auto& kv : ctx->live) (void)hipFree(kv.first);
   for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
+  for (auto& t : ctx->timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   (void)hipFree(ctx->dev_scalar);
   (void)hipHostFree(ctx->host_scalar);
   (void)hipStreamDestroy(ctx->stream);
@@ -1040,6 +1094,43 @@ int tnt_graph_destroy(tnt_ctx* ctx, tnt_graph* g) {
   (void)hipGraphExecDestroy(g->exec);
   (void)hipGraphDestroy(g->graph);
   delete g;
+  return TNT_OK;
+}
+
+int tnt_ctx_set_timing(tnt_ctx* ctx, int on) {
+  if (!ctx) return TNT_E_INVALID;
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "timing cannot change during graph capture");
+  ctx->timing = on != 0;
+  return TNT_OK;
+}
+
+int tnt_ctx_kernel_stats(tnt_ctx* ctx, int kind, size_t* launches, double* total_ms, double* total_work) {
+  if (!ctx || kind < 0 || kind > 3) return TNT_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  size_t n = 0;
+  double ms = 0, w = 0;
+  for (const auto& t : ctx->timed) {
+    if (t.kind != kind) continue;
+    float e = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&e, t.a, t.b));
+    ++n;
+    ms += e;
+    w += t.work;
+  }
+  if (launches) *launches = n;
+  if (total_ms) *total_ms = ms;
+  if (total_work) *total_work = w;
+  return TNT_OK;
+}
+
+int tnt_ctx_reset_stats(tnt_ctx* ctx) {
+  if (!ctx) return TNT_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& t : ctx->timed) {
+    ctx->event_pool.push_back(t.a);
+    ctx->event_pool.push_back(t.b);
+  }
+  ctx->timed.clear();
   return TNT_OK;
 }
 

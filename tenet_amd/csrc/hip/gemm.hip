@@ -14,10 +14,13 @@
 // BK=16 through LDS with a register prefetch of the next K slab, so global
 // loads of slab k+1 overlap the MFMAs of slab k.
 //
-// MFMA 16x16x4 lane mapping (one value per lane for A and B, four for D):
+// MFMA 16x16x4 lane mapping (one value per lane for A and B, four for D),
+// established on gfx950 with a one-hot probe of each builtin:
 //   A[i][k]: i = lane % 16, k = lane / 16
 //   B[k][j]: j = lane % 16, k = lane / 16
-//   D[i][j]: j = lane % 16, i = 4 * (lane / 16) + r,  r = 0..3
+//   D[i][j]: j = lane % 16, and for register r = 0..3
+//            FP64 (v_mfma_f64_16x16x4f64):  i = lane / 16 + 4 * r
+//            FP32 (v_mfma_f32_16x16x4f32):  i = 4 * (lane / 16) + r
 #include "kernels.hpp"
 
 namespace tnt_dev {
@@ -31,6 +34,7 @@ struct MfmaOp;
 template <>
 struct MfmaOp<double> {
   using acc_t = f64x4;
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -39,6 +43,7 @@ struct MfmaOp<double> {
 template <>
 struct MfmaOp<float> {
   using acc_t = f32x4;
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       if (col >= n) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+        const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
         if (row < m) {
           double* p = D + row + (int64_t)col * ldd;
           const double v = alpha * (double)acc[i][j][r];
