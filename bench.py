@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--sweep-tiles", action="store_true",
+                    help="time the workload with every forced GEMM tile shape (autotuner input)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,6 +130,23 @@ def main():
         wl.run()
     rt.synchronize()
     h = N.hip()
+    if args.sweep_tiles:
+        sweep = {}
+        for cfg in (-1, 0, 1, 2, 3):
+            h.tnt_set_tile_config(cfg)
+            rt.clear_caches()  # tile tables are built at artifact upload
+            wl.run()
+            rt.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                wl.run()
+            rt.synchronize()
+            sweep[cfg] = (time.perf_counter() - t) * 1e3 / args.steps
+            log(f"[bench] tile cfg {cfg}: {sweep[cfg]:.3f} ms/step")
+        h.tnt_set_tile_config(-1)
+        rt.clear_caches()
+        wl.run()
+        rt.synchronize()
     rt._check(h.tnt_ctx_reset_stats(rt._ctx))
     rt._check(h.tnt_ctx_set_timing(rt._ctx, 1))
     rt.synchronize()

@@ -114,8 +114,8 @@ struct tnt_plan {
 struct GemmTable {
   GJob* jobs = nullptr;
   GTile* tiles = nullptr;
-  uint32_t n_tiles_cfg[TILE_CFG_COUNT] = {0, 0, 0, 0};
-  uint32_t tile_start_cfg[TILE_CFG_COUNT] = {0, 0, 0, 0};
+  uint32_t n_tiles_cfg[TILE_CFG_COUNT] = {};
+  uint32_t tile_start_cfg[TILE_CFG_COUNT] = {};
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
 };
@@ -498,6 +498,7 @@ static std::vector<TileRule> tile_table;  // tuned table (tnt_set_tile_table)
 
 static int choose_tile(int64_t m, int64_t n, int64_t k) {
   (void)k;
+  if (n <= 16) return TILE_SKINNY;
   for (const auto& r : tile_table)
     if (m <= r.m_max && n <= r.n_max) return r.cfg;
   if (m >= 128 && n >= 128) return TILE_128x128;
@@ -521,7 +522,9 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs) {
   for (size_t oi : order) {
     const GJob& j = jobs[oi];
     if (j.m <= 0 || j.n <= 0) continue;
-    int cfg = tile_override >= 0 ? tile_override : choose_tile(j.m, j.n, j.k);
+    int cfg = choose_tile(j.m, j.n, j.k);
+    if (tile_override >= 0 && !(cfg == TILE_SKINNY && tile_override != TILE_SKINNY)) cfg = tile_override;
+    if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
     int bm = tile_rows(cfg), bn = tile_cols(cfg);
     int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
     for (int a = 0; a < tm; ++a)

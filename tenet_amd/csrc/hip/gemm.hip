@@ -58,8 +58,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   constexpr int MT = WM / 16, NT = WN / 16;
   constexpr int PA = 2, PB = 2;  // LDS row padding (elements)
   constexpr int A_LOADS = BM * BK / 256, B_LOADS = BN * BK / 256;
-  __shared__ T As[BK][BM + PA];
-  __shared__ T Bs[BK][BN + PB];
+  __shared__ T As[2][BK][BM + PA];
+  __shared__ T Bs[2][BK][BN + PB];
 
   const GTile tile = g.tiles[blockIdx.x];
   const GJob job = g.jobs[tile.job];
@@ -121,47 +121,48 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       }
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < A_LOADS; ++r) {
       const int e = tid + 256 * r;
-      if (!a_adj) As[e / BM][e % BM] = (T)ra[r];
-      else As[e % BK][e / BK] = (T)ra[r];
+      if (!a_adj) As[buf][e / BM][e % BM] = (T)ra[r];
+      else As[buf][e % BK][e / BK] = (T)ra[r];
     }
 #pragma unroll
     for (int r = 0; r < B_LOADS; ++r) {
       const int e = tid + 256 * r;
-      if (!b_adj) Bs[e % BK][e / BK] = (T)rb[r];
-      else Bs[e / BN][e % BN] = (T)rb[r];
+      if (!b_adj) Bs[buf][e % BK][e / BK] = (T)rb[r];
+      else Bs[buf][e / BN][e % BN] = (T)rb[r];
     }
   };
 
+  // Double-buffered LDS: slab t is read from buffer t&1 while slab t+1 is
+  // fetched into registers and then written to the other buffer; one
+  // barrier per slab.
   const int nk = (k + BK - 1) / BK;
   if (nk > 0) {
     load(0);
-    store();
+    store(0);
   }
   __syncthreads();
   for (int t = 0; t < nk; ++t) {
+    const int cur = t & 1;
     if (t + 1 < nk) load((t + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
       T af[MT], bf[NT];
 #pragma unroll
-      for (int i = 0; i < MT; ++i) af[i] = As[kk][wm + i * 16 + (lane & 15)];
+      for (int i = 0; i < MT; ++i) af[i] = As[cur][kk][wm + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bf[j] = Bs[kk][wn + j * 16 + (lane & 15)];
+      for (int j = 0; j < NT; ++j) bf[j] = Bs[cur][kk][wn + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < MT; ++i)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
     }
+    if (t + 1 < nk) store(cur ^ 1);
     __syncthreads();
-    if (t + 1 < nk) {
-      store();
-      __syncthreads();
-    }
   }
 
   // Epilogue: D = alpha * acc + beta * D  (D not read when beta == 0).
@@ -185,10 +186,63 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   }
 }
 
+// Narrow-output GEMM (n <= 16): memory-bound, so no matrix cores -- one
+// thread per output row, op(B) staged in LDS, FP64 FMA.  Reads op(A) once
+// (coalesced along rows for identity A) and writes D once.
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
+  constexpr int NMAX = 16, KC = 256;
+  __shared__ double Bsm[KC][NMAX];
+  const GTile tile = g.tiles[blockIdx.x];
+  const GJob job = g.jobs[tile.job];
+  const int m = job.m, n = job.n, k = job.k;
+  const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
+  const double* A = reinterpret_cast<const double*>(g.abase) + job.a_off;
+  const double* B = reinterpret_cast<const double*>(g.bbase) + job.b_off;
+  double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const int i = tile.tm * 256 + threadIdx.x;
+  double acc[NMAX];
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
+  for (int k0 = 0; k0 < k; k0 += KC) {
+    const int kc = min(KC, k - k0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kc * NMAX; e += 256) {
+      const int kk = e / NMAX, j = e % NMAX;
+      double v = 0.0;
+      if (j < n) v = b_adj ? B[j + (int64_t)(k0 + kk) * job.ldb] : B[(k0 + kk) + (int64_t)j * job.ldb];
+      Bsm[kk][j] = (double)(T)v;
+    }
+    __syncthreads();
+    if (i < m) {
+      for (int kk = 0; kk < kc; ++kk) {
+        const int gk = k0 + kk;
+        const double a = (double)(T)(a_adj ? A[gk + (int64_t)i * job.lda] : A[i + (int64_t)gk * job.lda]);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bsm[kk][j], acc[j]);
+      }
+    }
+  }
+  if (i < m) {
+    const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      if (j < n) {
+        double* p = D + i + (int64_t)j * job.ldd;
+        const double v = alpha * acc[j];
+        *p = (beta != 0.0) ? v + beta * *p : v;
+      }
+    }
+  }
+}
+
 template <typename T>
 static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
   if (n_tiles == 0) return;
   switch (cfg) {
+    case TILE_SKINNY:
+      hipLaunchKernelGGL((gemm_skinny_kernel<T>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
     case TILE_128x128:
       hipLaunchKernelGGL((gemm_kernel<T, 128, 128>), dim3(n_tiles), dim3(256), 0, s, a);
       break;

@@ -77,9 +77,22 @@ struct GemmArgs {
 };
 
 // Tile shapes (rows x cols per workgroup) available to the autotuner.
-enum TileCfg : int32_t { TILE_128x128 = 0, TILE_64x64 = 1, TILE_128x64 = 2, TILE_64x128 = 3, TILE_CFG_COUNT = 4 };
-inline int tile_rows(int cfg) { return (cfg == TILE_128x128 || cfg == TILE_128x64) ? 128 : 64; }
-inline int tile_cols(int cfg) { return (cfg == TILE_128x128 || cfg == TILE_64x128) ? 128 : 64; }
+// TILE_SKINNY: narrow outputs (n <= 16), 256 rows x all columns per workgroup
+// on the vector units.
+enum TileCfg : int32_t {
+  TILE_128x128 = 0,
+  TILE_64x64 = 1,
+  TILE_128x64 = 2,
+  TILE_64x128 = 3,
+  TILE_SKINNY = 4,
+  TILE_CFG_COUNT = 5
+};
+inline int tile_rows(int cfg) {
+  return cfg == TILE_SKINNY ? 256 : (cfg == TILE_128x128 || cfg == TILE_128x64) ? 128 : 64;
+}
+inline int tile_cols(int cfg) {
+  return cfg == TILE_SKINNY ? 16 : (cfg == TILE_128x128 || cfg == TILE_64x128) ? 128 : 64;
+}
 
 // Launch one grouped GEMM over the tile table.  All tiles of one launch must
 // share `cfg`; the caller splits the table by configuration.

@@ -21,6 +21,8 @@ namespace tnt_dev {
 
 static_assert(sizeof(XItem) == 24, "XItem layout");
 
+constexpr int kUnroll = 4;
+
 __device__ __forceinline__ int64_t base_offset(const XLayout& L, const int64_t* idx, int rank) {
   int64_t off = L.offset;
 #pragma unroll
@@ -84,13 +86,32 @@ __global__ __launch_bounds__(256) void xform_kernel(XformArgs a) {
       const int64_t d0 = Ld.strides[0], d1 = rank > 1 ? Ld.strides[1] : 0;
       const int64_t s0 = Ls.strides[0], s1 = rank > 1 ? Ls.strides[1] : 0;
       const double c = alpha * a.coeffs[b.coeff];
-      for (; e < e_hi; e += 64) {
-        const double v = a.src[os + x * s0 + y * s1];
-        double* p = a.dst + od + x * d0 + y * d1;
-        *p = readd ? c * v + beta * *p : c * v;
-        x += dx;
-        y += dy;
-        if (x >= n0) { x -= n0; y += 1; }
+      const double* __restrict__ src = a.src;
+      double* __restrict__ dst = a.dst;
+      // four independent loads in flight per lane before their stores
+      while (e < e_hi) {
+        double v[kUnroll];
+        int64_t dp[kUnroll];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (e < e_hi) {
+            v[u] = src[os + x * s0 + y * s1];
+            dp[u] = od + x * d0 + y * d1;
+            cnt = u + 1;
+            e += 64;
+            x += dx;
+            y += dy;
+            if (x >= n0) { x -= n0; y += 1; }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (u < cnt) {
+            double* p = dst + dp[u];
+            *p = readd ? c * v[u] + beta * *p : c * v[u];
+          }
+        }
       }
     } else if (kind == XK_MULTI) {
       const int nd = b.ndst, ns = b.nsrc;
