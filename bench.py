@@ -91,7 +91,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="u1_env", choices=["u1_env", "su2_env", "su2_fmove"])
+    ap.add_argument("--workload", default="u1_env",
+                    choices=["u1_env", "su2_env", "su2_fmove", "copy_calibration"])
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "split_bf16"])
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
@@ -110,6 +111,28 @@ def main():
     from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract
 
     rt = Runtime(0)
+    if args.workload == "copy_calibration":
+        # y = x over 2^27 doubles (1 GiB read + 1 GiB written per launch): the
+        # reference point that scales FETCH_SIZE / WRITE_SIZE to bytes
+        from tenet_amd.tensor import GradedSpace, HomSpace, Rule, TensorMap
+        rule = Rule("U1")
+        V = GradedSpace(rule, [(0, 1 << 27)])
+        W = GradedSpace(rule, [(0, 1)])
+        sp = HomSpace(rule, [V], [W])
+        x = TensorMap.rand(rt, sp, seed=1)
+        y = TensorMap.zeros(rt, sp)
+        for _ in range(args.warmup):
+            N.hip().tnt_axpby(rt._ctx, y._buf, x._buf, 1.0, 0.0)
+        rt.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            N.hip().tnt_axpby(rt._ctx, y._buf, x._buf, 1.0, 0.0)
+        rt.synchronize()
+        dt = (time.perf_counter() - t) / max(1, args.steps)
+        log(f"[bench] copy calibration: {args.steps} launches of {8 << 27} bytes read + written, "
+            f"{2 * (8 << 27) / dt / 1e9:.0f} GB/s")
+        rt.close()
+        return
     t = time.perf_counter()
     if args.workload == "u1_env":
         wl = MPSEnvUpdate(rt, "U1", args.D, args.precision)

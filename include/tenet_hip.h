@@ -250,6 +250,11 @@ int tnt_set_tile_config(int cfg);
  * order, first match wins; n = 0 clears it. */
 int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n);
 
+/* Number of BF16 digit planes per operand of TNT_PREC_SPLIT_BF16 (2..6,
+ * default 5: relative error ~1e-12 on random data; 4 trades accuracy for
+ * speed).  No reference counterpart. */
+int tnt_set_split_slices(int slices);
+
 /* ---- hipGraph batching ----------------------------------------------------- */
 int tnt_graph_begin(tnt_ctx* ctx);
 int tnt_graph_end(tnt_ctx* ctx, tnt_graph** out);

@@ -106,6 +106,7 @@ HIP_API = {
     "tnt_ctx_kernel_stats": (C.c_int, [VP, C.c_int, SZP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnt_ctx_reset_stats": (C.c_int, [VP]),
     "tnt_set_tile_config": (C.c_int, [C.c_int]),
+    "tnt_set_split_slices": (C.c_int, [C.c_int]),
     "tnt_set_tile_table": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32), SZ]),
     "tnt_graph_begin": (C.c_int, [VP]),

@@ -1144,6 +1144,8 @@ int tnt_set_tile_config(int cfg) {
   return TNT_OK;
 }
 
+int tnt_set_split_slices(int slices) { return split_set_slices(slices); }
+
 int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n) {
   if (n && (!m_max || !n_max || !cfg)) return TNT_E_INVALID;
   std::vector<TileRule> t;

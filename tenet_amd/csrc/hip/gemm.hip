@@ -51,9 +51,10 @@ struct MfmaOp<float> {
 
 constexpr int BK = 16;
 
-// T = LDS/MFMA element type; global operands are always FP64.
+// Register budget: at most 256 VGPRs+AGPRs per lane, so two waves share each
+// SIMD (one wave per SIMD leaves LDS and global latency exposed).
 template <typename T, int BM, int BN>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MT = WM / 16, NT = WN / 16;
   constexpr int PA = 2, PB = 2;  // LDS row padding (elements)
@@ -82,57 +83,67 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  double ra[A_LOADS], rb[B_LOADS];
+  // ---- loader geometry (computed once) ----
+  // A identity: contiguous along i (C = BM, O = BK); adjoint: along k (C = BK, O = BM)
+  // B identity: contiguous along k (C = BK, O = BN); adjoint: along j (C = BN, O = BK)
+  const int a_c = a_adj ? tid % BK : tid % BM;
+  const int a_o = a_adj ? tid / BK : tid / BM;
+  const int a_step = a_adj ? 256 / BK : 256 / BM;
+  const int b_c = b_adj ? tid % BN : tid % BK;
+  const int b_o = b_adj ? tid / BN : tid / BK;
+  const int b_step = b_adj ? 256 / BN : 256 / BK;
+  // element (row i, k kk) of op(A): identity A[i + kk*lda], adjoint A[kk + i*lda]
+  const double* pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
+  const int64_t a_rstride = (int64_t)a_step * lda;  // between consecutive loads r
+  const int64_t a_kadv = a_adj ? BK : (int64_t)BK * lda;
+  const double* pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
+  const int64_t b_rstride = (int64_t)b_step * ldb;
+  const int64_t b_kadv = b_adj ? (int64_t)BK * ldb : BK;
+  // LDS destinations: As[buf][kk][i], Bs[buf][kk][j]
+  // A identity: (kk = a_o + a_step*r, i = a_c); adjoint: (kk = a_c, i = a_o + a_step*r)
+  // B identity: (kk = b_c, j = b_o + b_step*r);  adjoint: (kk = b_o + b_step*r, j = b_c)
+  const bool m_full = m0 + BM <= m, n_full = n0 + BN <= n;
 
-  // Global -> registers for the K slab starting at k0.
+  double ra[A_LOADS], rb[B_LOADS];
   auto load = [&](int k0) {
-    if (!a_adj) {  // A[i][kk] = A[i + kk*lda]; contiguous along i
+    const bool k_full = k0 + BK <= k;
+    if (m_full && k_full) {
+#pragma unroll
+      for (int r = 0; r < A_LOADS; ++r) ra[r] = pa[r * a_rstride];
+    } else {
 #pragma unroll
       for (int r = 0; r < A_LOADS; ++r) {
-        const int e = tid + 256 * r;
-        const int i = e % BM, kk = e / BM;
-        const int gi = m0 + i, gk = k0 + kk;
-        ra[r] = (gi < m && gk < k) ? A[gi + (int64_t)gk * lda] : 0.0;
-      }
-    } else {  // op(A)[i][kk] = X[kk + i*lda]; contiguous along kk
-#pragma unroll
-      for (int r = 0; r < A_LOADS; ++r) {
-        const int e = tid + 256 * r;
-        const int kk = e % BK, i = e / BK;
-        const int gi = m0 + i, gk = k0 + kk;
-        ra[r] = (gi < m && gk < k) ? A[gk + (int64_t)gi * lda] : 0.0;
+        const int i = a_adj ? a_o + a_step * r : a_c;
+        const int kk = a_adj ? a_c : a_o + a_step * r;
+        const bool ok = (m0 + i < m) && (k0 + kk < k);
+        ra[r] = ok ? pa[r * a_rstride] : 0.0;
       }
     }
-    if (!b_adj) {  // B[kk][j] = B[kk + j*ldb]; contiguous along kk
+    if (n_full && k_full) {
+#pragma unroll
+      for (int r = 0; r < B_LOADS; ++r) rb[r] = pb[r * b_rstride];
+    } else {
 #pragma unroll
       for (int r = 0; r < B_LOADS; ++r) {
-        const int e = tid + 256 * r;
-        const int kk = e % BK, j = e / BK;
-        const int gj = n0 + j, gk = k0 + kk;
-        rb[r] = (gj < n && gk < k) ? B[gk + (int64_t)gj * ldb] : 0.0;
-      }
-    } else {  // op(B)[kk][j] = Y[j + kk*ldb]; contiguous along j
-#pragma unroll
-      for (int r = 0; r < B_LOADS; ++r) {
-        const int e = tid + 256 * r;
-        const int j = e % BN, kk = e / BN;
-        const int gj = n0 + j, gk = k0 + kk;
-        rb[r] = (gj < n && gk < k) ? B[gj + (int64_t)gk * ldb] : 0.0;
+        const int j = b_adj ? b_c : b_o + b_step * r;
+        const int kk = b_adj ? b_o + b_step * r : b_c;
+        const bool ok = (n0 + j < n) && (k0 + kk < k);
+        rb[r] = ok ? pb[r * b_rstride] : 0.0;
       }
     }
+    pa += a_kadv;
+    pb += b_kadv;
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < A_LOADS; ++r) {
-      const int e = tid + 256 * r;
-      if (!a_adj) As[buf][e / BM][e % BM] = (T)ra[r];
-      else As[buf][e % BK][e / BK] = (T)ra[r];
+      if (!a_adj) As[buf][a_o + a_step * r][a_c] = (T)ra[r];
+      else As[buf][a_c][a_o + a_step * r] = (T)ra[r];
     }
 #pragma unroll
     for (int r = 0; r < B_LOADS; ++r) {
-      const int e = tid + 256 * r;
-      if (!b_adj) Bs[buf][e % BK][e / BK] = (T)rb[r];
-      else Bs[buf][e / BN][e % BN] = (T)rb[r];
+      if (!b_adj) Bs[buf][b_c][b_o + b_step * r] = (T)rb[r];
+      else Bs[buf][b_o + b_step * r][b_c] = (T)rb[r];
     }
   };
 

@@ -16,6 +16,7 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
                    const std::vector<GTile>& host_tiles, const GTile* dev_tiles, uintptr_t d,
                    uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s,
                    bool capturing, std::string& err);
+int split_set_slices(int slices);
 SplitWorkspace* split_workspace_create();
 void split_workspace_destroy(SplitWorkspace* ws);
 

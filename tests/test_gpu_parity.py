@@ -65,7 +65,7 @@ GEMM_SHAPES = [(16, 4, 16), (1, 1, 1), (17, 5, 33), (64, 64, 64), (130, 70, 129)
 
 
 @pytest.mark.parametrize("lop,rop", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     rng = np.random.default_rng(7 + 2 * lop + rop)
     jobs, lhs, rhs, dst_parts, want = [], [], [], [], []
@@ -91,7 +91,7 @@ def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     got = _get(rt, bd, len(D))
     _free(rt, bl, br, bd)
     w = np.concatenate(want)
-    tol = 1e-13 if prec == 0 else 1e-5
+    tol = {0: 1e-13, 1: 1e-5, 2: 1e-10}[prec]
     o = 0
     for (m, k, n), part in zip(GEMM_SHAPES, want):
         assert rel(got[o:o + m * n], part) < tol, ((m, k, n), lop, rop, prec)
@@ -242,7 +242,7 @@ CONTRACTS = [([2, 3], [0, 1], [0, 1, 2, 3], 2), ([3, 2], [0, 1], [0, 1, 2, 3], 2
 
 @pytest.mark.parametrize("rule_name,pairs", [("U1", U1V), ("SU2", SU2V)])
 @pytest.mark.parametrize("ci", range(len(CONTRACTS)))
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16"])
 def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
     axa, axb, perm, nout = CONTRACTS[ci]
     ps, os_ = spaces(rule_name, [(pairs, False), (pairs, False)], [(pairs, False), (pairs, False)])
@@ -250,7 +250,8 @@ def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
     B = TensorMap.rand(rt, ps, seed=2)
     got = A.contract(axa, B, axb, perm, nout, precision=prec).to_numpy()
     want = OTensor(os_, A.to_numpy()).contract(axa, OTensor(os_, B.to_numpy()), axb, perm, nout).data
-    assert rel(got, want) < (1e-13 if prec == "f64" else 2e-6), (rule_name, ci, prec)
+    tol = {"f64": 1e-13, "f32": 2e-6, "split_bf16": 1e-10}[prec]
+    assert rel(got, want) < tol, (rule_name, ci, prec, rel(got, want))
 
 
 def _oracle_fill_fn(rule, c0):
@@ -337,3 +338,25 @@ def test_graph_capture_replays_match_eager(rt):
         rt.synchronize()
         assert np.array_equal(out.to_numpy(), ref), rep
     h.tnt_graph_destroy(rt._ctx, g)
+
+
+@pytest.mark.parametrize("slices,tol", [(4, 1e-8), (5, 1e-10), (6, 1e-12)])
+def test_split_bf16_error_vs_slices(rt, slices, tol):
+    """Relative Frobenius error of the split-BF16 GEMM against FP64, per slice count."""
+    rng = np.random.default_rng(slices)
+    m, k, n = 200, 700, 150
+    A = rng.standard_normal((m, k)) * np.exp(rng.uniform(-3, 3, (m, 1)))  # uneven row scales
+    B = rng.standard_normal((k, n))
+    bl, br = _buf(rt, A.reshape(-1, order="F")), _buf(rt, B.reshape(-1, order="F"))
+    bd = _buf(rt, np.zeros(m * n))
+    jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
+    rt._check(N.hip().tnt_set_split_slices(slices))
+    try:
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 1.0, 0.0, 2))
+        got = _get(rt, bd, m * n).reshape(n, m).T
+    finally:
+        N.hip().tnt_set_split_slices(5)
+        _free(rt, bl, br, bd)
+    err = rel(got, A @ B)
+    print(f"split-bf16 slices={slices}: rel err {err:.3e}")
+    assert err < tol
