@@ -65,6 +65,20 @@ def cpu_baseline(workload, D, budget_s=20.0):
             t2 = t1.contract([1, 2], W, [3, 1], [0, 2, 3, 1], 2)
             return Ad.contract([2, 0], t2, [0, 2], [0, 1, 2], 2)
         sample = f"full {rname} environment update at D={D}"
+    elif workload == "batched":
+        from tenet_amd.workloads import batched_specs
+        orule = make_rule("U1xZ2")
+        oP = OL([(orule.encode((0, 0)), 1), (orule.encode((1, 1)), 1)])
+        ops = []
+        nsample = 16
+        for i, (pl, pm, pr) in enumerate(batched_specs(256)[:nsample]):
+            Vl, Vm, Vr = (OL([(orule.encode(l), d) for l, d in x]) for x in (pl, pm, pr))
+            ops.append((OTensor.rand(OH(orule, [Vl, oP], [Vm]), 1000 + 2 * i),
+                        OTensor.rand(OH(orule, [Vm], [oP, Vr]), 1001 + 2 * i)))
+
+        def step():
+            return [a.compose(b) for a, b in ops]
+        sample = f"first {nsample} of the 256 merges (FP64 numpy oracle)"
     else:
         orule = make_rule("SU2")
         _, V = su2_fmove_space(8, 8)  # bounded sample: multiplicity floor(8/(2j+1))
@@ -92,13 +106,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="u1_env",
-                    choices=["u1_env", "su2_env", "su2_fmove", "copy_calibration"])
+                    choices=["u1_env", "su2_env", "su2_fmove", "batched", "copy_calibration"])
+    ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "split_bf16"])
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
-    ap.add_argument("--sweep-tiles", action="store_true",
-                    help="time the workload with every forced GEMM tile shape (autotuner input)")
+    ap.add_argument("--autotune", default="on", choices=["on", "off"],
+                    help="pick GEMM tile shapes per contraction by measurement before timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -108,9 +123,10 @@ def main():
 
     from tenet_amd import _native as N
     from tenet_amd.tensor import Runtime
-    from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract
+    from tenet_amd.workloads import BatchedMerge, MPSEnvUpdate, SU2FMoveContract
 
     rt = Runtime(0)
+    batched_err = None
     if args.workload == "copy_calibration":
         # y = x over 2^27 doubles (1 GiB read + 1 GiB written per launch): the
         # reference point that scales FETCH_SIZE / WRITE_SIZE to bytes
@@ -143,6 +159,25 @@ def main():
         wl = MPSEnvUpdate(rt, "SU2", args.D, args.precision)
         cfg = {"workload": "su2_mps_env_update", "bond_dim": args.D, "spins": "2j=0..20",
                "phys": "spin-1/2", "mpo_bond": "SU2 {0:2, 1:1}"}
+    elif args.workload == "batched":
+        import numpy as np
+        wl = BatchedMerge(rt, args.count, precision=args.precision)
+        cfg = {"workload": "batched_u1xz2_two_site_merge", "contractions": args.count,
+               "bond_dims": "uniform 64..1024 (seed 4)", "charges": "-6..6 x parity",
+               "phys": "U1xZ2 {(0,0),(1,1)}", "launch": "one grouped GEMM launch per tile shape, hipGraph replay",
+               "gemm_jobs": wl.n_jobs}
+        # FP64 reference of the same batch for the error of reduced-precision payloads
+        wl.run("f64")
+        rt.synchronize()
+        ref = [r.copy() for r in wl.results()]
+        wl.run()
+        rt.synchronize()
+        got = wl.results()
+        num = sum(float(np.linalg.norm(g - r) ** 2) for g, r in zip(got, ref))
+        den = sum(float(np.linalg.norm(r) ** 2) for r in ref)
+        batched_err = (num / den) ** 0.5
+        wl.capture()
+        log(f"[bench] batched: {wl.n_jobs} sector GEMMs, {args.precision} rel. Frobenius error vs FP64 {batched_err:.3e}")
     else:
         wl = SU2FMoveContract(rt, args.mult, 8, args.precision)
         cfg = {"workload": "su2_fmove_contract", "spins": "2j=0..8", "multiplicity": f"floor({args.mult}/(2j+1))",
@@ -153,23 +188,11 @@ def main():
         wl.run()
     rt.synchronize()
     h = N.hip()
-    if args.sweep_tiles:
-        sweep = {}
-        for cfg in (-1, 0, 1, 2, 3):
-            h.tnt_set_tile_config(cfg)
-            rt.clear_caches()  # tile tables are built at artifact upload
-            wl.run()
-            rt.synchronize()
-            t = time.perf_counter()
-            for _ in range(args.steps):
-                wl.run()
-            rt.synchronize()
-            sweep[cfg] = (time.perf_counter() - t) * 1e3 / args.steps
-            log(f"[bench] tile cfg {cfg}: {sweep[cfg]:.3f} ms/step")
-        h.tnt_set_tile_config(-1)
-        rt.clear_caches()
-        wl.run()
-        rt.synchronize()
+    tuned = None
+    if args.autotune == "on" and args.workload != "batched":
+        from tenet_amd.autotune import autotune
+        conts = list(wl.contractions) if hasattr(wl, "contractions") else [wl.contraction]
+        tuned = autotune(rt, conts, wl.run, reps=max(2, min(5, args.steps // 4)), log=log)
     rt._check(h.tnt_ctx_reset_stats(rt._ctx))
     rt._check(h.tnt_ctx_set_timing(rt._ctx, 1))
     rt.synchronize()
@@ -198,7 +221,15 @@ def main():
     if args.cpu_baseline == "on":
         import numpy as np
         t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D)
-        if args.workload == "su2_fmove":
+        if args.workload == "batched":
+            nsample = len(out)
+            flops_sample = sum(c.flops for *_, c in wl.pairs[:nsample])
+            got = wl.results()[:nsample]
+            num = sum(float(np.linalg.norm(g - o.data) ** 2) for g, o in zip(got, out))
+            den = sum(float(np.linalg.norm(o.data) ** 2) for o in out)
+            parity = (num / den) ** 0.5
+            cpu_val = flops_sample / t_cpu / 1e12
+        elif args.workload == "su2_fmove":
             flops_sample = None
             from tenet_amd.tensor import Runtime as _R  # noqa: F401
             from tenet_amd.workloads import SU2FMoveContract as _S
@@ -240,6 +271,8 @@ def main():
         "cpu_baseline": cpu,
         "parity_rel_err_vs_oracle": parity,
         "cold_compile_s": cold,
+        "tile_configs": tuned,
+        "rel_err_vs_f64": batched_err if args.workload == "batched" else None,
     }
     print(json.dumps(rec), flush=True)
     rt.close()

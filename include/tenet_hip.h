@@ -221,6 +221,10 @@ int tnt_tree_transform(tnt_ctx* ctx, const tnt_plan* plan, tnt_buf* dst, const t
 /* ---- contraction artifact -------------------------------------------------- */
 int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* desc, tnt_artifact** out);
 int tnt_contract_destroy(tnt_ctx* ctx, tnt_artifact* art);
+/* Rebuild the artifact's GEMM tile table with one tile configuration (see
+ * tnt_set_tile_config for the codes; -1 = table default).  The autotuner
+ * measures each configuration on the artifact and keeps the fastest. */
+int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* art, int cfg);
 int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_buf* lhs,
                  const tnt_buf* rhs, double alpha, double beta, int precision);
 
@@ -243,8 +247,8 @@ int tnt_ctx_reset_stats(tnt_ctx* ctx);
 
 /* ---- tile-shape autotuner --------------------------------------------------- */
 /* Force one GEMM tile configuration for tables built after this call
- * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128; -1: per-job choice from the
- * tuned table).  Used by the autotuner sweep; no reference counterpart. */
+ * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128, 4: narrow-output vector kernel;
+ * -1: per-job choice).  Used by the autotuner sweep; no reference counterpart. */
 int tnt_set_tile_config(int cfg);
 /* Install the tuned per-shape table: entries (m_max, n_max, cfg) scanned in
  * order, first match wins; n = 0 clears it. */

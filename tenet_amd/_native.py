@@ -98,6 +98,7 @@ HIP_API = {
     "tnt_tree_transform": (C.c_int, [VP, VP, VP, VP, C.c_double, C.c_double, C.c_int]),
     "tnt_contract_upload": (C.c_int, [VP, C.POINTER(ContractDesc), PP]),
     "tnt_contract_destroy": (C.c_int, [VP, VP]),
+    "tnt_contract_set_tile_config": (C.c_int, [VP, VP, C.c_int]),
     "tnt_contract": (C.c_int, [VP, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int]),
     "tnt_weighted_dot": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ,
                                    C.POINTER(C.c_double)]),

@@ -1,0 +1,82 @@
+"""Tile-shape autotuner for the grouped block GEMM.
+
+For each contraction artifact of a workload the tuner times the workload
+step with every GEMM tile configuration of that artifact (the other
+artifacts held fixed) and keeps the fastest -- coordinate descent over the
+artifacts, in the order they run.  Results are keyed by the artifact's
+job-shape signature (the sorted (m, k, n) list), kept on the Runtime and,
+when TENET_AMD_TILE_CACHE names a file, persisted as JSON so later runs
+start tuned.
+"""
+import hashlib
+import json
+import os
+import time
+
+from . import _native as N
+
+CONFIGS = (0, 1, 2, 3)  # 128x128, 64x64, 128x64, 64x128 (narrow jobs keep the vector kernel)
+
+
+def signature(contraction):
+    d = contraction.desc.contents
+    shapes = sorted((d.jobs[i].rows, d.jobs[i].contracted, d.jobs[i].cols) for i in range(d.n_jobs))
+    return hashlib.sha1(repr(shapes).encode()).hexdigest()[:16]
+
+
+def _cache_path():
+    return os.environ.get("TENET_AMD_TILE_CACHE")
+
+
+def load_cache():
+    p = _cache_path()
+    if p and os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh)
+    return {}
+
+
+def save_cache(table):
+    p = _cache_path()
+    if p:
+        with open(p, "w") as fh:
+            json.dump(table, fh, indent=1, sort_keys=True)
+
+
+def apply(rt, contraction, cfg):
+    rt._check(N.hip().tnt_contract_set_tile_config(rt._ctx, contraction.art, int(cfg)))
+    contraction.tile_cfg = int(cfg)
+
+
+def _time(rt, step, reps):
+    step()
+    rt.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        step()
+    rt.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def autotune(rt, contractions, step, reps=5, log=None):
+    """Pick a tile configuration per contraction artifact; returns {signature: cfg}."""
+    table = getattr(rt, "tile_table", None)
+    if table is None:
+        table = rt.tile_table = load_cache()
+    for c in contractions:
+        sig = signature(c)
+        if sig in table:
+            apply(rt, c, table[sig])
+            continue
+        best, best_t = -1, None
+        for cfg in (-1,) + CONFIGS:
+            apply(rt, c, cfg)
+            t = _time(rt, step, reps)
+            if log:
+                log(f"[autotune] {sig} cfg {cfg}: {t * 1e3:.3f} ms")
+            if best_t is None or t < best_t * 0.995:
+                best, best_t = cfg, t
+        apply(rt, c, best)
+        table[sig] = best
+    save_cache(table)
+    return table

@@ -417,9 +417,10 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     for (size_t i = 2; i < dims.size(); ++i) outer *= dims[i];
     int64_t elems = plane * outer;
     bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
-    // ~8K element-updates per work item (Multi weighted by its layouts)
+    // Single blocks: 1024-element items (4 per thread) swept in order by a
+    // persistent grid; Multi: ~8K element-updates per item, weighted by its layouts
     int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
-    int64_t target = std::max<int64_t>(256, 8192 / weight);
+    int64_t target = kind == XK_SINGLE ? 1024 : std::max<int64_t>(256, 8192 / weight);
     if (plane >= target) {
       int64_t chunk = ((target + 63) / 64) * 64;
       for (int64_t o = 0; o < outer; ++o)
@@ -494,22 +495,41 @@ struct TileRule {
   int64_t m_max, n_max;
   int32_t cfg;
 };
-static std::vector<TileRule> tile_table;  // tuned table (tnt_set_tile_table)
-
-static int choose_tile(int64_t m, int64_t n, int64_t k) {
-  (void)k;
-  if (n <= 16) return TILE_SKINNY;
-  for (const auto& r : tile_table)
-    if (m <= r.m_max && n <= r.n_max) return r.cfg;
-  if (m >= 128 && n >= 128) return TILE_128x128;
-  if (m >= 128 && n < 128) return TILE_128x64;
-  if (m < 128 && n >= 128) return TILE_64x128;
-  return TILE_64x64;
-}
+static std::vector<TileRule> tile_table;  // tuned per-shape rules (tnt_set_tile_table)
 
 static int tile_override = -1;  // set by tnt_set_tile_config (autotuner sweeps)
 
-static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs) {
+// Table-level default: the largest tile shape that still gives every CU at
+// least ~2 (128x128) or 1 (64x128 / 128x64) workgroups; narrow outputs go to
+// the vector-unit kernel.  A per-artifact override (autotuner) wins.
+static int default_family(const std::vector<GJob>& jobs) {
+  auto count = [&](int cfg) {
+    int64_t t = 0;
+    for (const auto& j : jobs) {
+      if (j.m <= 0 || j.n <= 0 || j.n <= 16) continue;
+      t += (int64_t)((j.m + tile_rows(cfg) - 1) / tile_rows(cfg)) * ((j.n + tile_cols(cfg) - 1) / tile_cols(cfg));
+    }
+    return t;
+  };
+  if (count(TILE_128x128) >= 4 * 256) return TILE_128x128;
+  if (count(TILE_64x128) >= 2 * 256) return TILE_64x128;
+  return TILE_64x64;
+}
+
+static int job_cfg(const GJob& j, int family) {
+  if (j.n <= 16) return TILE_SKINNY;
+  for (const auto& r : tile_table)
+    if (j.m <= r.m_max && j.n <= r.n_max) return r.cfg;
+  int cfg = family;
+  // keep narrow dimensions off oversized tiles
+  if (cfg == TILE_128x128 && j.m < 128 && j.n < 128) cfg = TILE_64x64;
+  else if (cfg == TILE_128x128 && j.m < 128) cfg = TILE_64x128;
+  else if (cfg == TILE_128x128 && j.n < 128) cfg = TILE_128x64;
+  else if (cfg == TILE_64x128 && j.m > 2 * j.n) cfg = TILE_128x64;
+  return cfg;
+}
+
+static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int forced = -1) {
   t.host_jobs = jobs;
   std::vector<std::vector<GTile>> per(TILE_CFG_COUNT);
   // larger jobs first for load balance
@@ -519,11 +539,13 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs) {
     double ca = double(jobs[a].m) * jobs[a].n * jobs[a].k, cb = double(jobs[b].m) * jobs[b].n * jobs[b].k;
     return ca > cb;
   });
+  const int ovr = forced >= 0 ? forced : tile_override;
+  const int family = default_family(jobs);
   for (size_t oi : order) {
     const GJob& j = jobs[oi];
     if (j.m <= 0 || j.n <= 0) continue;
-    int cfg = choose_tile(j.m, j.n, j.k);
-    if (tile_override >= 0 && !(cfg == TILE_SKINNY && tile_override != TILE_SKINNY)) cfg = tile_override;
+    int cfg = job_cfg(j, family);
+    if (ovr >= 0 && cfg != TILE_SKINNY) cfg = ovr;
     if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
     int bm = tile_rows(cfg), bn = tile_cols(cfg);
     int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
@@ -941,6 +963,35 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     a->scratch_c = static_cast<double*>(p);
   }
   *out = a;
+  return TNT_OK;
+}
+
+int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
+  if (!ctx || !a) return TNT_E_INVALID;
+  if (cfg < -1 || cfg >= TILE_CFG_COUNT) FAIL(ctx, TNT_E_INVALID, "unknown tile configuration");
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "tile configuration cannot change during graph capture");
+  if (int rc = set_device(ctx)) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<GJob> jobs = a->gemm.host_jobs;
+  GemmTable t;
+  build_gemm_table(t, jobs, cfg);
+  std::vector<void*> allocs;
+  if (int rc = upload_gemm_table(ctx, t, allocs)) {
+    for (auto p : allocs) pool_free(ctx, p);
+    return rc;
+  }
+  // release the previous tables (they are the first two gemm allocations of the artifact)
+  auto drop = [&](void* p) {
+    auto it = std::find(a->allocs.begin(), a->allocs.end(), p);
+    if (it != a->allocs.end()) {
+      pool_free(ctx, p);
+      a->allocs.erase(it);
+    }
+  };
+  drop(a->gemm.jobs);
+  drop(a->gemm.tiles);
+  a->gemm = t;
+  a->allocs.insert(a->allocs.end(), allocs.begin(), allocs.end());
   return TNT_OK;
 }
 

@@ -31,8 +31,17 @@ __device__ __forceinline__ int64_t base_offset(const XLayout& L, const int64_t* 
   return off;
 }
 
-__global__ __launch_bounds__(256) void xform_kernel(XformArgs a) {
-  const XItem it = a.items[blockIdx.x];
+__device__ void xform_item(const XformArgs& a, const XItem it);
+
+// Persistent grid-stride loop over work items: concurrently resident
+// workgroups process consecutive items, so the waves of the whole GPU sweep
+// one contiguous window of the payloads at a time (DRAM locality), like a
+// plain grid-stride copy.
+__global__ __launch_bounds__(256) void xform_kernel(XformArgs a, uint32_t n_items) {
+  for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) xform_item(a, a.items[i]);
+}
+
+__device__ void xform_item(const XformArgs& a, const XItem it) {
   const XBlock& b = a.blocks[it.block];
   const int rank = b.rank;
   const int kind = b.kind;
@@ -40,6 +49,54 @@ __global__ __launch_bounds__(256) void xform_kernel(XformArgs a) {
   const int64_t n1 = rank > 1 ? b.dims[1] : 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+
+  if (it.o_count == 1 && kind == XK_SINGLE) {
+    // one outer index, up to 1024 plane elements: element e = e_begin +
+    // 64 * (4u + wave) + lane, u < 4 -- the whole workgroup sweeps the item
+    // front to back in 256-element steps
+    int64_t idx[kMaxRank];
+    {
+      uint64_t rr = it.o_begin;
+#pragma unroll
+      for (int d = 2; d < kMaxRank; ++d) {
+        if (d < rank) {
+          idx[d] = (int64_t)(rr % (uint64_t)b.dims[d]);
+          rr /= (uint64_t)b.dims[d];
+        } else {
+          idx[d] = 0;
+        }
+      }
+    }
+    const XLayout& Ld = a.layouts[b.lay];
+    const XLayout& Ls = a.layouts[b.lay + 1];
+    const int64_t od = base_offset(Ld, idx, rank), os = base_offset(Ls, idx, rank);
+    const int64_t d0 = Ld.strides[0], d1 = rank > 1 ? Ld.strides[1] : 0;
+    const int64_t s0 = Ls.strides[0], s1 = rank > 1 ? Ls.strides[1] : 0;
+    const double c = a.alpha * a.coeffs[b.coeff];
+    const bool readd = a.overwrite == 0 && a.beta != 0.0;
+    const uint32_t n0u = (uint32_t)n0;
+    double v[kUnroll];
+    int64_t dp[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t f = 64u * (4u * u + wave) + lane;
+      if (f < it.e_count) {
+        const uint32_t e = it.e_begin + f;
+        const uint32_t x = e % n0u, y = e / n0u;
+        v[u] = a.src[os + x * s0 + y * s1];
+        dp[u] = od + x * d0 + y * d1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint32_t f = 64u * (4u * u + wave) + lane;
+      if (f < it.e_count) {
+        double* p = a.dst + dp[u];
+        *p = readd ? c * v[u] + a.beta * *p : c * v[u];
+      }
+    }
+    return;
+  }
 
   // Waves split outer indices when there are enough of them, else the plane.
   uint64_t o_first, o_step;
@@ -162,7 +219,8 @@ __global__ __launch_bounds__(256) void xform_kernel(XformArgs a) {
 
 void launch_xform(const XformArgs& a, uint32_t n_items, hipStream_t s) {
   if (n_items == 0) return;
-  hipLaunchKernelGGL(xform_kernel, dim3(n_items), dim3(256), 0, s, a);
+  const uint32_t grid = n_items < 4096u ? n_items : 4096u;
+  hipLaunchKernelGGL(xform_kernel, dim3(grid), dim3(256), 0, s, a, n_items);
 }
 
 }  // namespace tnt_dev

@@ -17,6 +17,8 @@ the environment, the reference's uniform [-1, 1) fill for the F-move case.
 """
 import math
 
+import numpy as np
+
 from .tensor import GradedSpace, Rule, TensorMap
 
 
@@ -135,3 +137,90 @@ class SU2FMoveContract:
         self.out = self.A.contract([1, 3], self.B, [2, 0], [0, 1, 2, 3], 2, out=self.out,
                                    precision=self.precision)
         return self.out
+
+
+def u1z2_bond(rng, D, qmax=6):
+    """U(1)xZ2 bond of dimension D: charges -qmax..qmax x parity, multinomial degeneracies."""
+    labels = [(q, p) for q in range(-qmax, qmax + 1) for p in (0, 1)]
+    w = np.exp(-np.array([q * q for q, _ in labels]) / (2.0 * (qmax / 2.0) ** 2))
+    counts = rng.multinomial(max(0, D - len(labels)), w / w.sum()) + 1  # every sector present
+    return [(l, int(c)) for l, c in zip(labels, counts)]
+
+
+def batched_specs(count=256, dmin=64, dmax=1024, seed=4):
+    """Bond legs (label, degeneracy) lists (V_l, V_m, V_r) of each merge, deterministic in seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        Dl, Dm, Dr = (int(x) for x in rng.integers(dmin, dmax + 1, size=3))
+        out.append((u1z2_bond(rng, Dl), u1z2_bond(rng, Dm), u1z2_bond(rng, Dr)))
+    return out
+
+
+class BatchedMerge:
+    """Many independent U(1)xZ2 two-site merges C_i = A_i . B_i in one grouped GEMM launch.
+
+    A_i in V_l (x) P <- V_m, B_i in V_m <- P (x) V_r, P = {(0,0), (1,1)} (d = 2);
+    bond dimensions uniform on [dmin, dmax] from seed 4.  All coupled-sector
+    GEMMs of all contractions go to tnt_gemm_grouped as pointer jobs (one
+    launch per tile shape), optionally captured in a hipGraph.
+    """
+
+    def __init__(self, rt, count=256, dmin=64, dmax=1024, seed=4, precision="f32"):
+        import ctypes as C
+        from . import _native as N
+        self.rt = rt
+        self.precision = precision
+        rule = Rule("U1xZ2")
+        P = GradedSpace(rule, [((0, 0), 1), ((1, 1), 1)])
+        self.pairs = []
+        jobs = []
+        self.flops = 0.0
+        h = N.hip()
+        for i, (pl, pm, pr) in enumerate(batched_specs(count, dmin, dmax, seed)):
+            Vl, Vm, Vr = GradedSpace(rule, pl), GradedSpace(rule, pm), GradedSpace(rule, pr)
+            A = TensorMap.rand(rt, [Vl, P], [Vm], seed=1000 + 2 * i)
+            B = TensorMap.rand(rt, [Vm], [P, Vr], seed=1001 + 2 * i)
+            c = rt.contraction(A.space, B.space, [2], [0], [0, 1, 2, 3], 2)
+            Cm = TensorMap.zeros(rt, c.dst_space)
+            d = c.desc.contents
+            ptr = {}
+            for name, t in (("a", A), ("b", B), ("c", Cm)):
+                p = C.c_void_p()
+                h.tnt_buf_info(t._buf, C.byref(p), None, None, None)
+                ptr[name] = p.value
+            for k in range(d.n_jobs):
+                j = d.jobs[k]
+                jobs.append(N.GemmPtrJob(ptr["c"] + 8 * j.dst_offset, ptr["a"] + 8 * j.lhs_offset,
+                                         ptr["b"] + 8 * j.rhs_offset, j.rows, j.contracted, j.cols,
+                                         max(1, j.rows), max(1, j.rows), max(1, j.contracted), 0, 0))
+            self.flops += c.flops
+            self.pairs.append((A, B, Cm, c))
+        self.jobs = (N.GemmPtrJob * len(jobs))(*jobs)
+        self.n_jobs = len(jobs)
+        self.graph = None
+
+    def run(self, precision=None):
+        from . import _native as N
+        from .tensor import PRECISIONS
+        prec = PRECISIONS[precision or self.precision]
+        if self.graph is not None and precision is None:
+            self.rt._check(N.hip().tnt_graph_launch(self.rt._ctx, self.graph))
+            return
+        self.rt._check(N.hip().tnt_gemm_grouped(self.rt._ctx, self.jobs, self.n_jobs, 1.0, 0.0, prec))
+
+    def capture(self):
+        """Record the grouped launch in a hipGraph (tables built by one eager run first)."""
+        import ctypes as C
+        from . import _native as N
+        self.run()
+        self.rt.synchronize()
+        h = N.hip()
+        self.rt._check(h.tnt_graph_begin(self.rt._ctx))
+        self.run()
+        g = C.c_void_p()
+        self.rt._check(h.tnt_graph_end(self.rt._ctx, C.byref(g)))
+        self.graph = g
+
+    def results(self):
+        return [Cm.to_numpy() for _, _, Cm, _ in self.pairs]

@@ -360,3 +360,31 @@ def test_split_bf16_error_vs_slices(rt, slices, tol):
     err = rel(got, A @ B)
     print(f"split-bf16 slices={slices}: rel err {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("split_bf16", 1e-10)])
+def test_batched_grouped_launch_matches_oracle(rt, prec, tol):
+    """configs[3] shape at small sizes: every merge of one grouped launch vs the FP64 oracle."""
+    from oracle.fusion import HomSpace as OH, Leg as OL
+    from tenet_amd.workloads import BatchedMerge, batched_specs
+    count, dmin, dmax = 6, 32, 64
+    wl = BatchedMerge(rt, count, dmin, dmax, seed=11, precision=prec)
+    wl.run()
+    rt.synchronize()
+    got = wl.results()
+    orule = make_rule("U1xZ2")
+    oP = OL([(orule.encode((0, 0)), 1), (orule.encode((1, 1)), 1)])
+    for i, ((pl, pm, pr), g) in enumerate(zip(batched_specs(count, dmin, dmax, 11), got)):
+        Vl, Vm, Vr = (OL([(orule.encode(l), d) for l, d in x]) for x in (pl, pm, pr))
+        a = OTensor(OH(orule, [Vl, oP], [Vm]), wl.pairs[i][0].to_numpy())
+        b = OTensor(OH(orule, [Vm], [oP, Vr]), wl.pairs[i][1].to_numpy())
+        assert rel(g, a.compose(b).data) < tol, (i, prec)
+    # hipGraph replay of the same grouped launch reproduces the eager result
+    wl.capture()
+    for rep in range(3):
+        for _, _, Cm, _ in wl.pairs:
+            Cm.scale_(0.0)
+        rt.synchronize()
+        wl.run()
+        rt.synchronize()
+        assert all(np.array_equal(x, y) for x, y in zip(wl.results(), got)), rep
