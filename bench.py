@@ -13,8 +13,11 @@ FP64 GEMM flops / wall time); `roofline` = the dominant kernel (grouped block
 GEMM) timed with HIP events on the backend stream inside the timed region;
 `cpu_baseline` = the CPU oracle (numpy, FP64) on the same workload.
 
-Other workloads: --workload su2_env (SU(2) bond of dimension 4096) and
---workload su2_fmove (SU(2) 2j<=8 F-move + contraction, 4.27 TFLOP/step).
+Other workloads: --workload su2_env (SU(2) bond of dimension 4096),
+--workload su2_fmove (SU(2) 2j<=8 F-move + contraction, 4.27 TFLOP/step),
+--workload batched (256 U(1)xZ2 merges in one grouped launch, hipGraph) and
+--workload serve (Poisson stream of two-site H_eff matvecs, continuous
+batching over HIP streams; reports matvecs/s and p50/p99 latency).
 """
 import argparse
 import ctypes as C
@@ -100,18 +103,118 @@ def cpu_baseline(workload, D, budget_s=20.0):
     return times[len(times) // 2], cores, sample, out, len(times)
 
 
+def cpu_serve_baseline(rule, trace, budget_s=20.0):
+    """Oracle (numpy FP64) H_eff matvecs over the head of the same trace."""
+    from oracle.fusion import HomSpace as OH, Leg as OL
+    from oracle.symmetry import make_rule
+    from oracle.tensor import OTensor
+    from tenet_amd.workloads import mps_env_spaces
+    orule = make_rule(rule)
+    env = {}
+    t_work = 0.0
+    done = 0
+    t_start = time.perf_counter()
+    for _, D, seed in trace:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        if D not in env:
+            _, V, P, M = mps_env_spaces(rule, D)
+            lg = lambda g: OL(list(zip(g.ids, g.degs)), g.is_dual)
+            oV, oP, oM = lg(V), lg(P), lg(M)
+            env[D] = (OTensor.randn(OH(orule, [oV, oM], [oV]), 1), OTensor.randn(OH(orule, [oM, oP], [oP, oM]), 2),
+                      OTensor.randn(OH(orule, [oM, oP], [oP, oM]), 3), OTensor.randn(OH(orule, [oV], [oV, oM]), 4),
+                      OH(orule, [oV], [oP, oP, oV]))
+        L, W1, W2, R, tsp = env[D]
+        th = OTensor.randn(tsp, seed)
+        t = time.perf_counter()
+        x1 = L.contract([2], th, [0], [0, 1, 2, 3, 4], 2)
+        x2 = x1.contract([1, 2], W1, [3, 1], [0, 3, 4, 1, 2], 2)
+        x3 = x2.contract([1, 3], W2, [3, 1], [0, 3, 1, 4, 2], 2)
+        y = x3.contract([1, 4], R, [2, 0], [0, 1, 2, 3], 1)
+        th.inner(y)
+        t_work += time.perf_counter() - t
+        done += 1
+    return done / t_work, done
+
+
+def run_serve(args):
+    """Continuous-batching H_eff server on a Poisson trace; one JSON line."""
+    import numpy as np
+
+    from tenet_amd.serving import HeffServer, poisson_trace
+    t = time.perf_counter()
+    srv = HeffServer(args.rule, n_streams=args.streams, max_inflight=args.inflight,
+                     precision=args.precision)
+    srv.warm()
+    cold = time.perf_counter() - t
+    log(f"[bench] serve: {len(srv.buckets)} bond-dimension buckets {srv.buckets}, environments resident, "
+        f"compiled on {args.streams} streams in {cold:.1f}s")
+    # warm-up pass over a short trace (pools, tables), then the timed trace
+    warm = poisson_trace(max(64, args.warmup * 32), rate=args.rate, seed=6, buckets=srv.buckets)
+    srv.serve(warm)
+    trace = poisson_trace(args.requests, rate=args.rate, seed=5, buckets=srv.buckets)
+    stats = srv.serve(trace)
+    log(f"[bench] serve: {stats['requests']} requests in {stats['wall_s']:.2f}s "
+        f"(offered {stats['offered_rate']:.0f}/s): {stats['matvecs_per_s']:.0f} matvecs/s, "
+        f"p50 {stats['p50_ms']:.3f} ms, p99 {stats['p99_ms']:.3f} ms, {stats['tflops']:.2f} TFLOP/s")
+    cpu = None
+    if args.cpu_baseline == "on":
+        rate, n = cpu_serve_baseline(args.rule, trace)
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        except Exception:  # pragma: no cover
+            cores = os.cpu_count() or 1
+        cpu = {"value": rate, "unit": "matvecs/s", "cores": int(cores), "kind": "port",
+               "sample": f"first {n} requests of the same trace, served back to back by the FP64 numpy oracle "
+                         f"(matvec + Rayleigh numerator, no queueing)"}
+        log(f"[bench] cpu baseline {rate:.2f} matvecs/s over {n} requests")
+    ds = np.array([d for _, d, _ in trace])
+    rec = {
+        "metric": "H_eff matvecs/s (continuous batching, p50/p99 latency)",
+        "value": stats["matvecs_per_s"],
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": stats["requests"],
+        "warmup": len(warm),
+        "ms_per_step": 1e3 / stats["matvecs_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)"}[args.precision]
+                 + ", f64 dot",
+        "data": "synthetic (seeded normal(0,1) environments and states on device)",
+        "config": {"workload": f"serve_two_site_heff_{args.rule.lower()}", "arrivals": f"poisson {args.rate:.0f}/s",
+                   "bond_dims": "lognormal median 512 sigma 0.7 clipped 32..4096, snapped to buckets",
+                   "buckets": srv.buckets, "streams": args.streams, "slots": args.inflight,
+                   "mean_D": float(ds.mean())},
+        "latency_ms": {"p50": stats["p50_ms"], "p99": stats["p99_ms"], "mean": stats["mean_ms"]},
+        "offered_rate": stats["offered_rate"],
+        "tflops": stats["tflops"],
+        "cpu_baseline": cpu,
+        "cold_compile_s": cold,
+    }
+    print(json.dumps(rec), flush=True)
+    srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="u1_env",
-                    choices=["u1_env", "su2_env", "su2_fmove", "batched", "copy_calibration"])
+                    choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration"])
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "split_bf16"])
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--rule", default="U1", choices=["U1", "SU2"], help="serve: symmetry of the chain")
+    ap.add_argument("--requests", type=int, default=4000, help="serve: requests in the timed trace")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: mean arrival rate (req/s)")
+    ap.add_argument("--streams", type=int, default=4, help="serve: backend contexts (HIP streams)")
+    ap.add_argument("--inflight", type=int, default=16, help="serve: request slots")
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
                     help="pick GEMM tile shapes per contraction by measurement before timing")
     args = ap.parse_args()
@@ -120,6 +223,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (sharding is by coupled "
               "sector; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "serve":
+        run_serve(args)
+        return
 
     from tenet_amd import _native as N
     from tenet_amd.tensor import Runtime

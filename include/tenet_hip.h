@@ -87,6 +87,7 @@ typedef struct tnt_buf tnt_buf;
 typedef struct tnt_plan tnt_plan;
 typedef struct tnt_artifact tnt_artifact;
 typedef struct tnt_graph tnt_graph;
+typedef struct tnt_event tnt_event;
 
 /* DenseGemmBatchJob (tenet-dense/src/executor.rs:15-22): column-major
  * dst[rows x cols] (+)= op(lhs)[rows x contracted] * op(rhs)[contracted x cols]. */
@@ -232,8 +233,21 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_
 /* out = sum_r weights[r] * sum_{i in region r} a[i] * b[i]  (FP64, deterministic). */
 int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
                      const double* weights, size_t n_regions, double* out);
+/* Stream-ordered tnt_weighted_dot: the result is written to out[out_index] on
+ * the device, no host synchronisation (the FP64 inner products of a device
+ * Krylov loop, KrylovVector::dot_real, tenet-krylov/src/vector.rs:13). */
+int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                           const double* weights, size_t n_regions, tnt_buf* out, size_t out_index);
 /* y = alpha * x + beta * y over whole payloads. */
 int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta);
+
+/* ---- completion events (serving loop) -------------------------------------- */
+/* Record on the context stream; query returns 1 when all work queued before the
+ * record has completed, 0 while it runs.  No reference counterpart. */
+int tnt_event_create(tnt_ctx* ctx, tnt_event** out);
+int tnt_event_record(tnt_ctx* ctx, tnt_event* ev);
+int tnt_event_query(tnt_ctx* ctx, const tnt_event* ev);
+int tnt_event_destroy(tnt_ctx* ctx, tnt_event* ev);
 
 /* ---- kernel timing ---------------------------------------------------------- */
 /* When on, every recoupling replay (kind 0) and block-GEMM launch (kind 1 + precision)

@@ -13,6 +13,7 @@ LIBDIR = os.path.join(HERE, "lib")
 MAX_RANK = 8
 
 OK = 0
+E_INVALID, E_UNSUPPORTED, E_SHAPE, E_OOB, E_DEVICE_MISMATCH, E_HIP, E_OOM = -1, -2, -3, -4, -5, -6, -7
 ERRORS = {-1: "INVALID", -2: "UNSUPPORTED", -3: "SHAPE", -4: "OOB", -5: "DEVICE_MISMATCH",
           -6: "HIP", -7: "OOM"}
 
@@ -103,6 +104,11 @@ HIP_API = {
     "tnt_weighted_dot": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ,
                                    C.POINTER(C.c_double)]),
     "tnt_axpby": (C.c_int, [VP, VP, VP, C.c_double, C.c_double]),
+    "tnt_weighted_dot_async": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ, VP, SZ]),
+    "tnt_event_create": (C.c_int, [VP, PP]),
+    "tnt_event_record": (C.c_int, [VP, VP]),
+    "tnt_event_query": (C.c_int, [VP, VP]),
+    "tnt_event_destroy": (C.c_int, [VP, VP]),
     "tnt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "tnt_ctx_kernel_stats": (C.c_int, [VP, C.c_int, SZP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnt_ctx_reset_stats": (C.c_int, [VP]),

@@ -47,6 +47,9 @@ struct tnt_ctx {
   // optional per-kernel-class event timing (tnt_ctx_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
+  // stream-ordered partials of tnt_weighted_dot_async
+  void* dot_parts = nullptr;
+  size_t dot_parts_len = 0;
   struct Timed {
     int kind;
     hipEvent_t a, b;
@@ -1090,6 +1093,101 @@ int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   pool_free(ctx, parts);
   *out = *ctx->host_scalar;
+  return TNT_OK;
+}
+
+// Stream-ordered variant: the result lands in out[out_index] on the device;
+// nothing blocks, so it can be queued behind a matvec and captured.
+int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                           const double* weights, size_t n_regions, tnt_buf* out, size_t out_index) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, a, "a")) return rc;
+  if (int rc = check_buf(ctx, b, "b")) return rc;
+  if (int rc = check_buf(ctx, out, "out")) return rc;
+  if (out_index >= out->len) FAIL(ctx, TNT_E_OOB, "result index outside the output buffer");
+  if (n_regions && (!regions || !weights)) FAIL(ctx, TNT_E_INVALID, "null region list");
+  std::vector<int64_t> cb, ce;
+  std::vector<int32_t> cr;
+  std::vector<double> w(weights, weights + n_regions);
+  const int64_t chunk = 1 << 16;
+  for (size_t r = 0; r < n_regions; ++r) {
+    const tnt_region& g = regions[r];
+    if (g.offset < 0 || g.len < 0 || (size_t)(g.offset + g.len) > a->len || (size_t)(g.offset + g.len) > b->len)
+      FAIL(ctx, TNT_E_OOB, "region outside the payloads");
+    for (int64_t s0 = 0; s0 < g.len; s0 += chunk) {
+      cb.push_back(g.offset + s0);
+      ce.push_back(g.offset + std::min(g.len, s0 + chunk));
+      cr.push_back((int32_t)r);
+    }
+  }
+  if (int rc = set_device(ctx)) return rc;
+  double* dst = static_cast<double*>(out->ptr) + out_index;
+  if (cb.empty()) {
+    launch_fill(dst, 1, 0.0, ctx->stream);
+    return TNT_OK;
+  }
+  void *dcb, *dce, *dcr, *dw, *parts;
+  int rc = cached_table(ctx, cb.data(), 8 * cb.size(), "DB", &dcb);
+  if (!rc) rc = cached_table(ctx, ce.data(), 8 * ce.size(), "DE", &dce);
+  if (!rc) rc = cached_table(ctx, cr.data(), 4 * cr.size(), "DR", &dcr);
+  if (!rc) rc = cached_table(ctx, w.data(), 8 * w.size(), "DW", &dw);
+  if (rc) return rc;
+  // partials: one grow-only buffer per context; every user is on the context
+  // stream, so stream order keeps consecutive calls from overlapping in it
+  if (ctx->dot_parts_len < cb.size()) {
+    if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "dot scratch must grow during graph capture; run the call once before capturing");
+    if (ctx->dot_parts) pool_free(ctx, ctx->dot_parts);
+    ctx->dot_parts = nullptr;
+    ctx->dot_parts_len = 0;
+    if ((rc = pool_alloc(ctx, 8 * cb.size(), &ctx->dot_parts))) return rc;
+    ctx->dot_parts_len = cb.size();
+  }
+  parts = ctx->dot_parts;
+  launch_weighted_dot_partials(static_cast<const double*>(a->ptr), static_cast<const double*>(b->ptr),
+                               static_cast<int64_t*>(dcb), static_cast<int64_t*>(dce),
+                               static_cast<int32_t*>(dcr), static_cast<double*>(dw),
+                               static_cast<double*>(parts), (uint32_t)cb.size(), ctx->stream);
+  launch_sum_partials(static_cast<double*>(parts), (uint32_t)cb.size(), dst, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
+struct tnt_event {
+  hipEvent_t e = nullptr;
+};
+
+int tnt_event_create(tnt_ctx* ctx, tnt_event** out) {
+  if (!ctx || !out) return TNT_E_INVALID;
+  if (int rc = set_device(ctx)) return rc;
+  auto* ev = new tnt_event;
+  hipError_t e = hipEventCreateWithFlags(&ev->e, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    delete ev;
+    FAIL(ctx, TNT_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+  }
+  *out = ev;
+  return TNT_OK;
+}
+
+int tnt_event_record(tnt_ctx* ctx, tnt_event* ev) {
+  if (!ctx || !ev) return TNT_E_INVALID;
+  HIPCHK(ctx, hipEventRecord(ev->e, ctx->stream));
+  return TNT_OK;
+}
+
+// 1: everything queued before the record has finished; 0: still running.
+int tnt_event_query(tnt_ctx* ctx, const tnt_event* ev) {
+  if (!ctx || !ev) return TNT_E_INVALID;
+  hipError_t e = hipEventQuery(ev->e);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  FAIL(ctx, TNT_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+}
+
+int tnt_event_destroy(tnt_ctx* ctx, tnt_event* ev) {
+  if (!ctx || !ev) return TNT_E_INVALID;
+  (void)hipEventDestroy(ev->e);
+  delete ev;
   return TNT_OK;
 }
 

@@ -1,0 +1,116 @@
+"""Two-site effective Hamiltonian on the device and a Lanczos ground-state solver.
+
+The reference's Krylov layer (tenet-krylov) is a matrix-free solver over
+``LinearOperator::apply`` and ``KrylovVector::dot_real``
+(tenet-krylov/src/operator.rs:2-15, vector.rs:2-48, cg.rs:64-190); it has no
+Lanczos and no H_eff.  Here ``TwoSiteHeff.apply`` is the operator (a fixed
+chain of four block-sparse contractions over resident environments) and the
+FP64 inner products run on the device (tnt_weighted_dot).
+
+Index conventions (legs of V = bond, P = physical, M = MPO bond):
+    theta in V <- P (x) P (x) V          theta[a; s1, s2, c]
+    L     in V (x) M <- V                L[a', w; a]
+    W     in M (x) P <- P (x) M          W[w, s; s', w']
+    R     in V <- V (x) M                R[c; c', w]
+    X1 = L . theta                       (a', w ; s1, s2, c)
+    X2 = X1 . W1  (w, s1 contracted)     (a', w1 ; s1', s2, c)
+    X3 = X2 . W2  (w1, s2 contracted)    (a', w2 ; s1', s2', c)
+    H theta = X3 . R (w2, c contracted)  (a' ; s1', s2', c')
+"""
+import numpy as np
+
+from .tensor import TensorMap
+from .workloads import mps_env_spaces
+
+
+class TwoSiteHeff:
+    """Environments of one bond dimension, resident in HBM."""
+
+    def __init__(self, rt, rule_name="U1", D=512, seed=7, precision="f64"):
+        self.rt = rt
+        self.rule, V, P, M = mps_env_spaces(rule_name, D)
+        self.V, self.P, self.M = V, P, M
+        self.D = D
+        self.precision = precision
+        self.L = TensorMap.rand(rt, [V, M], [V], seed=seed, normal=True)
+        self.R = TensorMap.rand(rt, [V], [V, M], seed=seed + 1, normal=True)
+        self.W1 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 2, normal=True)
+        self.W2 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 3, normal=True)
+        self.theta_space = TensorMap.zeros(rt, [V], [P, P, V]).space
+
+    def gram(self, theta, scratch=None):
+        """G theta = L^T (L theta): a real symmetric positive semidefinite operator on
+        the theta space built from the same block-sparse contractions (lazy adjoint)."""
+        sc = scratch if scratch is not None else {}
+        x = self.L.contract([2], theta, [0], [0, 1, 2, 3, 4], 2, out=sc.get("g1"), precision=self.precision)
+        sc["g1"] = x
+        y = self.L.adjoint().contract([1, 2], x, [0, 1], [0, 1, 2, 3], 1, out=sc.get("g2"),
+                                      precision=self.precision)
+        sc["g2"] = y
+        return y
+
+    def on(self, rt):
+        """Views of the environments for another runtime (same device, no copy)."""
+        v = object.__new__(TwoSiteHeff)
+        v.__dict__.update(self.__dict__)
+        v.rt = rt
+        for name in ("L", "R", "W1", "W2"):
+            t = getattr(self, name)
+            setattr(v, name, TensorMap(rt, t.space, _buf=t._buf, _base=t))
+        return v
+
+    def apply(self, theta, scratch=None, precision=None):
+        """H_eff theta (four contractions); scratch: dict of reusable intermediates."""
+        prec = precision or self.precision
+        sc = scratch if scratch is not None else {}
+        x1 = self.L.contract([2], theta, [0], [0, 1, 2, 3, 4], 2, out=sc.get("x1"), precision=prec)
+        sc["x1"] = x1
+        x2 = x1.contract([1, 2], self.W1, [3, 1], [0, 3, 4, 1, 2], 2, out=sc.get("x2"), precision=prec)
+        sc["x2"] = x2
+        x3 = x2.contract([1, 3], self.W2, [3, 1], [0, 3, 1, 4, 2], 2, out=sc.get("x3"), precision=prec)
+        sc["x3"] = x3
+        y = x3.contract([1, 4], self.R, [2, 0], [0, 1, 2, 3], 1, out=sc.get("y"), precision=prec)
+        sc["y"] = y
+        if not hasattr(self, "_flops"):
+            rt = self.rt
+            keys = [
+                (self.L.space, theta.space, [2], [0], [0, 1, 2, 3, 4], 2),
+                (x1.space, self.W1.space, [1, 2], [3, 1], [0, 3, 4, 1, 2], 2),
+                (x2.space, self.W2.space, [1, 3], [3, 1], [0, 3, 1, 4, 2], 2),
+                (x3.space, self.R.space, [1, 4], [2, 0], [0, 1, 2, 3], 1),
+            ]
+            self._flops = sum(rt.contraction(*k).flops for k in keys)
+        return y
+
+
+def lanczos(op, v0, iters=30, tol=1e-10):
+    """Lowest eigenvalue of a real symmetric operator by Lanczos with full
+    reorthogonalisation; vectors and FP64 inner products stay on the device.
+
+    op: callable TensorMap -> TensorMap (same space); v0: start vector.
+    Returns (eigenvalue, ritz residual norm, iterations).
+    """
+    q = v0.copy()
+    q.scale_(1.0 / q.norm())
+    Q = [q]
+    alphas, betas = [], []
+    theta_min, res = None, None
+    for k in range(iters):
+        w = op(Q[-1]).copy()
+        a = Q[-1].inner(w)
+        alphas.append(a)
+        # full reorthogonalisation (classical Gram-Schmidt twice)
+        for _ in range(2):
+            for qi in Q:
+                w.axpy_(qi, -qi.inner(w))
+        b = w.norm()
+        T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
+        evals, evecs = np.linalg.eigh(T)
+        theta_min = float(evals[0])
+        res = abs(b * evecs[-1, 0])
+        if res < tol or b < 1e-300:
+            return theta_min, res, k + 1
+        betas.append(b)
+        w.scale_(1.0 / b)
+        Q.append(w)
+    return theta_min, res, iters

@@ -1,0 +1,201 @@
+"""Continuous-batching server for two-site H_eff matvecs (one MI355X).
+
+A request is (arrival time, bond dimension D, seed): build theta (seeded
+normal fill on the device), apply H_eff of its D bucket (four block-sparse
+contractions, split-BF16 GEMMs by default) and take the FP64 Rayleigh
+numerator <theta, H theta> on the device.  Environments (L, R, W) of every D
+bucket stay resident in HBM for the life of the server.
+
+Scheduling (host thread):
+  * arrivals are admitted when a slot is free (at most `max_inflight` in
+    flight) and the HBM footprint of the in-flight requests plus the new
+    one fits the budget -- admission by HBM footprint;
+  * admitted requests are issued round-robin over `n_streams` backend
+    contexts (one HIP stream each, default priority), so requests that arrive
+    while others run overlap on the device;
+  * completion is polled with stream events, freeing slot and footprint,
+    after which waiting arrivals are admitted.
+Every request uses its own buffers and deterministic kernels, so its result
+is bit-for-bit the same alone or in any mix (tests/test_gpu_serving.py).
+Latency = completion seen by the scheduler - arrival.
+"""
+import ctypes as C
+import math
+import time
+from collections import deque
+
+import numpy as np
+
+from . import _native as N
+from .heff import TwoSiteHeff
+from .tensor import Runtime, TensorMap
+
+
+def poisson_trace(n, rate=2000.0, median_d=512, sigma=0.7, dmin=32, dmax=4096, seed=5, buckets=None):
+    """n requests: exponential inter-arrivals at `rate`/s, D ~ lognormal clipped, snapped to buckets."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    d = np.clip(np.exp(rng.normal(math.log(median_d), sigma, size=n)), dmin, dmax)
+    if buckets is None:
+        buckets = default_buckets(dmin, dmax)
+    b = np.array(buckets)
+    snapped = [int(b[np.argmin(np.abs(np.log(b) - math.log(x)))]) for x in d]
+    return [(float(ti), di, 10_000 + i) for i, (ti, di) in enumerate(zip(t, snapped))]
+
+
+def default_buckets(dmin=32, dmax=4096, n=15):
+    """Log-spaced bond dimensions (multiples of 32) the environments are built for."""
+    g = np.exp(np.linspace(math.log(dmin), math.log(dmax), n))
+    return sorted(set(int(max(dmin, 32 * round(x / 32))) for x in g))
+
+
+class _Req:
+    __slots__ = ("rid", "arrival", "D", "seed", "ctx_index", "event", "theta", "y", "scratch",
+                 "footprint", "t_issue", "t_done", "slot")
+
+
+class HeffServer:
+    def __init__(self, rule="U1", buckets=None, n_streams=4, max_inflight=16,
+                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7):
+        self.rule = rule
+        self.precision = precision
+        self.max_inflight = max_inflight
+        self.budget = hbm_budget_bytes
+        self.rts = [Runtime(0, precision=precision) for _ in range(n_streams)]
+        self.buckets = list(buckets or default_buckets())
+        self.env = {}       # D -> TwoSiteHeff on context 0
+        self.views = {}     # (D, ctx index) -> environment views on that context
+        for D in self.buckets:
+            self.env[D] = TwoSiteHeff(self.rts[0], rule, D, seed=env_seed + D, precision=precision)
+        self.rts[0].synchronize()  # the other streams read the environments
+        self.flops = {}
+        self.fp = {}
+        self.results = None
+
+    # -- per-bucket structures (cold: compiled per context once) -------------
+    def warm(self):
+        """Compile every (bucket, context) artifact and size the scratch pools."""
+        for D in self.buckets:
+            for k, rt in enumerate(self.rts):
+                v = self._view(D, k)
+                th = TensorMap.rand(rt, v.theta_space, seed=1, normal=True)
+                sc = {}
+                y = v.apply(th, sc)
+                self.flops[D] = v._flops
+                fp = 8 * (th.space.len + sum(t.space.len for t in sc.values()))
+                # transform scratch of the four artifacts is held per context; count the
+                # per-request buffers (theta, intermediates, result) in the footprint
+                self.fp[D] = fp
+                rt.synchronize()
+                del th, y, sc
+
+    def _view(self, D, k):
+        key = (D, k)
+        if key not in self.views:
+            self.views[key] = self.env[D] if k == 0 else self.env[D].on(self.rts[k])
+        return self.views[key]
+
+    # -- serving loop ----------------------------------------------------------
+    def serve(self, trace, keep_results=False, log=None):
+        """Run a trace [(arrival_s, D, seed)]; returns a stats dict."""
+        h = N.hip()
+        n = len(trace)
+        # one FP64 result per request, written on the device by the request's own stream
+        out_buf = [TensorMap.zeros(rt, self._vector_space(n)) for rt in self.rts]
+        slots = deque(range(self.max_inflight))
+        pending = deque(sorted(enumerate(trace), key=lambda x: x[1][0]))
+        inflight = []
+        used = 0
+        done = []
+        next_ctx = 0
+        results = {}
+        for rt in self.rts:
+            rt.synchronize()
+        t0 = time.perf_counter()
+        while pending or inflight:
+            now = time.perf_counter() - t0
+            # admit arrivals (slot + HBM footprint)
+            while pending and pending[0][1][0] <= now and slots:
+                i, (arr, D, seed) = pending[0]
+                fp = self.fp[D]
+                if used + fp > self.budget and inflight:
+                    break
+                pending.popleft()
+                r = _Req()
+                r.rid, r.arrival, r.D, r.seed = i, arr, D, seed
+                r.footprint = fp
+                r.slot = slots.popleft()
+                r.ctx_index = next_ctx
+                next_ctx = (next_ctx + 1) % len(self.rts)
+                rt = self.rts[r.ctx_index]
+                v = self._view(D, r.ctx_index)
+                r.t_issue = now
+                r.theta = TensorMap.rand(rt, v.theta_space, seed=seed, normal=True)
+                r.scratch = {}
+                r.y = v.apply(r.theta, r.scratch)
+                self._dot_async(rt, r.theta, r.y, out_buf[r.ctx_index], i)
+                ev = C.c_void_p()
+                rt._check(h.tnt_event_create(rt._ctx, C.byref(ev)))
+                rt._check(h.tnt_event_record(rt._ctx, ev))
+                r.event = ev
+                used += fp
+                inflight.append(r)
+            # retire finished requests
+            still = []
+            for r in inflight:
+                rt = self.rts[r.ctx_index]
+                q = h.tnt_event_query(rt._ctx, r.event)
+                if q < 0:
+                    rt._check(q)
+                if q == 1:
+                    r.t_done = time.perf_counter() - t0
+                    if keep_results:
+                        results[(r.rid, "y")] = r.y.to_numpy()
+                    h.tnt_event_destroy(rt._ctx, r.event)
+                    used -= r.footprint
+                    slots.append(r.slot)
+                    done.append(r)
+                    r.theta = r.y = r.scratch = None
+                else:
+                    still.append(r)
+            inflight = still
+            if not inflight and pending and pending[0][1][0] > now:
+                time.sleep(min(1e-4, pending[0][1][0] - now))
+        wall = time.perf_counter() - t0
+        for k, rt in enumerate(self.rts):
+            vals = out_buf[k].to_numpy()
+            for r in done:
+                if r.ctx_index == k:
+                    results[r.rid] = float(vals[r.rid])
+        lat = np.array([r.t_done - r.arrival for r in done])
+        flops = sum(self.flops[r.D] for r in done)
+        self.results = results
+        return {
+            "requests": n,
+            "wall_s": wall,
+            "matvecs_per_s": n / wall,
+            "tflops": flops / wall / 1e12,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3),
+            "offered_rate": n / max(trace[-1][0], 1e-9),
+        }
+
+    # -- helpers -----------------------------------------------------------------
+    @staticmethod
+    def _vector_space(n):
+        from .tensor import GradedSpace, HomSpace, Rule
+        rule = Rule("U1")
+        return HomSpace(rule, [GradedSpace(rule, [(0, max(1, n))])], [GradedSpace(rule, [(0, 1)])])
+
+    def _dot_async(self, rt, a, b, out, slot):
+        secs = a.space.sectors
+        regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * k) for _, o, r, k, _ in secs])
+        ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
+        rt._check(N.hip().tnt_weighted_dot_async(rt._ctx, a._buf, b._buf, regs, ws, len(secs), out._buf, slot))
+
+    def close(self):
+        self.env.clear()
+        self.views.clear()
+        for rt in self.rts:
+            rt.close()

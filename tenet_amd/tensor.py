@@ -241,8 +241,10 @@ class _Transform:
             pass
 
 
-class _Contraction:
-    def __init__(self, rt, a, b, axes_a, axes_b, out_perm, out_nout, adj_a, adj_b):
+class _HostContraction:
+    """Host-compiled contraction structure; shared by every Runtime (device-independent)."""
+
+    def __init__(self, a, b, axes_a, axes_b, out_perm, out_nout, adj_a, adj_b):
         h = N.host()
         out = C.c_void_p()
         N.check_host(h.tnth_contract_compile(
@@ -252,6 +254,31 @@ class _Contraction:
         self.dst_space = HomSpace(a.rule, None, None, _handle=h.tnth_contract_dst_space(out), _owner=self)
         self.flops = float(h.tnth_contract_flops(out))
         self.desc = h.tnth_contract_desc(out)
+
+    def __del__(self):
+        try:
+            N.host().tnth_contract_destroy(self._h)
+        except Exception:
+            pass
+
+
+_HOST_CONTRACTIONS = {}
+
+
+def _host_contraction(key, *args):
+    hc = _HOST_CONTRACTIONS.get(key)
+    if hc is None:
+        hc = _HostContraction(*args)
+        _HOST_CONTRACTIONS[key] = hc
+    return hc
+
+
+class _Contraction:
+    def __init__(self, rt, host):
+        self.host = host
+        self.dst_space = host.dst_space
+        self.flops = host.flops
+        self.desc = host.desc
         art = C.c_void_p()
         rt._check(N.hip().tnt_contract_upload(rt._ctx, self.desc, C.byref(art)))
         self.art = art
@@ -260,7 +287,6 @@ class _Contraction:
     def __del__(self):
         try:
             N.hip().tnt_contract_destroy(self.rt._ctx, self.art)
-            N.host().tnth_contract_destroy(self._h)
         except Exception:
             pass
 
@@ -307,7 +333,8 @@ class Runtime:
         key = (a.key(), b.key(), tuple(axes_a), tuple(axes_b), tuple(out_perm), out_nout, adj_a, adj_b)
         c = self._contractions.get(key)
         if c is None:
-            c = _Contraction(self, a, b, list(axes_a), list(axes_b), list(out_perm), out_nout, adj_a, adj_b)
+            host = _host_contraction(key, a, b, list(axes_a), list(axes_b), list(out_perm), out_nout, adj_a, adj_b)
+            c = _Contraction(self, host)
             self._contractions[key] = c
         return c
 

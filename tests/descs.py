@@ -98,3 +98,10 @@ def contract_desc_replay(cdesc, lhs, rhs, core_len):
     if d.out:
         return replay_desc(d.out, core)
     return core
+
+
+def ospace(hs):
+    """Oracle HomSpace with the same legs as a product HomSpace (ids are shared)."""
+    orule = make_rule(hs.rule.name)
+    mk = lambda g: OLeg(list(zip(g.ids, g.degs)), g.is_dual)
+    return OHomSpace(orule, [mk(g) for g in hs.codomain], [mk(g) for g in hs.domain])
