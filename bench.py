@@ -156,7 +156,8 @@ def run_serve(args):
     stats = srv.serve(trace)
     log(f"[bench] serve: {stats['requests']} requests in {stats['wall_s']:.2f}s "
         f"(offered {stats['offered_rate']:.0f}/s): {stats['matvecs_per_s']:.0f} matvecs/s, "
-        f"p50 {stats['p50_ms']:.3f} ms, p99 {stats['p99_ms']:.3f} ms, {stats['tflops']:.2f} TFLOP/s")
+        f"p50 {stats['p50_ms']:.3f} ms, p99 {stats['p99_ms']:.3f} ms, {stats['tflops']:.2f} TFLOP/s; "
+        f"host issue {stats['host_issue_ms_per_request']:.3f} ms/request, peak in flight {stats['max_inflight']}")
     cpu = None
     if args.cpu_baseline == "on":
         rate, n = cpu_serve_baseline(args.rule, trace)
@@ -191,6 +192,7 @@ def run_serve(args):
         "latency_ms": {"p50": stats["p50_ms"], "p99": stats["p99_ms"], "mean": stats["mean_ms"]},
         "offered_rate": stats["offered_rate"],
         "tflops": stats["tflops"],
+        "host_issue_ms_per_request": stats["host_issue_ms_per_request"],
         "cpu_baseline": cpu,
         "cold_compile_s": cold,
     }

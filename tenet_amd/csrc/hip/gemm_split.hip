@@ -2,24 +2,32 @@
 //
 // FP64 C = op(A) op(B) from BF16 matrix-core products:
 //   1. per row i of op(A) an exponent e_i with max_k |A_ik| < 2^e_i, per
-//      column j of op(B) an exponent f_j (pre-pass);
+//      column j of op(B) an exponent f_j;
 //   2. a = A_ik 2^-e_i in (-1, 1) is cut into S base-256 digits by repeated
 //      round-to-nearest: d_1 in [-256, 256], d_p in [-128, 128] for p > 1 --
-//      small integers, exact in BF16 (pre-pass, written as S BF16 planes,
-//      k-contiguous per row / column, zero-padded to a multiple of 32);
-//   3. C_ij = 2^(e_i+f_j) sum_{p+q<=S+1} 2^-8(p+q) sum_k dA_p,ik dB_q,kj.
-//      Products of one level l = p+q go to one FP32 accumulator.  Within a
-//      K chunk of 128 every partial sum is an integer of magnitude
-//      < 1.75 * 2^16 * 128 < 2^24, so the FP32 MFMA accumulation is exact;
-//      after each chunk the level sums are scaled by 2^-8l and added in FP64.
-// The only rounding is the truncation of the digit expansion (relative
-// 2^-8S per operand) and the dropped levels l > S+1; S = 5 gives relative
-// Frobenius errors around 1e-12 on random data (tests hold it to 1e-10).
+//      small integers, exact in BF16.  Plane p stores d_p 2^-8(p-1) (an exact
+//      exponent shift), so a = 2^(e_i - 8) sum_p plane_p;
+//   3. C_ij = 2^(e_i + f_j - 16) sum_{p+q <= S-1 (0-based)} plane_p(A) plane_q(B).
+//      Products are summed on v_mfma_f32_16x16x32_bf16 into FP32 accumulator
+//      groups by level l = p + q: l = 0 and l = 1 each have their own group;
+//      within a K window of 128 every partial sum of those two groups is a
+//      multiple of 2^-8l below 2^(23-8l), so their FP32 accumulation is exact.
+//      All remaining levels share one group (two for S = 6); their FP32
+//      rounding is 2^-24 relative to a level-2 sum, i.e. about 2^-40 of the
+//      result.  After each window the groups are added into FP64.
+// Truncating the digit expansion (relative 2^-8S per operand) and dropping
+// the levels l > S-1 are the only other errors: S = 5 gives relative
+// Frobenius errors of a few 1e-12 on normal data.
 //
-// MFMA: v_mfma_f32_16x16x32_bf16.  Both operands use the same k-to-lane
-// assignment (k = 8*(lane/16) + 0..7 within a 32-wide step), so the result
-// does not depend on the hardware's internal k order; rows/cols map to
-// lane % 16 and D follows the 16x16 layout (row 4*(lane/16)+r, col lane%16).
+// Planes are stored in MFMA fragment order: one fragment = 16 lines x 32 k
+// (1 KiB), lane l holding line l%16, k = 8(l/16)..8(l/16)+7, the S planes of
+// one (line block, k step) adjacent.  The GEMM copies fragments straight into
+// LDS with global_load_lds_dwordx4 (no staging registers), double-buffered in
+// two distinct LDS arrays so that the wait for one stage does not wait for the
+// prefetch of the next, and every wave reads its operands with one
+// conflict-free ds_read_b128 per fragment.  Both operands use the same
+// k-to-lane assignment, so the result does not depend on the hardware's
+// internal k order; D follows the 16x16 layout (row 4(lane/16)+r, col lane%16).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,28 +43,32 @@ namespace tnt_dev {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int SPLIT_KS = 32;     // k per LDS stage (one MFMA k-step)
-constexpr int SPLIT_CHUNK = 128; // exact FP32 accumulation window
-constexpr int SPLIT_BM = 64, SPLIT_BN = 64;
-constexpr int SPLIT_LDS_PAD = 8;  // bf16 elements of row padding
+constexpr int FRAG = 512;         // bf16 per fragment (16 lines x 32 k)
+constexpr int KSTEP = 32;         // k per fragment / MFMA step
+constexpr int WINDOW_STEPS = 4;   // 128 k per exact FP32 window
 
 struct SplitJob {
   int64_t d_off;     // element offset of C from dbase
-  int64_t a_row0;    // plane line base: row i of this job starts at (a_row0 + i) * kp
-  int64_t b_col0;    // plane line base of the columns of op(B)
-  int64_t a_exp0;    // first row of this job in the A exponent array
-  int64_t b_exp0;    // first column of this job in the B exponent array
-  int32_t m, n, kp;  // kp = k rounded up to SPLIT_KS
+  int64_t fa0, fb0;  // first fragment of this job's A / B planes
+  int64_t ea0, eb0;  // first exponent of this job's A rows / B columns
+  int32_t m, n, ns;  // ns = number of 32-k steps
   int32_t ldd;
+  int32_t lba, lbb;  // line blocks of 16 (A rows, B columns)
+  int32_t cfg, pad;
 };
+
+// Tile shapes: wave tile (16 WM) x (16 WN), 2 x 2 waves per workgroup.
+enum SplitShape { SPLIT_64x64 = 0, SPLIT_128x64 = 1, SPLIT_64x128 = 2, SPLIT_SHAPES = 3 };
+constexpr int shape_wm(int c) { return c == SPLIT_128x64 ? 4 : 2; }
+constexpr int shape_wn(int c) { return c == SPLIT_64x128 ? 4 : 2; }
 
 struct SplitTables {
   SplitJob* jobs = nullptr;
   GTile* tiles = nullptr;
-  int64_t* prefix = nullptr;   // per-job start of its workgroups in the flat pre-pass grids
-  uint32_t n_tiles = 0;
-  uint32_t exp_blocks_a = 0, exp_blocks_b = 0, slice_blocks_a = 0, slice_blocks_b = 0;
-  int64_t a_elems = 0, b_elems = 0, a_rows = 0, b_cols = 0;
+  int64_t* prefix = nullptr;   // per-side per-job start of its line blocks in the pre-pass grid
+  uint32_t tile_start[SPLIT_SHAPES] = {0, 0, 0}, tile_count[SPLIT_SHAPES] = {0, 0, 0};
+  uint32_t blocks_a = 0, blocks_b = 0;
+  int64_t frags_a = 0, frags_b = 0, a_rows = 0, b_cols = 0;
 };
 
 struct SplitWorkspace {
@@ -74,9 +86,10 @@ SplitWorkspace* split_workspace_create() { return new SplitWorkspace(); }
 void split_workspace_destroy(SplitWorkspace* ws) { delete ws; }
 
 // ---------------------------------------------------------------------------
-// Pre-passes.  Both run over a flat grid; a workgroup finds its job by a
-// binary search of the per-job prefix of workgroup counts.
+// Pre-pass: one workgroup per (job, block of 16 lines), over all of K.
 //   element (line, kk): side 0 -> op(A)[line][kk], side 1 -> op(B)[kk][line]
+// Pass 1 takes the line maxima (coalesced along whichever index is
+// contiguous in the source), pass 2 cuts the digits and writes fragments.
 // ---------------------------------------------------------------------------
 struct PrepArgs {
   const GJob* jobs;
@@ -85,10 +98,8 @@ struct PrepArgs {
   int n_jobs;
   uintptr_t base;         // operand base (A or B)
   int32_t* exps;          // exponents of this side
-  __bf16* planes;         // planes of this side
-  int64_t plane_stride;
+  __bf16* planes;         // fragments of this side
   int side;
-  int slices;
 };
 
 __device__ __forceinline__ int find_job(const int64_t* prefix, int n, int64_t b) {
@@ -110,81 +121,68 @@ __device__ __forceinline__ int64_t elem_index(const GJob& g, int side, int line,
   return (g.ops & 2) ? (line + (int64_t)kk * g.ldb) : (kk + (int64_t)line * g.ldb);
 }
 
-// Exponent per line: 256 lines per workgroup (line-contiguous sources, one
-// thread per line) or 4 lines per workgroup (k-contiguous, one wave per line).
-__global__ __launch_bounds__(256) void split_exponents(PrepArgs x) {
+template <int S>
+__global__ __launch_bounds__(256) void split_prepare(PrepArgs x) {
+  __shared__ double red[16][17];
+  __shared__ int sexp[16];
   const int j = find_job(x.prefix, x.n_jobs, blockIdx.x);
   const GJob g = x.jobs[j];
   const SplitJob s = x.sj[j];
-  const int64_t local = blockIdx.x - x.prefix[j];
-  const int lines = x.side == 0 ? g.m : g.n;
-  const int64_t exp0 = x.side == 0 ? s.a_exp0 : s.b_exp0;
-  const double* M = reinterpret_cast<const double*>(x.base) + (x.side == 0 ? g.a_off : g.b_off);
-  double mx = 0.0;
-  int line;
-  if (line_contiguous(g, x.side)) {
-    line = (int)(local * 256 + threadIdx.x);
-    if (line >= lines) return;
-    for (int kk = 0; kk < g.k; ++kk) mx = fmax(mx, fabs(M[elem_index(g, x.side, line, kk)]));
-  } else {
-    const int lane = threadIdx.x & 63;
-    line = (int)(local * 4 + (threadIdx.x >> 6));
-    if (line >= lines) return;
-    for (int kk = lane; kk < g.k; kk += 64) mx = fmax(mx, fabs(M[elem_index(g, x.side, line, kk)]));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
-    if (lane != 0) return;
-  }
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1): |x| < 2^e
-  x.exps[exp0 + line] = e;
-}
-
-// Digit planes: one 64x64 (line x k) tile per workgroup, staged through LDS
-// so that both the FP64 read and the BF16 writes are coalesced.
-__global__ __launch_bounds__(256) void split_slices(PrepArgs x) {
-  __shared__ double T[64][65];
-  const int j = find_job(x.prefix, x.n_jobs, blockIdx.x);
-  const GJob g = x.jobs[j];
-  const SplitJob s = x.sj[j];
-  const int64_t local = blockIdx.x - x.prefix[j];
-  const int lines = x.side == 0 ? g.m : g.n;
-  const int ktiles = (s.kp + 63) / 64;
-  const int l0 = (int)(local / ktiles) * 64, k0 = (int)(local % ktiles) * 64;
-  const int64_t line0 = x.side == 0 ? s.a_row0 : s.b_col0;
-  const int64_t exp0 = x.side == 0 ? s.a_exp0 : s.b_exp0;
-  const double* M = reinterpret_cast<const double*>(x.base) + (x.side == 0 ? g.a_off : g.b_off);
+  const int lb = (int)(blockIdx.x - x.prefix[j]);
+  const int side = x.side;
+  const int lines = side == 0 ? g.m : g.n;
+  const int K = g.k;
+  const double* M = reinterpret_cast<const double*>(x.base) + (side == 0 ? g.a_off : g.b_off);
   const int t = threadIdx.x;
-  const bool lc = line_contiguous(g, x.side);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    int li, ki;
-    if (lc) { li = t & 63; ki = (t >> 6) + 4 * r; }
-    else { ki = t & 63; li = (t >> 6) + 4 * r; }
-    const int line = l0 + li, kk = k0 + ki;
-    double v = 0.0;
-    if (line < lines && kk < g.k) v = ldexp(M[elem_index(g, x.side, line, kk)], -x.exps[exp0 + line]);
-    T[li][ki] = v;
+  {
+    const bool lc = line_contiguous(g, side);
+    const int li = lc ? (t & 15) : (t >> 4), k0 = lc ? (t >> 4) : (t & 15);
+    const int line = lb * 16 + li;
+    double mx = 0.0;
+    if (line < lines)
+      for (int kk = k0; kk < K; kk += 16) mx = fmax(mx, fabs(M[elem_index(g, side, line, kk)]));
+    red[li][k0] = mx;
   }
   __syncthreads();
+  if (t < 16) {
+    double mx = 0.0;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int ki = t & 63, li = (t >> 6) + 4 * r;
-    const int line = l0 + li, kk = k0 + ki;
-    if (line >= lines || kk >= s.kp) continue;
-    double v = T[li][ki];
-    __bf16* out = x.planes + (line0 + line) * (int64_t)s.kp + kk;
-    for (int p = 0; p < x.slices; ++p) {
-      v *= 256.0;
-      const double d = rint(v);
-      v -= d;
-      out[p * x.plane_stride] = (__bf16)(float)d;
+    for (int c = 0; c < 16; ++c) mx = fmax(mx, red[t][c]);
+    int e = 0;
+    if (mx > 0.0) frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |x| < 2^e
+    sexp[t] = e;
+    if (lb * 16 + t < lines) x.exps[(side == 0 ? s.ea0 : s.eb0) + lb * 16 + t] = e;
+  }
+  __syncthreads();
+  const int lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int line = lb * 16 + li;
+  const int e = sexp[li];
+  const int64_t f0 = (side == 0 ? s.fa0 : s.fb0) + (int64_t)lb * s.ns * S;
+  for (int st = wave; st < s.ns; st += 4) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kk = st * KSTEP + kg * 8 + q;
+      v[q] = (line < lines && kk < K) ? ldexp(M[elem_index(g, side, line, kk)], -e) : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      bf16x8 o;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v[q] *= 256.0;
+        const double d = rint(v[q]);
+        v[q] -= d;
+        o[q] = (__bf16)(float)ldexp(d, -8 * p);
+      }
+      *reinterpret_cast<bf16x8*>(x.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Split GEMM kernel: 64x64 tile per workgroup, 4 waves (2x2) of 32x32.
+// Split GEMM kernel
 // ---------------------------------------------------------------------------
 struct SplitGemmArgs {
   const SplitJob* sj;
@@ -194,105 +192,180 @@ struct SplitGemmArgs {
   const __bf16* b_planes;
   const int32_t* a_exp;
   const int32_t* b_exp;
-  int64_t a_stride, b_stride;  // plane strides (elements)
   double alpha, beta;
 };
 
-template <int S>
-__global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemmArgs g) {
-  constexpr int LDK = SPLIT_KS + SPLIT_LDS_PAD;
-  __shared__ __attribute__((aligned(16))) __bf16 As[S][SPLIT_BM][LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[S][SPLIT_BN][LDK];
-  const GTile tile = g.tiles[blockIdx.x];
-  const SplitJob job = g.sj[tile.job];
-  const int m0 = tile.tm * SPLIT_BM, n0 = tile.tn * SPLIT_BN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+__device__ __forceinline__ void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-  f32x4 acc[S][2][2];
-  double acc64[2][2][4];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[i][j][r] = 0.0;
+// 16 bytes per lane from global memory straight into LDS at lds + 16 * lane.
+// Written as asm so that the compiler's wait insertion does not treat every
+// later LDS read as depending on every copy in flight (it cannot tell the two
+// stage buffers apart across this many copies and would wait for all of
+// them, prefetch included); the kernel waits with explicit vmcnt instead.
+// The copy has no register result, so nothing is left in flight in a VGPR.
+__device__ __forceinline__ void copy_to_lds(const void* src, const void* lds) {
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)lds);
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(m0)
+               : "memory", "m0");
+}
 
-  const int nsteps = job.kp / SPLIT_KS;
-  const int steps_per_chunk = SPLIT_CHUNK / SPLIT_KS;
-  for (int step = 0; step < nsteps; ++step) {
-    if (step % steps_per_chunk == 0) {
+// s_waitcnt with only vmcnt constrained (expcnt and lgkmcnt at their maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int S, int WM, int WN>
+struct SplitKernel {
+  static constexpr int NACC = S >= 6 ? 4 : 3;
+  static constexpr int LBM = 2 * WM, LBN = 2 * WN;     // line blocks per workgroup tile
+  static constexpr int FA = LBM * S, FB = LBN * S;     // fragments per stage
+  static constexpr int PER_WAVE = (FA + FB) / 4;
+  static_assert((FA + FB) % 4 == 0, "fragments per stage must split over 4 waves");
+
+  struct Ctx {
+    const __bf16* ap;  // first fragment of this tile's A line blocks at step 0
+    const __bf16* bp;
+    int64_t astep_a, astep_b;  // elements between consecutive line blocks
+    int ns;
+  };
+
+  // Issue this wave's share of one stage's fragments (always PER_WAVE loads).
+  static __device__ __forceinline__ void load_stage(const SplitGemmArgs& g, const SplitJob& job, int tm, int tn,
+                                                    int step, bf16x8 (&As)[FA][64], bf16x8 (&Bs)[FB][64],
+                                                    int wave, int lane) {
 #pragma unroll
-      for (int l = 0; l < S; ++l)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[l][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-    // stage S planes of a 64x32 A slab and a 64x32 B slab: 16-byte vectors,
-    // 4 per row, 64 rows, S planes -> S*256 vectors per operand
-    const int k0 = step * SPLIT_KS;
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-      const int row = tid >> 2, seg = (tid & 3) * 8;
-      const int gi = m0 + row, gj = n0 + row;
-      uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
-      if (gi < job.m)
-        va = *reinterpret_cast<const uint4*>(g.a_planes + p * g.a_stride + (job.a_row0 + gi) * (int64_t)job.kp + k0 + seg);
-      if (gj < job.n)
-        vb = *reinterpret_cast<const uint4*>(g.b_planes + p * g.b_stride + (job.b_col0 + gj) * (int64_t)job.kp + k0 + seg);
-      *reinterpret_cast<uint4*>(&As[p][row][seg]) = va;
-      *reinterpret_cast<uint4*>(&Bs[p][row][seg]) = vb;
-    }
-    __syncthreads();
-    bf16x8 bf[S][2];
-#pragma unroll
-    for (int q = 0; q < S; ++q)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bf[q][j] = *reinterpret_cast<const bf16x8*>(&Bs[q][wn + j * 16 + r16][kq]);
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-      bf16x8 af[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[p][wm + i * 16 + r16][kq]);
-#pragma unroll
-      for (int q = 0; q < S - p; ++q) {  // level (p+1)+(q+1) <= S+1
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acc[p + q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[q][j], acc[p + q][i][j], 0, 0, 0);
-      }
-    }
-    if ((step + 1) % steps_per_chunk == 0 || step + 1 == nsteps) {
-#pragma unroll
-      for (int l = 0; l < S; ++l) {
-        const double sc = ldexp(1.0, -8 * (l + 2));
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc64[i][j][r] = fma((double)acc[l][i][j][r], sc, acc64[i][j][r]);
+    for (int r = 0; r < PER_WAVE; ++r) {
+      const int f = wave + 4 * r;
+      if (f < FA) {
+        const int i = f / S, p = f % S;
+        const int lb = min(tm * LBM + i, job.lba - 1);  // clamp: rows past m are discarded
+        const __bf16* src = g.a_planes + ((job.fa0 + ((int64_t)lb * job.ns + step) * S + p) * FRAG + lane * 8);
+        copy_to_lds(src, &As[f][0]);
+      } else {
+        const int fb = f - FA;
+        const int jb = fb / S, q = fb % S;
+        const int lb = min(tn * LBN + jb, job.lbb - 1);
+        const __bf16* src = g.b_planes + ((job.fb0 + ((int64_t)lb * job.ns + step) * S + q) * FRAG + lane * 8);
+        copy_to_lds(src, &Bs[fb][0]);
       }
     }
   }
-  // epilogue
+
+  static __device__ __forceinline__ void compute(bf16x8 (&As)[FA][64], bf16x8 (&Bs)[FB][64], int wm, int wn,
+                                                 int lane, f32x4 (&acc)[NACC][WM][WN]) {
+    bf16x8 bq[S][WN];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) bq[q][j] = Bs[(wn * WN + j) * S + q][lane];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      bf16x8 ap[WM];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) ap[i] = As[(wm * WM + i) * S + p][lane];
+#pragma unroll
+      for (int q = 0; q < S - p; ++q) {
+        constexpr int last = NACC - 1;
+        const int grp = (p + q) < last ? (p + q) : last;
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+          for (int j = 0; j < WN; ++j)
+            acc[grp][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i], bq[q][j], acc[grp][i][j], 0, 0, 0);
+      }
+    }
+  }
+
+  static __device__ __forceinline__ void zero(f32x4 (&acc)[NACC][WM][WN]) {
+#pragma unroll
+    for (int l = 0; l < NACC; ++l)
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[l][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+
+  static __device__ __forceinline__ void flush(f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4]) {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double s = 0.0;
+#pragma unroll
+          for (int l = NACC - 1; l >= 0; --l) s += (double)acc[l][i][j][r];
+          acc64[i][j][r] += s;
+        }
+  }
+
+  // One k step: prefetch the next stage into (An, Bn), wait for (Ac, Bc), compute.
+  static __device__ __forceinline__ void step(const SplitGemmArgs& g, const SplitJob& job, int tm, int tn, int s,
+                                              bf16x8 (&Ac)[FA][64], bf16x8 (&Bc)[FB][64],
+                                              bf16x8 (&An)[FA][64], bf16x8 (&Bn)[FB][64], int wave, int lane,
+                                              f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4]) {
+    if (s + 1 < job.ns) {
+      load_stage(g, job, tm, tn, s + 1, An, Bn, wave, lane);
+      wait_vmcnt<PER_WAVE>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    wg_barrier();
+    if (s % WINDOW_STEPS == 0) zero(acc);
+    compute(Ac, Bc, wave & 1, wave >> 1, lane, acc);
+    if (s % WINDOW_STEPS == WINDOW_STEPS - 1 || s + 1 == job.ns) flush(acc, acc64);
+    wg_barrier();  // every wave is done with (Ac, Bc) before they are refilled
+  }
+};
+
+template <int S, int WM, int WN>
+__global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemmArgs g) {
+  using K = SplitKernel<S, WM, WN>;
+  __shared__ bf16x8 A0[K::FA][64];
+  __shared__ bf16x8 B0[K::FB][64];
+  __shared__ bf16x8 A1[K::FA][64];
+  __shared__ bf16x8 B1[K::FB][64];
+  const GTile tile = g.tiles[blockIdx.x];
+  const SplitJob job = g.sj[tile.job];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  f32x4 acc[K::NACC][WM][WN];
+  double acc64[WM][WN][4];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[i][j][r] = 0.0;
+  K::zero(acc);
+
+  K::load_stage(g, job, tile.tm, tile.tn, 0, A0, B0, wave, lane);
+  for (int s = 0; s < job.ns; s += 2) {
+    K::step(g, job, tile.tm, tile.tn, s, A0, B0, A1, B1, wave, lane, acc, acc64);
+    if (s + 1 < job.ns) K::step(g, job, tile.tm, tile.tn, s + 1, A1, B1, A0, B0, wave, lane, acc, acc64);
+  }
+
+  // epilogue: C = alpha 2^(e_i + f_j - 16) acc64 (+ beta C)
   double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const int m0 = tile.tm * K::LBM * 16 + wm * WM * 16, n0 = tile.tn * K::LBN * 16 + wn * WN * 16;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int col = n0 + wn + j * 16 + r16;
+  for (int j = 0; j < WN; ++j) {
+    const int col = n0 + j * 16 + (lane & 15);
     if (col >= job.n) continue;
-    const int fcol = g.b_exp[job.b_exp0 + col];
+    const int fcol = g.b_exp[job.eb0 + col] - 16;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < WM; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int row = m0 + i * 16 + (lane >> 4) * 4 + r;
         if (row < job.m) {
-          const double v = g.alpha * ldexp(acc64[i][j][r], g.a_exp[job.a_exp0 + row] + fcol);
+          const double v = g.alpha * ldexp(acc64[i][j][r], g.a_exp[job.ea0 + row] + fcol);
           double* p = D + row + (int64_t)col * job.ldd;
           *p = (g.beta != 0.0) ? v + g.beta * *p : v;
         }
@@ -309,14 +382,26 @@ int split_set_slices(int slices) {
   return TNT_OK;
 }
 
-static void launch_split_gemm(int S, const SplitGemmArgs& a, uint32_t n, hipStream_t s) {
-  switch (S) {
-    case 2: hipLaunchKernelGGL(split_gemm_kernel<2>, dim3(n), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(split_gemm_kernel<3>, dim3(n), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(split_gemm_kernel<4>, dim3(n), dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(split_gemm_kernel<6>, dim3(n), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL(split_gemm_kernel<5>, dim3(n), dim3(256), 0, s, a); break;
+template <int S>
+static void launch_split_s(const SplitTables& t, const SplitGemmArgs& a0, hipStream_t s) {
+  SplitGemmArgs a = a0;
+  for (int c = 0; c < SPLIT_SHAPES; ++c) {
+    if (!t.tile_count[c]) continue;
+    a.tiles = t.tiles + t.tile_start[c];
+    const dim3 grid(t.tile_count[c]);
+    if constexpr (S >= 6) {  // the wide shapes would spill at six planes
+      hipLaunchKernelGGL((split_gemm_kernel<S, 2, 2>), grid, dim3(256), 0, s, a);
+    } else {
+      if (c == SPLIT_64x64) hipLaunchKernelGGL((split_gemm_kernel<S, 2, 2>), grid, dim3(256), 0, s, a);
+      else if (c == SPLIT_128x64) hipLaunchKernelGGL((split_gemm_kernel<S, 4, 2>), grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((split_gemm_kernel<S, 2, 4>), grid, dim3(256), 0, s, a);
+    }
   }
+}
+
+template <int S>
+static void launch_prepare_s(const PrepArgs& pa, uint32_t blocks, hipStream_t s) {
+  if (blocks) hipLaunchKernelGGL(split_prepare<S>, dim3(blocks), dim3(256), 0, s, pa);
 }
 
 #define SPLIT_HIP(expr)                                                    \
@@ -327,6 +412,14 @@ static void launch_split_gemm(int S, const SplitGemmArgs& a, uint32_t n, hipStre
       return _e == hipErrorOutOfMemory ? TNT_E_OOM : TNT_E_HIP;            \
     }                                                                      \
   } while (0)
+
+// Tile shape of one job: the 64-wide shapes only where the job fills them.
+static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
+  if (S >= 6 || total_64x64_tiles < 512) return SPLIT_64x64;  // small launches want more workgroups
+  if (j.m >= 128 && j.m >= j.n) return SPLIT_128x64;
+  if (j.n >= 128) return SPLIT_64x128;
+  return SPLIT_64x64;
+}
 
 int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
                    const std::vector<GTile>& /*host_tiles*/, const GTile* /*dev_tiles*/, uintptr_t d,
@@ -344,68 +437,49 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     }
     SplitTables t;
     std::vector<SplitJob> sj(host_jobs.size());
-    std::vector<GTile> tiles;
-    std::vector<int64_t> pre_ea{0}, pre_eb{0}, pre_sa{0}, pre_sb{0};
+    std::vector<GTile> per[SPLIT_SHAPES];
+    std::vector<int64_t> pre_a{0}, pre_b{0};
+    int64_t small_tiles = 0;
+    for (const GJob& j : host_jobs) small_tiles += (int64_t)((j.m + 63) / 64) * ((j.n + 63) / 64);
     for (size_t i = 0; i < host_jobs.size(); ++i) {
       const GJob& j = host_jobs[i];
       SplitJob x{};
       x.d_off = j.d_off;
       x.m = j.m;
       x.n = j.n;
-      x.kp = ((std::max(j.k, 1) + SPLIT_KS - 1) / SPLIT_KS) * SPLIT_KS;
+      x.ns = (std::max(j.k, 1) + KSTEP - 1) / KSTEP;
       x.ldd = j.ldd;
-      // planes are [line][kp] per job, concatenated: row0 counts elements / kp
-      // is not uniform across jobs, so store element offsets and rows separately
-      x.a_row0 = t.a_elems;  // element offset (converted below)
-      x.b_col0 = t.b_elems;
-      t.a_elems += (int64_t)j.m * x.kp;
-      t.b_elems += (int64_t)j.n * x.kp;
+      x.lba = (j.m + 15) / 16;
+      x.lbb = (j.n + 15) / 16;
+      x.fa0 = t.frags_a;
+      x.fb0 = t.frags_b;
+      t.frags_a += (int64_t)x.lba * x.ns * S;
+      t.frags_b += (int64_t)x.lbb * x.ns * S;
+      x.ea0 = t.a_rows;
+      x.eb0 = t.b_cols;
+      t.a_rows += j.m;
+      t.b_cols += j.n;
+      x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
       sj[i] = x;
-      bool a_lc = !(j.ops & 1), b_lc = (j.ops & 2) != 0;
-      pre_ea.push_back(pre_ea.back() + (a_lc ? (j.m + 255) / 256 : (j.m + 3) / 4));
-      pre_eb.push_back(pre_eb.back() + (b_lc ? (j.n + 255) / 256 : (j.n + 3) / 4));
-      int kt = (x.kp + 63) / 64;
-      pre_sa.push_back(pre_sa.back() + (int64_t)((j.m + 63) / 64) * kt);
-      pre_sb.push_back(pre_sb.back() + (int64_t)((j.n + 63) / 64) * kt);
-      for (int tm = 0; tm < (j.m + SPLIT_BM - 1) / SPLIT_BM; ++tm)
-        for (int tn = 0; tn < (j.n + SPLIT_BN - 1) / SPLIT_BN; ++tn) tiles.push_back({(int32_t)i, tm, tn, 0});
+      pre_a.push_back(pre_a.back() + (j.n > 0 ? x.lba : 0));
+      pre_b.push_back(pre_b.back() + (j.m > 0 ? x.lbb : 0));
+      if (j.m == 0 || j.n == 0) continue;
+      const int bm = 16 * 2 * shape_wm(x.cfg), bn = 16 * 2 * shape_wn(x.cfg);
+      for (int tm = 0; tm < (j.m + bm - 1) / bm; ++tm)
+        for (int tn = 0; tn < (j.n + bn - 1) / bn; ++tn) per[x.cfg].push_back({(int32_t)i, tm, tn, x.cfg});
     }
-    // Exponents and planes are indexed by a per-side "line" counter; planes
-    // address line * kp, so give every job a line base with base * kp equal
-    // to its element offset: round the element offset up to a multiple of kp.
-    int64_t ae = 0, be = 0, ar = 0, bc = 0;
-    std::vector<int64_t> exp_row(sj.size()), exp_col(sj.size());
-    for (size_t i = 0; i < sj.size(); ++i) {
-      SplitJob& x = sj[i];
-      ae = (ae + x.kp - 1) / x.kp * x.kp;
-      be = (be + x.kp - 1) / x.kp * x.kp;
-      x.a_row0 = ae / x.kp;
-      x.b_col0 = be / x.kp;
-      ae += (int64_t)x.m * x.kp;
-      be += (int64_t)x.n * x.kp;
-      exp_row[i] = ar;
-      exp_col[i] = bc;
-      x.a_exp0 = ar;
-      x.b_exp0 = bc;
-      ar += x.m;
-      bc += x.n;
+    std::vector<GTile> tiles;
+    for (int c = 0; c < SPLIT_SHAPES; ++c) {
+      t.tile_start[c] = (uint32_t)tiles.size();
+      t.tile_count[c] = (uint32_t)per[c].size();
+      tiles.insert(tiles.end(), per[c].begin(), per[c].end());
     }
-    t.a_elems = ae;
-    t.b_elems = be;
-    t.a_rows = ar;
-    t.b_cols = bc;
-    (void)exp_row;
-    (void)exp_col;
-    t.n_tiles = (uint32_t)tiles.size();
-    t.exp_blocks_a = (uint32_t)pre_ea.back();
-    t.exp_blocks_b = (uint32_t)pre_eb.back();
-    t.slice_blocks_a = (uint32_t)pre_sa.back();
-    t.slice_blocks_b = (uint32_t)pre_sb.back();
+    t.blocks_a = (uint32_t)pre_a.back();
+    t.blocks_b = (uint32_t)pre_b.back();
+    // a job with an empty output still gets no pre-pass work (its planes are never read)
     std::vector<int64_t> pre;
-    pre.insert(pre.end(), pre_ea.begin(), pre_ea.end());
-    pre.insert(pre.end(), pre_eb.begin(), pre_eb.end());
-    pre.insert(pre.end(), pre_sa.begin(), pre_sa.end());
-    pre.insert(pre.end(), pre_sb.begin(), pre_sb.end());
+    pre.insert(pre.end(), pre_a.begin(), pre_a.end());
+    pre.insert(pre.end(), pre_b.begin(), pre_b.end());
     void *pj, *pt, *pp;
     SPLIT_HIP(hipMalloc(&pj, sizeof(SplitJob) * sj.size()));
     ws->table_allocs.push_back(pj);
@@ -423,8 +497,8 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     it = ws->tables.emplace(key, t).first;
   }
   const SplitTables& t = it->second;
-  // workspace: S A planes + S B planes (bf16), exponents per plane line (int32)
-  const size_t bytes = 2 * (size_t)S * (size_t)(t.a_elems + t.b_elems) + 4 * (size_t)(t.a_rows + t.b_cols) + 256;
+  // workspace: A fragments | B fragments (bf16) | A exps | B exps (int32)
+  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) + 4 * (size_t)(t.a_rows + t.b_cols) + 256;
   if (bytes > ws->bytes) {
     if (capturing) {
       err = "split-BF16 workspace must be sized before graph capture; run the call once first";
@@ -438,21 +512,20 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     ws->bytes = bytes;
   }
   __bf16* ap = static_cast<__bf16*>(ws->planes);
-  __bf16* bp = ap + (size_t)S * t.a_elems;
-  int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)S * t.b_elems);
+  __bf16* bp = ap + (size_t)FRAG * t.frags_a;
+  int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)FRAG * t.frags_b);
   int32_t* be = ae + t.a_rows;
   const int n_jobs = (int)host_jobs.size();
-  const int64_t* pre = t.prefix;
-  PrepArgs pa{dev_jobs, t.jobs, pre, n_jobs, a, ae, ap, t.a_elems, 0, S};
-  PrepArgs pb{dev_jobs, t.jobs, pre + (n_jobs + 1), n_jobs, b, be, bp, t.b_elems, 1, S};
-  if (t.exp_blocks_a) hipLaunchKernelGGL(split_exponents, dim3(t.exp_blocks_a), dim3(256), 0, s, pa);
-  if (t.exp_blocks_b) hipLaunchKernelGGL(split_exponents, dim3(t.exp_blocks_b), dim3(256), 0, s, pb);
-  pa.prefix = pre + 2 * (n_jobs + 1);
-  pb.prefix = pre + 3 * (n_jobs + 1);
-  if (t.slice_blocks_a) hipLaunchKernelGGL(split_slices, dim3(t.slice_blocks_a), dim3(256), 0, s, pa);
-  if (t.slice_blocks_b) hipLaunchKernelGGL(split_slices, dim3(t.slice_blocks_b), dim3(256), 0, s, pb);
-  SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ae, be, t.a_elems, t.b_elems, alpha, beta};
-  if (t.n_tiles) launch_split_gemm(S, g, t.n_tiles, s);
+  PrepArgs pa{dev_jobs, t.jobs, t.prefix, n_jobs, a, ae, ap, 0};
+  PrepArgs pb{dev_jobs, t.jobs, t.prefix + (n_jobs + 1), n_jobs, b, be, bp, 1};
+  SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ae, be, alpha, beta};
+  switch (S) {
+    case 2: launch_prepare_s<2>(pa, t.blocks_a, s); launch_prepare_s<2>(pb, t.blocks_b, s); launch_split_s<2>(t, g, s); break;
+    case 3: launch_prepare_s<3>(pa, t.blocks_a, s); launch_prepare_s<3>(pb, t.blocks_b, s); launch_split_s<3>(t, g, s); break;
+    case 4: launch_prepare_s<4>(pa, t.blocks_a, s); launch_prepare_s<4>(pb, t.blocks_b, s); launch_split_s<4>(t, g, s); break;
+    case 6: launch_prepare_s<6>(pa, t.blocks_a, s); launch_prepare_s<6>(pb, t.blocks_b, s); launch_split_s<6>(t, g, s); break;
+    default: launch_prepare_s<5>(pa, t.blocks_a, s); launch_prepare_s<5>(pb, t.blocks_b, s); launch_split_s<5>(t, g, s); break;
+  }
   SPLIT_HIP(hipGetLastError());
   return TNT_OK;
 }

@@ -109,6 +109,8 @@ class HeffServer:
         done = []
         next_ctx = 0
         results = {}
+        issue_s = 0.0
+        max_inflight_seen = 0
         for rt in self.rts:
             rt.synchronize()
         t0 = time.perf_counter()
@@ -129,7 +131,7 @@ class HeffServer:
                 next_ctx = (next_ctx + 1) % len(self.rts)
                 rt = self.rts[r.ctx_index]
                 v = self._view(D, r.ctx_index)
-                r.t_issue = now
+                r.t_issue = time.perf_counter()
                 r.theta = TensorMap.rand(rt, v.theta_space, seed=seed, normal=True)
                 r.scratch = {}
                 r.y = v.apply(r.theta, r.scratch)
@@ -138,8 +140,10 @@ class HeffServer:
                 rt._check(h.tnt_event_create(rt._ctx, C.byref(ev)))
                 rt._check(h.tnt_event_record(rt._ctx, ev))
                 r.event = ev
+                issue_s += time.perf_counter() - r.t_issue
                 used += fp
                 inflight.append(r)
+            max_inflight_seen = max(max_inflight_seen, len(inflight))
             # retire finished requests
             still = []
             for r in inflight:
@@ -179,6 +183,8 @@ class HeffServer:
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3),
             "offered_rate": n / max(trace[-1][0], 1e-9),
+            "host_issue_ms_per_request": issue_s / max(1, n) * 1e3,
+            "max_inflight": max_inflight_seen,
         }
 
     # -- helpers -----------------------------------------------------------------
