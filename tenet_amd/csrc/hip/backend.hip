@@ -119,9 +119,17 @@ struct GemmTable {
   GTile* tiles = nullptr;
   uint32_t n_tiles_cfg[TILE_CFG_COUNT] = {};
   uint32_t tile_start_cfg[TILE_CFG_COUNT] = {};
+  // the last n_narrow_cfg[c] tiles of each shape belong to narrow jobs
+  uint32_t n_narrow_cfg[TILE_CFG_COUNT] = {};
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
+  std::vector<GJob> split_jobs;  // host_jobs with the narrow jobs emptied
 };
+
+// A job with a dimension of 16 or less is bound by HBM, not by the matrix
+// cores: the split-BF16 precision runs it on the exact FP64 kernels instead
+// of cutting it into digit planes padded to 16 lines and 32 k.
+static bool narrow_job(const GJob& j) { return j.m <= 16 || j.n <= 16 || j.k <= 16; }
 
 struct tnt_artifact {
   int device = 0;
@@ -557,10 +565,17 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int fo
   }
   t.host_tiles.clear();
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+    // narrow jobs' tiles last (stable: larger jobs still lead)
+    auto mid = std::stable_partition(per[c].begin(), per[c].end(),
+                                     [&](const GTile& g) { return !narrow_job(jobs[g.job]); });
+    t.n_narrow_cfg[c] = (uint32_t)(per[c].end() - mid);
     t.tile_start_cfg[c] = (uint32_t)t.host_tiles.size();
     t.n_tiles_cfg[c] = (uint32_t)per[c].size();
     t.host_tiles.insert(t.host_tiles.end(), per[c].begin(), per[c].end());
   }
+  t.split_jobs = jobs;
+  for (auto& j : t.split_jobs)
+    if (narrow_job(j)) j.m = j.n = 0;
 }
 
 static int upload_gemm_table(tnt_ctx* ctx, GemmTable& t, std::vector<void*>& allocs) {
@@ -579,9 +594,18 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
                           double alpha, double beta, int precision) {
   TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
   if (precision == TNT_PREC_SPLIT_BF16) {
+    for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+      if (!t.n_narrow_cfg[c]) continue;
+      const uint32_t first = t.tile_start_cfg[c] + t.n_tiles_cfg[c] - t.n_narrow_cfg[c];
+      GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
+      launch_gemm_f64(g, c, t.n_narrow_cfg[c], ctx->stream);
+    }
     if (!ctx->split) ctx->split.reset(split_workspace_create());
-    return split_gemm_run(ctx->split.get(), t.host_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
-                          alpha, beta, ctx->stream, ctx->capturing, ctx->err);
+    int rc = split_gemm_run(ctx->split.get(), t.split_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
+                            alpha, beta, ctx->stream, ctx->capturing, ctx->err);
+    if (rc) return rc;
+    HIPCHK(ctx, hipGetLastError());
+    return TNT_OK;
   }
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
     if (!t.n_tiles_cfg[c]) continue;

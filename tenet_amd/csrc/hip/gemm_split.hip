@@ -54,7 +54,9 @@ struct SplitJob {
   int32_t m, n, ns;  // ns = number of 32-k steps
   int32_t ldd;
   int32_t lba, lbb;  // line blocks of 16 (A rows, B columns)
-  int32_t cfg, pad;
+  int32_t cfg;
+  int32_t nkc;       // K chunks of the pre-pass
+  int64_t pa0, pb0;  // first partial-exponent record (16 ints) of this job's A / B items
 };
 
 // Tile shapes: wave tile (16 WM) x (16 WN), 2 x 2 waves per workgroup.
@@ -65,14 +67,14 @@ constexpr int shape_wn(int c) { return c == SPLIT_64x128 ? 4 : 2; }
 struct SplitTables {
   SplitJob* jobs = nullptr;
   GTile* tiles = nullptr;
-  int64_t* prefix = nullptr;   // per-side per-job start of its line blocks in the pre-pass grid
+  int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid
   uint32_t tile_start[SPLIT_SHAPES] = {0, 0, 0}, tile_count[SPLIT_SHAPES] = {0, 0, 0};
-  uint32_t blocks_a = 0, blocks_b = 0;
-  int64_t frags_a = 0, frags_b = 0, a_rows = 0, b_cols = 0;
+  uint32_t items_a = 0, items_b = 0;
+  int64_t frags_a = 0, frags_b = 0, a_rows = 0, b_cols = 0, precs = 0;
 };
 
 struct SplitWorkspace {
-  void* planes = nullptr;  // A planes | B planes (bf16) | A exps | B exps (int32)
+  void* planes = nullptr;  // A planes | B planes (bf16) | A exps | B exps | partial exps (int32)
   size_t bytes = 0;
   std::unordered_map<std::string, SplitTables> tables;
   std::vector<void*> table_allocs;
@@ -86,20 +88,31 @@ SplitWorkspace* split_workspace_create() { return new SplitWorkspace(); }
 void split_workspace_destroy(SplitWorkspace* ws) { delete ws; }
 
 // ---------------------------------------------------------------------------
-// Pre-pass: one workgroup per (job, block of 16 lines), over all of K.
+// Pre-pass: two launches over items (operand side, job, block of 16 lines,
+// chunk of 256 k), both operands in one grid.
 //   element (line, kk): side 0 -> op(A)[line][kk], side 1 -> op(B)[kk][line]
-// Pass 1 takes the line maxima (coalesced along whichever index is
-// contiguous in the source), pass 2 cuts the digits and writes fragments.
+// split_maxexp writes, per item, the exponents of the 16 line maxima of its
+// chunk (read coalesced along whichever index is contiguous in the source);
+// split_digits combines the chunk exponents of its line block, stores the
+// line exponents (chunk 0) and cuts its chunk into digit fragments.
 // ---------------------------------------------------------------------------
+constexpr int KCHUNK = 256;
+constexpr int EXP_NONE = -4096;  // exponent record of an all-zero line chunk
+
+struct PrepSide {
+  uintptr_t base;          // operand base
+  int32_t* exps;           // line exponents of this side
+  int32_t* pexp;           // partial exponents (16 per item)
+  __bf16* planes;          // fragments of this side
+  const int64_t* prefix;   // n_jobs + 1 entries: first item of each job
+};
+
 struct PrepArgs {
   const GJob* jobs;
   const SplitJob* sj;
-  const int64_t* prefix;  // n_jobs + 1 entries
   int n_jobs;
-  uintptr_t base;         // operand base (A or B)
-  int32_t* exps;          // exponents of this side
-  __bf16* planes;         // fragments of this side
-  int side;
+  uint32_t items_a;
+  PrepSide side[2];
 };
 
 __device__ __forceinline__ int find_job(const int64_t* prefix, int n, int64_t b) {
@@ -121,50 +134,84 @@ __device__ __forceinline__ int64_t elem_index(const GJob& g, int side, int line,
   return (g.ops & 2) ? (line + (int64_t)kk * g.ldb) : (kk + (int64_t)line * g.ldb);
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void split_prepare(PrepArgs x) {
+struct PrepItem {
+  int side, job, lb, kc;
+};
+
+__device__ __forceinline__ PrepItem prep_item(const PrepArgs& x) {
+  PrepItem it;
+  it.side = blockIdx.x < x.items_a ? 0 : 1;
+  const int64_t local = blockIdx.x - (it.side ? x.items_a : 0);
+  it.job = find_job(x.side[it.side].prefix, x.n_jobs, local);
+  const int nkc = x.sj[it.job].nkc;
+  const int64_t r = local - x.side[it.side].prefix[it.job];
+  it.lb = (int)(r / nkc);
+  it.kc = (int)(r % nkc);
+  return it;
+}
+
+__global__ __launch_bounds__(256) void split_maxexp(PrepArgs x) {
   __shared__ double red[16][17];
-  __shared__ int sexp[16];
-  const int j = find_job(x.prefix, x.n_jobs, blockIdx.x);
-  const GJob g = x.jobs[j];
-  const SplitJob s = x.sj[j];
-  const int lb = (int)(blockIdx.x - x.prefix[j]);
-  const int side = x.side;
-  const int lines = side == 0 ? g.m : g.n;
-  const int K = g.k;
-  const double* M = reinterpret_cast<const double*>(x.base) + (side == 0 ? g.a_off : g.b_off);
+  const PrepItem it = prep_item(x);
+  const GJob g = x.jobs[it.job];
+  const SplitJob s = x.sj[it.job];
+  const PrepSide& sd = x.side[it.side];
+  const int lines = it.side == 0 ? g.m : g.n;
+  const double* M = reinterpret_cast<const double*>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
   const int t = threadIdx.x;
-  {
-    const bool lc = line_contiguous(g, side);
-    const int li = lc ? (t & 15) : (t >> 4), k0 = lc ? (t >> 4) : (t & 15);
-    const int line = lb * 16 + li;
-    double mx = 0.0;
-    if (line < lines)
-      for (int kk = k0; kk < K; kk += 16) mx = fmax(mx, fabs(M[elem_index(g, side, line, kk)]));
-    red[li][k0] = mx;
-  }
+  const bool lc = line_contiguous(g, it.side);
+  const int li = lc ? (t & 15) : (t >> 4), k0 = lc ? (t >> 4) : (t & 15);
+  const int line = it.lb * 16 + li;
+  const int kend = min(g.k, (it.kc + 1) * KCHUNK);
+  double mx = 0.0;
+  if (line < lines)
+    for (int kk = it.kc * KCHUNK + k0; kk < kend; kk += 16) mx = fmax(mx, fabs(M[elem_index(g, it.side, line, kk)]));
+  red[li][k0] = mx;
   __syncthreads();
   if (t < 16) {
-    double mx = 0.0;
+    double m2 = 0.0;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) mx = fmax(mx, red[t][c]);
-    int e = 0;
-    if (mx > 0.0) frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1): |x| < 2^e
+    for (int c = 0; c < 16; ++c) m2 = fmax(m2, red[t][c]);
+    int e = EXP_NONE;
+    if (m2 > 0.0) frexp(m2, &e);  // m2 = f 2^e, f in [0.5, 1): |x| < 2^e
+    const int64_t rec = (it.side == 0 ? s.pa0 : s.pb0) + (int64_t)it.lb * s.nkc + it.kc;
+    sd.pexp[rec * 16 + t] = e;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
+  __shared__ int sexp[16];
+  const PrepItem it = prep_item(x);
+  const GJob g = x.jobs[it.job];
+  const SplitJob s = x.sj[it.job];
+  const PrepSide& sd = x.side[it.side];
+  const int lines = it.side == 0 ? g.m : g.n;
+  const int K = g.k;
+  const double* M = reinterpret_cast<const double*>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
+  const int t = threadIdx.x;
+  if (t < 16) {
+    const int64_t rec0 = (it.side == 0 ? s.pa0 : s.pb0) + (int64_t)it.lb * s.nkc;
+    int e = EXP_NONE;
+    for (int c = 0; c < s.nkc; ++c) e = max(e, sd.pexp[(rec0 + c) * 16 + t]);
+    if (e == EXP_NONE) e = 0;  // all-zero line: digits are zero whatever e is
     sexp[t] = e;
-    if (lb * 16 + t < lines) x.exps[(side == 0 ? s.ea0 : s.eb0) + lb * 16 + t] = e;
+    const int line = it.lb * 16 + t;
+    if (it.kc == 0 && line < lines) sd.exps[(it.side == 0 ? s.ea0 : s.eb0) + line] = e;
   }
   __syncthreads();
   const int lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
-  const int line = lb * 16 + li;
+  const int line = it.lb * 16 + li;
   const int e = sexp[li];
-  const int64_t f0 = (side == 0 ? s.fa0 : s.fb0) + (int64_t)lb * s.ns * S;
-  for (int st = wave; st < s.ns; st += 4) {
+  const int64_t f0 = (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)it.lb * s.ns * S;
+  const int st_end = min(s.ns, (it.kc + 1) * (KCHUNK / KSTEP));
+  for (int st = it.kc * (KCHUNK / KSTEP) + wave; st < st_end; st += 4) {
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int kk = st * KSTEP + kg * 8 + q;
-      v[q] = (line < lines && kk < K) ? ldexp(M[elem_index(g, side, line, kk)], -e) : 0.0;
+      v[q] = (line < lines && kk < K) ? ldexp(M[elem_index(g, it.side, line, kk)], -e) : 0.0;
     }
 #pragma unroll
     for (int p = 0; p < S; ++p) {
@@ -176,7 +223,7 @@ __global__ __launch_bounds__(256) void split_prepare(PrepArgs x) {
         v[q] -= d;
         o[q] = (__bf16)(float)ldexp(d, -8 * p);
       }
-      *reinterpret_cast<bf16x8*>(x.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
+      *reinterpret_cast<bf16x8*>(sd.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
     }
   }
 }
@@ -400,8 +447,8 @@ static void launch_split_s(const SplitTables& t, const SplitGemmArgs& a0, hipStr
 }
 
 template <int S>
-static void launch_prepare_s(const PrepArgs& pa, uint32_t blocks, hipStream_t s) {
-  if (blocks) hipLaunchKernelGGL(split_prepare<S>, dim3(blocks), dim3(256), 0, s, pa);
+static void launch_digits_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
+  if (items) hipLaunchKernelGGL(split_digits<S>, dim3(items), dim3(256), 0, s, pa);
 }
 
 #define SPLIT_HIP(expr)                                                    \
@@ -460,9 +507,16 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       t.a_rows += j.m;
       t.b_cols += j.n;
       x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
+      x.nkc = (std::max(j.k, 1) + KCHUNK - 1) / KCHUNK;
+      // operands of a job with an empty output are never read: no pre-pass items
+      const int64_t ia = (j.n > 0) ? (int64_t)x.lba * x.nkc : 0, ib = (j.m > 0) ? (int64_t)x.lbb * x.nkc : 0;
+      x.pa0 = t.precs;
+      t.precs += ia;
+      x.pb0 = t.precs;
+      t.precs += ib;
       sj[i] = x;
-      pre_a.push_back(pre_a.back() + (j.n > 0 ? x.lba : 0));
-      pre_b.push_back(pre_b.back() + (j.m > 0 ? x.lbb : 0));
+      pre_a.push_back(pre_a.back() + ia);
+      pre_b.push_back(pre_b.back() + ib);
       if (j.m == 0 || j.n == 0) continue;
       const int bm = 16 * 2 * shape_wm(x.cfg), bn = 16 * 2 * shape_wn(x.cfg);
       for (int tm = 0; tm < (j.m + bm - 1) / bm; ++tm)
@@ -474,9 +528,12 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       t.tile_count[c] = (uint32_t)per[c].size();
       tiles.insert(tiles.end(), per[c].begin(), per[c].end());
     }
-    t.blocks_a = (uint32_t)pre_a.back();
-    t.blocks_b = (uint32_t)pre_b.back();
-    // a job with an empty output still gets no pre-pass work (its planes are never read)
+    if (pre_a.back() + pre_b.back() > (int64_t)UINT32_MAX) {
+      err = "split-BF16 pre-pass grid too large";
+      return TNT_E_UNSUPPORTED;
+    }
+    t.items_a = (uint32_t)pre_a.back();
+    t.items_b = (uint32_t)pre_b.back();
     std::vector<int64_t> pre;
     pre.insert(pre.end(), pre_a.begin(), pre_a.end());
     pre.insert(pre.end(), pre_b.begin(), pre_b.end());
@@ -497,8 +554,9 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     it = ws->tables.emplace(key, t).first;
   }
   const SplitTables& t = it->second;
-  // workspace: A fragments | B fragments (bf16) | A exps | B exps (int32)
-  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) + 4 * (size_t)(t.a_rows + t.b_cols) + 256;
+  // workspace: A fragments | B fragments (bf16) | A exps | B exps | partial exps (int32)
+  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) +
+                       4 * (size_t)(t.a_rows + t.b_cols + 16 * t.precs) + 256;
   if (bytes > ws->bytes) {
     if (capturing) {
       err = "split-BF16 workspace must be sized before graph capture; run the call once first";
@@ -515,16 +573,19 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
   __bf16* bp = ap + (size_t)FRAG * t.frags_a;
   int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)FRAG * t.frags_b);
   int32_t* be = ae + t.a_rows;
+  int32_t* pe = be + t.b_cols;
   const int n_jobs = (int)host_jobs.size();
-  PrepArgs pa{dev_jobs, t.jobs, t.prefix, n_jobs, a, ae, ap, 0};
-  PrepArgs pb{dev_jobs, t.jobs, t.prefix + (n_jobs + 1), n_jobs, b, be, bp, 1};
+  PrepArgs pa{dev_jobs, t.jobs, n_jobs, t.items_a,
+              {{a, ae, pe, ap, t.prefix}, {b, be, pe, bp, t.prefix + (n_jobs + 1)}}};
+  const uint32_t items = t.items_a + t.items_b;
+  if (items) hipLaunchKernelGGL(split_maxexp, dim3(items), dim3(256), 0, s, pa);
   SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ae, be, alpha, beta};
   switch (S) {
-    case 2: launch_prepare_s<2>(pa, t.blocks_a, s); launch_prepare_s<2>(pb, t.blocks_b, s); launch_split_s<2>(t, g, s); break;
-    case 3: launch_prepare_s<3>(pa, t.blocks_a, s); launch_prepare_s<3>(pb, t.blocks_b, s); launch_split_s<3>(t, g, s); break;
-    case 4: launch_prepare_s<4>(pa, t.blocks_a, s); launch_prepare_s<4>(pb, t.blocks_b, s); launch_split_s<4>(t, g, s); break;
-    case 6: launch_prepare_s<6>(pa, t.blocks_a, s); launch_prepare_s<6>(pb, t.blocks_b, s); launch_split_s<6>(t, g, s); break;
-    default: launch_prepare_s<5>(pa, t.blocks_a, s); launch_prepare_s<5>(pb, t.blocks_b, s); launch_split_s<5>(t, g, s); break;
+    case 2: launch_digits_s<2>(pa, items, s); launch_split_s<2>(t, g, s); break;
+    case 3: launch_digits_s<3>(pa, items, s); launch_split_s<3>(t, g, s); break;
+    case 4: launch_digits_s<4>(pa, items, s); launch_split_s<4>(t, g, s); break;
+    case 6: launch_digits_s<6>(pa, items, s); launch_split_s<6>(t, g, s); break;
+    default: launch_digits_s<5>(pa, items, s); launch_split_s<5>(t, g, s); break;
   }
   SPLIT_HIP(hipGetLastError());
   return TNT_OK;
