@@ -10,8 +10,9 @@
 //   3. C_ij = 2^(e_i + f_j - 16) sum_{p+q <= S-1 (0-based)} plane_p(A) plane_q(B).
 //      Products are summed on v_mfma_f32_16x16x32_bf16 into FP32 accumulator
 //      groups by level l = p + q: l = 0 and l = 1 each have their own group;
-//      within a K window of 128 every partial sum of those two groups is a
-//      multiple of 2^-8l below 2^(23-8l), so their FP32 accumulation is exact.
+//      within a K window of 256 every partial sum of those two groups is a
+//      multiple of 2^-8l of magnitude at most 2^(24-8l) (|d_1| <= 2^8,
+//      |d_p| <= 2^7), so FP32 holds it exactly.
 //      All remaining levels share one group (two for S = 6); their FP32
 //      rounding is 2^-24 relative to a level-2 sum, i.e. about 2^-40 of the
 //      result.  After each window the groups are added into FP64.
@@ -45,7 +46,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int FRAG = 512;         // bf16 per fragment (16 lines x 32 k)
 constexpr int KSTEP = 32;         // k per fragment / MFMA step
-constexpr int WINDOW_STEPS = 4;   // 128 k per exact FP32 window
+constexpr int WINDOW_STEPS = 8;   // 256 k per exact FP32 window
 
 struct SplitJob {
   int64_t d_off;     // element offset of C from dbase
@@ -244,19 +245,28 @@ struct SplitGemmArgs {
 
 __device__ __forceinline__ void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-// 16 bytes per lane from global memory straight into LDS at lds + 16 * lane.
-// Written as asm so that the compiler's wait insertion does not treat every
-// later LDS read as depending on every copy in flight (it cannot tell the two
-// stage buffers apart across this many copies and would wait for all of
-// them, prefetch included); the kernel waits with explicit vmcnt instead.
-// The copy has no register result, so nothing is left in flight in a VGPR.
-__device__ __forceinline__ void copy_to_lds(const void* src, const void* lds) {
-  const uint32_t m0 = __builtin_amdgcn_readfirstlane(
-      (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)lds);
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+// 16 bytes per lane from global memory straight into LDS at lds + 16 * lane:
+// global address = base (uniform, SGPR pair) + voff (per lane).  Written as
+// asm so that the compiler's wait insertion does not treat every later LDS
+// read as depending on every copy in flight (it cannot tell the two stage
+// buffers apart across this many copies and would wait for all of them,
+// prefetch included); the kernel waits with explicit vmcnt instead.  The
+// copy has no register result, so nothing is left in flight in a VGPR.
+__device__ __forceinline__ void copy_to_lds(uint64_t base, uint32_t voff, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
                :
-               : "v"(src), "s"(m0)
+               : "v"(voff), "s"(base), "s"(lds)
                : "memory", "m0");
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t lds_address(const void* p) {
+  return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p);
 }
 
 // s_waitcnt with only vmcnt constrained (expcnt and lgkmcnt at their maxima).
@@ -274,32 +284,46 @@ struct SplitKernel {
   static constexpr int PER_WAVE = (FA + FB) / 4;
   static_assert((FA + FB) % 4 == 0, "fragments per stage must split over 4 waves");
 
-  struct Ctx {
-    const __bf16* ap;  // first fragment of this tile's A line blocks at step 0
-    const __bf16* bp;
-    int64_t astep_a, astep_b;  // elements between consecutive line blocks
-    int ns;
+  // Step-invariant part of this wave's copies: fragment r of every stage
+  // comes from src[r] + step * S KiB and goes to LDS slot f = wave + 4r of
+  // the stage buffer (A fragments for r < FA/4, then B).
+  struct Copies {
+    uint64_t src[PER_WAVE];
   };
 
-  // Issue this wave's share of one stage's fragments (always PER_WAVE loads).
-  static __device__ __forceinline__ void load_stage(const SplitGemmArgs& g, const SplitJob& job, int tm, int tn,
-                                                    int step, bf16x8 (&As)[FA][64], bf16x8 (&Bs)[FB][64],
-                                                    int wave, int lane) {
+  static __device__ __forceinline__ void setup(const SplitGemmArgs& g, const SplitJob& job, int tm, int tn,
+                                               int wave, Copies& c) {
 #pragma unroll
     for (int r = 0; r < PER_WAVE; ++r) {
       const int f = wave + 4 * r;
-      if (f < FA) {
+      int64_t frag;
+      uintptr_t planes;
+      if (r < FA / 4) {
         const int i = f / S, p = f % S;
         const int lb = min(tm * LBM + i, job.lba - 1);  // clamp: rows past m are discarded
-        const __bf16* src = g.a_planes + ((job.fa0 + ((int64_t)lb * job.ns + step) * S + p) * FRAG + lane * 8);
-        copy_to_lds(src, &As[f][0]);
+        frag = job.fa0 + (int64_t)lb * job.ns * S + p;
+        planes = (uintptr_t)g.a_planes;
       } else {
         const int fb = f - FA;
         const int jb = fb / S, q = fb % S;
         const int lb = min(tn * LBN + jb, job.lbb - 1);
-        const __bf16* src = g.b_planes + ((job.fb0 + ((int64_t)lb * job.ns + step) * S + q) * FRAG + lane * 8);
-        copy_to_lds(src, &Bs[fb][0]);
+        frag = job.fb0 + (int64_t)lb * job.ns * S + q;
+        planes = (uintptr_t)g.b_planes;
       }
+      c.src[r] = uniform64(planes + (uint64_t)frag * (2 * FRAG));
+    }
+  }
+
+  // Issue this wave's share of one stage's fragments (always PER_WAVE copies)
+  // into the stage buffers at LDS addresses as_lds (A) and bs_lds (B).
+  static __device__ __forceinline__ void load_stage(const Copies& c, int step, uint32_t as_lds, uint32_t bs_lds,
+                                                    int wave, uint32_t voff) {
+    const uint64_t step_bytes = (uint64_t)step * (S * 2 * FRAG);
+#pragma unroll
+    for (int r = 0; r < PER_WAVE; ++r) {
+      const int f = wave + 4 * r;
+      const uint32_t lds = r < FA / 4 ? as_lds + f * (2 * FRAG) : bs_lds + (f - FA) * (2 * FRAG);
+      copy_to_lds(c.src[r] + step_bytes, voff, lds);
     }
   }
 
@@ -352,20 +376,18 @@ struct SplitKernel {
   }
 
   // One k step: prefetch the next stage into (An, Bn), wait for (Ac, Bc), compute.
-  static __device__ __forceinline__ void step(const SplitGemmArgs& g, const SplitJob& job, int tm, int tn, int s,
+  static __device__ __forceinline__ void step(const Copies& c, int ns, int s,
                                               bf16x8 (&Ac)[FA][64], bf16x8 (&Bc)[FB][64],
-                                              bf16x8 (&An)[FA][64], bf16x8 (&Bn)[FB][64], int wave, int lane,
-                                              f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4]) {
-    if (s + 1 < job.ns) {
-      load_stage(g, job, tm, tn, s + 1, An, Bn, wave, lane);
+                                              uint32_t an_lds, uint32_t bn_lds, int wave, int lane,
+                                              f32x4 (&acc)[NACC][WM][WN]) {
+    if (s + 1 < ns) {
+      load_stage(c, s + 1, an_lds, bn_lds, wave, lane * 16);
       wait_vmcnt<PER_WAVE>();
     } else {
       wait_vmcnt<0>();
     }
     wg_barrier();
-    if (s % WINDOW_STEPS == 0) zero(acc);
     compute(Ac, Bc, wave & 1, wave >> 1, lane, acc);
-    if (s % WINDOW_STEPS == WINDOW_STEPS - 1 || s + 1 == job.ns) flush(acc, acc64);
     wg_barrier();  // every wave is done with (Ac, Bc) before they are refilled
   }
 };
@@ -379,7 +401,8 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemmArgs g) {
   __shared__ bf16x8 B1[K::FB][64];
   const GTile tile = g.tiles[blockIdx.x];
   const SplitJob job = g.sj[tile.job];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave & 1, wn = wave >> 1;
 
   f32x4 acc[K::NACC][WM][WN];
@@ -390,12 +413,21 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemmArgs g) {
     for (int j = 0; j < WN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[i][j][r] = 0.0;
-  K::zero(acc);
-
-  K::load_stage(g, job, tile.tm, tile.tn, 0, A0, B0, wave, lane);
-  for (int s = 0; s < job.ns; s += 2) {
-    K::step(g, job, tile.tm, tile.tn, s, A0, B0, A1, B1, wave, lane, acc, acc64);
-    if (s + 1 < job.ns) K::step(g, job, tile.tm, tile.tn, s + 1, A1, B1, A0, B0, wave, lane, acc, acc64);
+  typename K::Copies cp;
+  K::setup(g, job, tile.tm, tile.tn, wave, cp);
+  const int ns = job.ns;
+  const uint32_t a0 = lds_address(&A0[0][0]), b0 = lds_address(&B0[0][0]);
+  const uint32_t a1 = lds_address(&A1[0][0]), b1 = lds_address(&B1[0][0]);
+  K::load_stage(cp, 0, a0, b0, wave, lane * 16);
+  // windows of WINDOW_STEPS (even) steps: every window starts on stage buffer 0
+  for (int w0 = 0; w0 < ns; w0 += WINDOW_STEPS) {
+    const int w1 = min(ns, w0 + WINDOW_STEPS);
+    K::zero(acc);
+    for (int s = w0; s < w1; s += 2) {
+      K::step(cp, ns, s, A0, B0, a1, b1, wave, lane, acc);
+      if (s + 1 < w1) K::step(cp, ns, s + 1, A1, B1, a0, b0, wave, lane, acc);
+    }
+    K::flush(acc, acc64);
   }
 
   // epilogue: C = alpha 2^(e_i + f_j - 16) acc64 (+ beta C)
@@ -460,9 +492,11 @@ static void launch_digits_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
     }                                                                      \
   } while (0)
 
-// Tile shape of one job: the 64-wide shapes only where the job fills them.
+// Tile shape of one job.  The 128-line shapes hold one workgroup per CU
+// (LDS) against two for 64x64; they win only when a tile runs many k steps
+// (measured: K ~ 7000 +12%, K ~ 400 -15%), and only where the job fills them.
 static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
-  if (S >= 6 || total_64x64_tiles < 512) return SPLIT_64x64;  // small launches want more workgroups
+  if (S >= 6 || total_64x64_tiles < 512 || j.k < 1024) return SPLIT_64x64;
   if (j.m >= 128 && j.m >= j.n) return SPLIT_128x64;
   if (j.n >= 128) return SPLIT_64x128;
   return SPLIT_64x64;
