@@ -33,6 +33,29 @@ PEAKS = {"f64": 78.6, "f32": 157.3, "split_bf16": 2500.0}  # dense MFMA TFLOP/s
 HBM_PEAK_GBS = 8000.0
 
 
+GEMM_KERNELS = ["gemm_kernel<{}, 128, 128>", "gemm_kernel<{}, 64, 64>", "gemm_kernel<{}, 128, 64>",
+                "gemm_kernel<{}, 64, 128>", "gemm_skinny_kernel<{}>"]
+SPLIT_PRODUCTS = 15  # bf16 MFMA products per FP64 product at five digit planes
+
+
+def roofline(precision, dom):
+    """Roofline record of the dominant GEMM kernel: algorithmic flops per launch
+    over its average launch time (HIP events on the backend stream)."""
+    avg_s = dom["ms"] * 1e-3 / max(1, dom["launches"])
+    per_launch = dom["flops"] / max(1, dom["launches"])
+    f64_eq = per_launch / avg_s / 1e12 if avg_s else 0.0
+    rec = {"bound": "mfma", "unit": "TFLOP/s", "traffic": None, "kernel": dom["kernel"],
+           "launches": dom["launches"], "avg_launch_ms": avg_s * 1e3,
+           "flops_per_launch": per_launch}
+    if precision == "split_bf16":
+        # bf16 MFMA work actually issued: SPLIT_PRODUCTS digit-plane products per FP64 product
+        ach = f64_eq * SPLIT_PRODUCTS
+        rec.update(achieved=ach, peak=PEAKS["split_bf16"], frac=ach / PEAKS["split_bf16"], f64_equivalent=f64_eq)
+    else:
+        rec.update(achieved=f64_eq, peak=PEAKS[precision], frac=f64_eq / PEAKS[precision])
+    return rec
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -95,7 +118,10 @@ def cpu_baseline(workload, D, budget_s=20.0):
     times = []
     out = None
     t_start = time.perf_counter()
-    while len(times) < 3 and (time.perf_counter() - t_start) < budget_s:
+    # at least 3 runs and about 10 s of CPU work, at most budget_s
+    while len(times) < 3 or (time.perf_counter() - t_start) < min(10.0, budget_s):
+        if len(times) >= 3 and (time.perf_counter() - t_start) >= budget_s:
+            break
         t = time.perf_counter()
         out = step()
         times.append(time.perf_counter() - t)
@@ -313,17 +339,36 @@ def main():
     rt._check(h.tnt_ctx_set_timing(rt._ctx, 0))
     ms = (t1 - t0) * 1e3 / args.steps
     value = wl.flops / (ms * 1e-3) / 1e12
-    prec_kind = {"f64": 1, "f32": 2, "split_bf16": 3}[args.precision]
+    prec_id = {"f64": 0, "f32": 1, "split_bf16": 2}[args.precision]
     n, tms, work = C.c_size_t(), C.c_double(), C.c_double()
-    rt._check(h.tnt_ctx_kernel_stats(rt._ctx, prec_kind, C.byref(n), C.byref(tms), C.byref(work)))
-    gemm_tflops = work.value / (tms.value * 1e-3) / 1e12 if tms.value else 0.0
-    gemm_launches, gemm_ms = n.value, tms.value
-    rt._check(h.tnt_ctx_kernel_stats(rt._ctx, 0, C.byref(n), C.byref(tms), C.byref(work)))
-    xf_gbs = work.value / (tms.value * 1e-3) / 1e9 if tms.value else 0.0
-    xf_launches, xf_ms, xf_bytes = n.value, tms.value, work.value
+
+    def stats(kind):
+        rt._check(h.tnt_ctx_kernel_stats(rt._ctx, kind, C.byref(n), C.byref(tms), C.byref(work)))
+        return n.value, tms.value, work.value
+
+    # all block-GEMM table runs (every launch, split pre-passes included)
+    gemm_launches, gemm_ms, gemm_work = stats(1 + prec_id)
+    gemm_tflops = gemm_work / (gemm_ms * 1e-3) / 1e12 if gemm_ms else 0.0
+    # dominant kernel: the GEMM kernel (one tile shape) with the most time
+    if args.workload == "batched":
+        # the step is one hipGraph replay of the grouped launch(es); graph replays are
+        # not bracketed per kernel, so the step time (device-synchronised) stands in
+        dom = {"kernel": f"grouped GEMM hipGraph replay ({args.precision})", "launches": args.steps,
+               "ms": ms * args.steps, "flops": wl.flops * args.steps}
+    elif args.precision == "split_bf16":
+        dom = {"kernel": "split-BF16 table run (digit pre-pass + bf16 MFMA GEMM; FP64 kernels for narrow jobs)",
+               "launches": gemm_launches, "ms": gemm_ms, "flops": gemm_work}
+    else:
+        per = [(stats(8 + 8 * prec_id + c), c) for c in range(5)]
+        (dl, dms, dw), dc = max(per, key=lambda x: x[0][1])
+        dom = {"kernel": GEMM_KERNELS[dc].format("double" if prec_id == 0 else "float"),
+               "launches": dl, "ms": dms, "flops": dw}
+    xf_launches, xf_ms, xf_bytes = stats(0)
+    xf_gbs = xf_bytes / (xf_ms * 1e-3) / 1e9 if xf_ms else 0.0
     log(f"[bench] {ms:.3f} ms/step, {value:.2f} TFLOP/s; GEMM {gemm_tflops:.2f} TFLOP/s over "
-        f"{gemm_launches} launches ({gemm_ms / args.steps:.3f} ms/step); recoupling {xf_gbs:.0f} GB/s over "
-        f"{xf_launches} launches ({xf_ms / args.steps:.3f} ms/step)")
+        f"{gemm_launches} table runs ({gemm_ms / args.steps:.3f} ms/step); recoupling {xf_gbs:.0f} GB/s over "
+        f"{xf_launches} launches ({xf_ms / args.steps:.3f} ms/step); dominant kernel {dom['kernel']}: "
+        f"{dom['launches']} launches, {dom['ms'] / max(1, dom['launches']):.4f} ms avg")
 
     cpu = None
     parity = None
@@ -369,10 +414,9 @@ def main():
         "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)"}[args.precision],
         "data": "synthetic (seeded normal(0,1) on device)",
         "config": cfg,
-        "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAKS[args.precision],
-                     "unit": "TFLOP/s", "frac": gemm_tflops / PEAKS[args.precision], "traffic": None,
-                     "kernel": "grouped block GEMM", "launches": gemm_launches,
-                     "avg_launch_ms": gemm_ms / max(1, gemm_launches)},
+        "roofline": roofline(args.precision, dom),
+        "gemm_all_launches": {"tflops": gemm_tflops, "launches": gemm_launches,
+                              "ms_per_step": gemm_ms / args.steps},
         "recoupling": {"bound": "hbm", "achieved": xf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": xf_gbs / HBM_PEAK_GBS, "launches": xf_launches,
                        "avg_launch_ms": xf_ms / max(1, xf_launches),

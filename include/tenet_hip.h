@@ -250,9 +250,12 @@ int tnt_event_query(tnt_ctx* ctx, const tnt_event* ev);
 int tnt_event_destroy(tnt_ctx* ctx, tnt_event* ev);
 
 /* ---- kernel timing ---------------------------------------------------------- */
-/* When on, every recoupling replay (kind 0) and block-GEMM launch (kind 1 + precision)
- * is bracketed by HIP events on the context stream; stats report launches, total
- * milliseconds and total algorithmic work (bytes for kind 0, flops otherwise).
+/* When on, every recoupling replay (kind 0), every block-GEMM table run (kind
+ * 1 + precision: all launches of one call, split-BF16 pre-passes included) and
+ * every FP64/FP32 GEMM kernel launch (kind 8 + 8 * precision + tile shape, shapes
+ * as in tnt_set_tile_config) is bracketed by HIP events on the context stream;
+ * stats report launches, total milliseconds and total algorithmic work (bytes
+ * for kind 0, flops otherwise).
  * No reference counterpart (the reference's profiles are host wall-clock,
  * tenet-operations/src/profile.rs:16-59). */
 int tnt_ctx_set_timing(tnt_ctx* ctx, int on);

@@ -16,7 +16,7 @@ Pinning (what fixes each part independently of this restatement):
 * layout / fill        -- microbench checksums (docs/su2_authority.md:81-83),
                           TensorKit repartition fixtures and invariant stream
                           (benchmarks/tensorkit_semantic_oracle.out),
-                          closed-form offsets (tests/test_oracle_layout.py).
+                          closed-form offsets (tests/test_oracle_stream.py).
 * SU(2) F/R/dim/FS     -- the 181 TensorKit F values and R/dim/FS tables,
                           pentagon and hexagon identities, F orthogonality.
 * braid / bend coeffs  -- SU(2) KAT 0.5 / 0.8660254037844386 / 53 / 92.6025...

@@ -70,6 +70,11 @@ static hipEvent_t take_event(tnt_ctx* ctx) {
 }
 
 // Brackets one kernel-class launch with events on the context stream.
+// Timing kinds: 0 recoupling replay, 1 + precision one block-GEMM table run
+// (all its launches), 8 + 8 * precision + shape one FP64/FP32 GEMM kernel launch.
+static int timing_kind_kernel(int precision, int cfg) { return 8 + 8 * precision + cfg; }
+constexpr int kTimingKinds = 8 + 8 * 2 + TILE_CFG_COUNT;
+
 struct TimedScope {
   tnt_ctx* ctx;
   int kind;
@@ -121,6 +126,8 @@ struct GemmTable {
   uint32_t tile_start_cfg[TILE_CFG_COUNT] = {};
   // the last n_narrow_cfg[c] tiles of each shape belong to narrow jobs
   uint32_t n_narrow_cfg[TILE_CFG_COUNT] = {};
+  double flops_cfg[TILE_CFG_COUNT] = {};         // algorithmic flops per shape
+  double flops_narrow_cfg[TILE_CFG_COUNT] = {};  // ... of its narrow jobs
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
   std::vector<GJob> split_jobs;  // host_jobs with the narrow jobs emptied
@@ -428,10 +435,11 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     for (size_t i = 2; i < dims.size(); ++i) outer *= dims[i];
     int64_t elems = plane * outer;
     bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
-    // Single blocks: 1024-element items (4 per thread) swept in order by a
-    // persistent grid; Multi: ~8K element-updates per item, weighted by its layouts
+    // Single blocks: kSingleItem-element items (kSingleUnroll per thread) swept in
+    // order by a persistent grid; Multi: ~8K element-updates per item, weighted by
+    // its layouts
     int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
-    int64_t target = kind == XK_SINGLE ? 1024 : std::max<int64_t>(256, 8192 / weight);
+    int64_t target = kind == XK_SINGLE ? kSingleItem : std::max<int64_t>(256, 8192 / weight);
     if (plane >= target) {
       int64_t chunk = ((target + 63) / 64) * 64;
       for (int64_t o = 0; o < outer; ++o)
@@ -565,6 +573,16 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int fo
   }
   t.host_tiles.clear();
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+    t.flops_cfg[c] = t.flops_narrow_cfg[c] = 0.0;
+    int32_t last = -1;
+    for (const GTile& g : per[c]) {  // tiles of one job are adjacent
+      if (g.job == last) continue;
+      last = g.job;
+      const GJob& j = jobs[g.job];
+      const double f = 2.0 * double(j.m) * double(j.n) * double(j.k);
+      t.flops_cfg[c] += f;
+      if (narrow_job(j)) t.flops_narrow_cfg[c] += f;
+    }
     // narrow jobs' tiles last (stable: larger jobs still lead)
     auto mid = std::stable_partition(per[c].begin(), per[c].end(),
                                      [&](const GTile& g) { return !narrow_job(jobs[g.job]); });
@@ -598,6 +616,7 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
       if (!t.n_narrow_cfg[c]) continue;
       const uint32_t first = t.tile_start_cfg[c] + t.n_tiles_cfg[c] - t.n_narrow_cfg[c];
       GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
+      TimedScope tk(ctx, timing_kind_kernel(TNT_PREC_F64, c), ctx->timing ? t.flops_narrow_cfg[c] : 0.0);
       launch_gemm_f64(g, c, t.n_narrow_cfg[c], ctx->stream);
     }
     if (!ctx->split) ctx->split.reset(split_workspace_create());
@@ -610,6 +629,7 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
     if (!t.n_tiles_cfg[c]) continue;
     GemmArgs g{t.jobs, t.tiles + t.tile_start_cfg[c], d, a, b, alpha, beta};
+    TimedScope tk(ctx, timing_kind_kernel(precision, c), ctx->timing ? t.flops_cfg[c] : 0.0);
     if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
     else launch_gemm_f64(g, c, t.n_tiles_cfg[c], ctx->stream);
   }
@@ -1281,7 +1301,7 @@ int tnt_ctx_set_timing(tnt_ctx* ctx, int on) {
 }
 
 int tnt_ctx_kernel_stats(tnt_ctx* ctx, int kind, size_t* launches, double* total_ms, double* total_work) {
-  if (!ctx || kind < 0 || kind > 3) return TNT_E_INVALID;
+  if (!ctx || kind < 0 || kind >= kTimingKinds) return TNT_E_INVALID;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   size_t n = 0;
   double ms = 0, w = 0;

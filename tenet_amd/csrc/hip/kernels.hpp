@@ -9,6 +9,8 @@ namespace tnt_dev {
 
 constexpr int kMaxRank = 8;
 constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in registers
+constexpr int kSingleUnroll = 4;  // elements per thread of a Single work item
+constexpr int kSingleItem = 256 * kSingleUnroll;  // plane elements per Single work item
 
 enum XKind : int32_t { XK_SINGLE = 0, XK_MULTI = 1, XK_SCALE = 2 };
 

@@ -22,6 +22,7 @@ namespace tnt_dev {
 static_assert(sizeof(XItem) == 24, "XItem layout");
 
 constexpr int kUnroll = 4;
+static_assert(kSingleItem == 256 * kSingleUnroll, "Single items are swept 256 elements at a time");
 
 __device__ __forceinline__ int64_t base_offset(const XLayout& L, const int64_t* idx, int rank) {
   int64_t off = L.offset;
@@ -51,9 +52,10 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
   const int wave = threadIdx.x >> 6;
 
   if (it.o_count == 1 && kind == XK_SINGLE) {
-    // one outer index, up to 1024 plane elements: element e = e_begin +
-    // 64 * (4u + wave) + lane, u < 4 -- the whole workgroup sweeps the item
-    // front to back in 256-element steps
+    // one outer index, up to kSingleItem plane elements: element
+    // e = e_begin + 64 * (4u + wave) + lane, u < kSingleUnroll -- the whole
+    // workgroup sweeps the item front to back in 256-element steps, with all
+    // kSingleUnroll loads of a lane in flight before its stores
     int64_t idx[kMaxRank];
     {
       uint64_t rr = it.o_begin;
@@ -75,21 +77,29 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
     const double c = a.alpha * a.coeffs[b.coeff];
     const bool readd = a.overwrite == 0 && a.beta != 0.0;
     const uint32_t n0u = (uint32_t)n0;
-    double v[kUnroll];
-    int64_t dp[kUnroll];
+    // (x, y) of the lane's first element, then steps of 256 elements
+    const uint32_t f0 = 64u * wave + lane;
+    uint32_t x = (it.e_begin + f0) % n0u, y = (it.e_begin + f0) / n0u;
+    const uint32_t sx = 256u % n0u, sy = 256u / n0u;
+    double v[kSingleUnroll];
+    int64_t dp[kSingleUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint32_t f = 64u * (4u * u + wave) + lane;
+    for (int u = 0; u < kSingleUnroll; ++u) {
+      const uint32_t f = f0 + 256u * u;
       if (f < it.e_count) {
-        const uint32_t e = it.e_begin + f;
-        const uint32_t x = e % n0u, y = e / n0u;
         v[u] = a.src[os + x * s0 + y * s1];
         dp[u] = od + x * d0 + y * d1;
       }
+      x += sx;
+      y += sy;
+      if (x >= n0u) {
+        x -= n0u;
+        y += 1;
+      }
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint32_t f = 64u * (4u * u + wave) + lane;
+    for (int u = 0; u < kSingleUnroll; ++u) {
+      const uint32_t f = f0 + 256u * u;
       if (f < it.e_count) {
         double* p = a.dst + dp[u];
         *p = readd ? c * v[u] + a.beta * *p : c * v[u];

@@ -2,6 +2,8 @@
 
 * ``mps_env``: one left-environment update of a two-site-MPS sweep,
       T1 = L . A,  T2 = T1 . W,  L' = A^dagger . T2
+  (T2 stays in the core GEMM's output order, so only one recoupling pass
+  sits between each pair of GEMMs)
   with bond V (dimension D), physical P (d=2) and MPO bond M (5).
   U(1): V charges -10..10 with Gaussian degeneracies (sigma 4, largest
   remainder), P = {0, 1}, M = {0:3, +1:1, -1:1} (Heisenberg S^z / S^+- channels).
@@ -94,20 +96,26 @@ class MPSEnvUpdate:
         self._tmp = {}
         self.run()  # compile every structure (cold) and size the outputs
         c1 = rt.contraction(self.L.space, self.A.space, [2], [0], [0, 1, 2, 3], 2)
-        c2 = rt.contraction(self._tmp["T1"].space, self.W.space, [1, 2], [3, 1], [0, 2, 3, 1], 2)
-        c3 = rt.contraction(self.A.space, self._tmp["T2"].space, [2, 0], [0, 2], [0, 1, 2], 2, True, False)
+        c2 = rt.contraction(self._tmp["T1"].space, self.W.space, *self.T2_AXES)
+        c3 = rt.contraction(self.A.space, self._tmp["T2"].space, *self.LP_AXES, True, False)
         self.contractions = (c1, c2, c3)
         self.flops = sum(c.flops for c in self.contractions)
+
+    # T2 is kept in the order the core GEMM produces it, T2[a', b; w, s'] (no
+    # output recoupling); the last contraction reads it in that order.  L' is the
+    # same tensor, in the same layout, as with T2[a', w; s', b].
+    T2_AXES = ([1, 2], [3, 1], [0, 1, 2, 3], 2)
+    LP_AXES = ([2, 0], [0, 3], [0, 2, 1], 2)
 
     def run(self, L=None):
         L = L or self.L
         t1 = L.contract([2], self.A, [0], [0, 1, 2, 3], 2, out=self._tmp.get("T1"),
                         precision=self.precision)
         self._tmp["T1"] = t1
-        t2 = t1.contract([1, 2], self.W, [3, 1], [0, 2, 3, 1], 2, out=self._tmp.get("T2"),
+        t2 = t1.contract(self.T2_AXES[0], self.W, *self.T2_AXES[1:], out=self._tmp.get("T2"),
                          precision=self.precision)
         self._tmp["T2"] = t2
-        self.out = self.A.adjoint().contract([2, 0], t2, [0, 2], [0, 1, 2], 2, out=self.out,
+        self.out = self.A.adjoint().contract(self.LP_AXES[0], t2, *self.LP_AXES[1:], out=self.out,
                                              precision=self.precision)
         return self.out
 
