@@ -109,6 +109,33 @@ const tnth_space* tnth_contract_dst_space(const tnth_contract* c);
 /* Algorithmic FP64 flops of the core GEMMs (sum 2 m k n) and of the recoupling GEMMs. */
 double tnth_contract_flops(const tnth_contract* c);
 
+/* ---- snapshots: TeNeT's v1 semantic wire format ----------------------------
+ * Replace TensorMap::to_bytes_with / from_bytes_with and GradedSpace's pair
+ * (tenet/src/typed/serialization.rs:912-957, :1569-1640, :1700-1740) for dense
+ * f64 tensors, with a fixed codec per rule: provider keys "u1", "su2", "z2",
+ * "u1xz2"; labels = charge as LE i32 (U1), one byte 2j (SU2), one byte parity
+ * (Z2), charge then parity (U1xZ2) -- the reference test codecs
+ * (tenet/tests/typed_serialization.rs:63-131) for U1 and SU2.
+ * Encoders return a byte buffer owned by the caller (tnth_bytes_free). */
+typedef struct tnth_bytes tnth_bytes;
+const uint8_t* tnth_bytes_data(const tnth_bytes* b);
+size_t tnth_bytes_size(const tnth_bytes* b);
+void tnth_bytes_free(tnth_bytes* b);
+/* Tensor snapshot of a host copy of the payload (adjoint != 0: a lazy adjoint). */
+int tnth_snapshot_tensor(const tnth_space* space, const double* payload, int64_t len, int adjoint,
+                         tnth_bytes** out);
+/* Decode in two steps: the space (and adjoint flag), then the payload into a
+ * caller buffer of tnth_space_len(space) doubles (untouched on failure). */
+int tnth_snapshot_tensor_space(const uint8_t* bytes, size_t n, tnth_space** out, int* adjoint);
+int tnth_snapshot_tensor_payload(const uint8_t* bytes, size_t n, const tnth_space* space,
+                                 double* payload, int64_t len);
+/* One graded space (leg); ids as in tnth_space_create. */
+int tnth_snapshot_leg(int rule, size_t n, const uint64_t* ids, const int64_t* degs, int is_dual,
+                      tnth_bytes** out);
+/* Sizes first (ids, degs NULL), then the sectors sorted by id. */
+int tnth_snapshot_leg_decode(const uint8_t* bytes, size_t n, int* rule, size_t* n_sectors,
+                             uint64_t* ids, int64_t* degs, int* is_dual);
+
 #ifdef __cplusplus
 }
 #endif

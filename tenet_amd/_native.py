@@ -160,6 +160,15 @@ HOST_API = {
     "tnth_contract_desc": (C.POINTER(ContractDesc), [VP]),
     "tnth_contract_dst_space": (VP, [VP]),
     "tnth_contract_flops": (C.c_double, [VP]),
+    "tnth_bytes_data": (C.POINTER(C.c_uint8), [VP]),
+    "tnth_bytes_size": (C.c_size_t, [VP]),
+    "tnth_bytes_free": (None, [VP]),
+    "tnth_snapshot_tensor": (C.c_int, [VP, C.POINTER(C.c_double), C.c_int64, C.c_int, PP]),
+    "tnth_snapshot_tensor_space": (C.c_int, [C.c_char_p, SZ, PP, C.POINTER(C.c_int)]),
+    "tnth_snapshot_tensor_payload": (C.c_int, [C.c_char_p, SZ, VP, C.POINTER(C.c_double), C.c_int64]),
+    "tnth_snapshot_leg": (C.c_int, [C.c_int, SZ, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.c_int, PP]),
+    "tnth_snapshot_leg_decode": (C.c_int, [C.c_char_p, SZ, C.POINTER(C.c_int), SZP, C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
 }
 
 _host = None

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/host_capi.cpp"]
+HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/snapshot.cpp", "csrc/host/host_capi.cpp"]
 HIP_SRC = ["csrc/hip/backend.hip", "csrc/hip/xform.hip", "csrc/hip/gemm.hip",
            "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

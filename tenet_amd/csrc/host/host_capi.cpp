@@ -223,4 +223,68 @@ const tnt_contract_desc* tnth_contract_desc(const tnth_contract* c) { return &c-
 const tnth_space* tnth_contract_dst_space(const tnth_contract* c) { return &c->dst; }
 double tnth_contract_flops(const tnth_contract* c) { return c->c->flops; }
 
+struct tnth_bytes {
+  std::vector<uint8_t> v;
+};
+const uint8_t* tnth_bytes_data(const tnth_bytes* b) { return b ? b->v.data() : nullptr; }
+size_t tnth_bytes_size(const tnth_bytes* b) { return b ? b->v.size() : 0; }
+void tnth_bytes_free(tnth_bytes* b) { delete b; }
+
+int tnth_snapshot_tensor(const tnth_space* s, const double* payload, int64_t len, int adjoint, tnth_bytes** out) {
+  return guard([&] {
+    if (!s || !out || (!payload && len)) throw Error("null argument");
+    *out = new tnth_bytes{encode_tensor_snapshot(*s->s, payload, len, adjoint != 0)};
+  });
+}
+
+int tnth_snapshot_tensor_space(const uint8_t* bytes, size_t n, tnth_space** out, int* adjoint) {
+  return guard([&] {
+    if (!bytes || !out) throw Error("null argument");
+    bool adj = false;
+    auto sp = decode_tensor_space(bytes, n, adj);
+    if (adjoint) *adjoint = adj ? 1 : 0;
+    *out = new tnth_space{sp};
+  });
+}
+
+int tnth_snapshot_tensor_payload(const uint8_t* bytes, size_t n, const tnth_space* s, double* payload, int64_t len) {
+  return guard([&] {
+    if (!bytes || !s || (!payload && len)) throw Error("null argument");
+    decode_tensor_payload(bytes, n, *s->s, payload, len);
+  });
+}
+
+int tnth_snapshot_leg(int rule, size_t n, const uint64_t* ids, const int64_t* degs, int is_dual, tnth_bytes** out) {
+  return guard([&] {
+    Rule r = rule_of(rule);
+    if (!out || (n && (!ids || !degs))) throw Error("null argument");
+    std::vector<std::pair<Sid, int64_t>> p;
+    for (size_t i = 0; i < n; ++i) p.emplace_back(ids[i], degs[i]);
+    std::sort(p.begin(), p.end());
+    Leg leg;
+    for (auto& x : p) {
+      leg.sectors.push_back(x.first);
+      leg.degs.push_back(x.second);
+    }
+    leg.is_dual = is_dual != 0;
+    *out = new tnth_bytes{encode_leg_snapshot(r.kind, leg)};
+  });
+}
+
+int tnth_snapshot_leg_decode(const uint8_t* bytes, size_t n, int* rule, size_t* n_sectors, uint64_t* ids,
+                             int64_t* degs, int* is_dual) {
+  return guard([&] {
+    if (!bytes) throw Error("null argument");
+    RuleKind k;
+    Leg leg = decode_leg_snapshot(bytes, n, k);
+    if (rule) *rule = (int)k;
+    if (n_sectors) *n_sectors = leg.sectors.size();
+    if (is_dual) *is_dual = leg.is_dual ? 1 : 0;
+    for (size_t i = 0; i < leg.sectors.size(); ++i) {
+      if (ids) ids[i] = leg.sectors[i];
+      if (degs) degs[i] = leg.degs[i];
+    }
+  });
+}
+
 }  // extern "C"

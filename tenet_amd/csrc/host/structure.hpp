@@ -148,4 +148,12 @@ struct Error : std::exception {
   const char* what() const noexcept override { return msg.c_str(); }
 };
 
+// Snapshots in TeNeT's v1 wire format (snapshot.cpp).
+std::vector<uint8_t> encode_leg_snapshot(RuleKind k, const Leg& leg);
+Leg decode_leg_snapshot(const uint8_t* bytes, size_t n, RuleKind& k);
+std::vector<uint8_t> encode_tensor_snapshot(const HomSpace& s, const double* data, int64_t len, bool adjoint);
+std::shared_ptr<HomSpace> decode_tensor_space(const uint8_t* bytes, size_t n, bool& adjoint);
+void decode_tensor_payload(const uint8_t* bytes, size_t n, const HomSpace& s, double* out, int64_t len);
+
+
 }  // namespace tnth

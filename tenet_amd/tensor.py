@@ -68,6 +68,16 @@ class Rule:
         return f"Rule({self.name})"
 
 
+def _take_bytes(handle):
+    """Copy and free a tnth_bytes buffer."""
+    h = N.host()
+    try:
+        n = h.tnth_bytes_size(handle)
+        return C.string_at(h.tnth_bytes_data(handle), n) if n else b""
+    finally:
+        h.tnth_bytes_free(handle)
+
+
 class GradedSpace:
     """One leg: (sector, degeneracy) pairs and a dual flag (typed.rs:7795-7830)."""
 
@@ -97,6 +107,28 @@ class GradedSpace:
 
     def __hash__(self):
         return hash(self.key())
+
+    def to_bytes(self):
+        """Space snapshot in TeNeT's v1 wire format (GradedSpace::to_bytes_with,
+        tenet/src/typed/serialization.rs:1700-1712)."""
+        h = N.host()
+        b = C.c_void_p()
+        n = len(self.ids)
+        N.check_host(h.tnth_snapshot_leg(self.rule.id, n, (C.c_uint64 * max(1, n))(*self.ids),
+                                         (C.c_int64 * max(1, n))(*self.degs), int(self.is_dual), C.byref(b)))
+        return _take_bytes(b)
+
+    @classmethod
+    def from_bytes(cls, data):
+        """Inverse of to_bytes (GradedSpace::from_bytes_with)."""
+        h = N.host()
+        data = bytes(data)
+        rule, n, dual = C.c_int(), C.c_size_t(), C.c_int()
+        N.check_host(h.tnth_snapshot_leg_decode(data, len(data), C.byref(rule), C.byref(n), None, None, None))
+        ids, degs = (C.c_uint64 * max(1, n.value))(), (C.c_int64 * max(1, n.value))()
+        N.check_host(h.tnth_snapshot_leg_decode(data, len(data), None, None, ids, degs, C.byref(dual)))
+        name = {v: k for k, v in N.RULES.items()}[rule.value]
+        return cls(Rule(name), [(ids[i], degs[i]) for i in range(n.value)], bool(dual.value), _ids=True)
 
     def __repr__(self):
         pairs = [(self.rule.label(s), d) for s, d in zip(self.ids, self.degs)]
@@ -147,6 +179,13 @@ class HomSpace:
         N.check_host(h.tnth_space_leg(self._h, axis, None, ids, degs, C.byref(dual)))
         return GradedSpace(self.rule, [(ids[i], degs[i]) for i in range(n.value)], bool(dual.value),
                            _ids=True)
+
+    @classmethod
+    def _adopt(cls, rule, handle):
+        """Wrap a space handle returned by the host library, taking ownership."""
+        sp = cls(rule, None, None, _handle=handle)
+        sp._own = True
+        return sp
 
     def __del__(self):
         if getattr(self, "_own", False) and N._host is not None:
@@ -429,6 +468,33 @@ class TensorMap:
             self.rt._check(N.hip().tnt_buf_download(self.rt._ctx, self._buf, out.ctypes.data, self.space.len))
         return out
 
+
+    def to_bytes(self):
+        """Tensor snapshot in TeNeT's v1 wire format (TensorMap::to_bytes_with,
+        tenet/src/typed/serialization.rs:912-957): the payload is downloaded and
+        encoded on the host; a lazy adjoint is stored as the adjoint of its parent."""
+        base = self._base if self._adjoint else self
+        data = base.to_numpy()
+        b = C.c_void_p()
+        N.check_host(N.host().tnth_snapshot_tensor(base.space._h, data.ctypes.data_as(C.POINTER(C.c_double)),
+                                                   base.space.len, int(self._adjoint), C.byref(b)))
+        return _take_bytes(b)
+
+    @classmethod
+    def from_bytes(cls, rt, data):
+        """Inverse of to_bytes (TensorMap::from_bytes_with, serialization.rs:1569-1640):
+        validates the whole snapshot on the host, then uploads the payload."""
+        h = N.host()
+        data = bytes(data)
+        sp, adj = C.c_void_p(), C.c_int()
+        N.check_host(h.tnth_snapshot_tensor_space(data, len(data), C.byref(sp), C.byref(adj)))
+        rule_name = {v: k for k, v in N.RULES.items()}[h.tnth_space_rule(sp)]
+        space = HomSpace._adopt(Rule(rule_name), sp.value)
+        payload = np.empty(space.len)
+        N.check_host(h.tnth_snapshot_tensor_payload(data, len(data), space._h,
+                                                    payload.ctypes.data_as(C.POINTER(C.c_double)), space.len))
+        t = cls.from_numpy(rt, space, payload)
+        return t.adjoint() if adj.value else t
     @property
     def codomain_rank(self):
         return self.space.nout

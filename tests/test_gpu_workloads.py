@@ -50,3 +50,68 @@ def test_fmove_contract_matches_oracle(rt):
     got = wl.run().to_numpy()
     want = _o(wl.A).contract([1, 3], _o(wl.B), [2, 0], [0, 1, 2, 3], 2).data
     assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("rule_name,D", [("U1", 128), ("SU2", 256)])
+def test_snapshot_roundtrip_of_device_environment(rt, rule_name, D):
+    """Checkpoint / resume: an environment written from the device and read
+    back on another context is bit-identical, also as a lazy adjoint."""
+    wl = MPSEnvUpdate(rt, rule_name, D, "f64")
+    env = wl.run()
+    data = env.to_bytes()
+    rt2 = Runtime(0)
+    back = adj = None
+    try:
+        back = type(env).from_bytes(rt2, data)
+        assert back.space == env.space
+        assert np.array_equal(back.to_numpy().view(np.uint64), env.to_numpy().view(np.uint64))
+        assert back.to_bytes() == data
+        adj = type(env).from_bytes(rt2, env.adjoint().to_bytes())
+        assert adj._adjoint and np.array_equal(adj._base.to_numpy(), env.to_numpy())
+    finally:
+        del back, adj
+        rt2.close()
+
+
+def _env_network():
+    from tenet_amd.network import Network
+    # L[p, w; k] A[k; s, b] W[v, s; t, w] conj(A)[p; t, y]  ->  L'[y, v; b]
+    return Network([["p", "w", "k"], ["k", "s", "b"], ["v", "s", "t", "w"], ["p", "t", "y"]],
+                   [False, False, False, True], [2, 1, 2, 1], ["y", "v", "b"], 2)
+
+
+@pytest.mark.parametrize("rule_name,D", [("U1", 128), ("SU2", 256)])
+def test_network_env_update_matches_workload_and_oracle(rt, rule_name, D):
+    from tenet_amd.network import LabelOrderOptimizer
+    wl = MPSEnvUpdate(rt, rule_name, D, "f64")
+    ref = wl.run().to_numpy()
+    tensors = [wl.L, wl.A, wl.W, wl.A]
+    net = _env_network()
+    explicit = net.plan_with(tensors, [(0, 1), (4, 2), (3, 5)])
+    got = explicit.execute(tensors)
+    assert got.space == wl.out.space
+    assert rel(got.to_numpy(), ref) < 1e-13
+    L, A, W = _o(wl.L), _o(wl.A), _o(wl.W)
+    t1 = L.contract([2], A, [0], [0, 1, 2, 3], 2)
+    t2 = t1.contract([1, 2], W, [3, 1], [0, 2, 3, 1], 2)
+    want = A.adjoint().contract([2, 0], t2, [0, 2], [0, 1, 2], 2).data
+    for planned in (net.plan(tensors), net.plan(tensors, LabelOrderOptimizer(["w", "k", "s", "t", "p"]))):
+        out = planned.execute(tensors)
+        assert out.space == wl.out.space
+        assert rel(out.to_numpy(), want) < 1e-13
+
+
+@pytest.mark.parametrize("rule_name", ["U1", "SU2"])
+def test_network_output_permutation_and_split(rt, rule_name):
+    from tenet_amd.network import Network
+    from descs import spaces
+    leg = [(-1, 3), (0, 4), (1, 2)] if rule_name == "U1" else [(0, 3), (1, 2), (2, 2)]
+    ps, _ = spaces(rule_name, [(leg, False), (leg, False)], [(leg, False), (leg, False)])
+    from tenet_amd.tensor import TensorMap
+    A = TensorMap.rand(rt, ps, seed=1)
+    B = TensorMap.rand(rt, ps, seed=2)
+    # C[e, b; a, f] = A[a, b; c, d] B[c, d; e, f]  (open labels reordered and re-split)
+    net = Network([["a", "b", "c", "d"], ["c", "d", "e", "f"]], [False, False], [2, 2], ["e", "b", "a", "f"], 2)
+    got = net.plan([A, B]).execute([A, B]).to_numpy()
+    want = _o(A).contract([2, 3], _o(B), [0, 1], [2, 1, 0, 3], 2).data
+    assert rel(got, want) < 1e-13
