@@ -61,15 +61,15 @@ struct SplitJob {
 };
 
 // Tile shapes: wave tile (16 WM) x (16 WN), 2 x 2 waves per workgroup.
-enum SplitShape { SPLIT_64x64 = 0, SPLIT_128x64 = 1, SPLIT_64x128 = 2, SPLIT_SHAPES = 3 };
-constexpr int shape_wm(int c) { return c == SPLIT_128x64 ? 4 : 2; }
-constexpr int shape_wn(int c) { return c == SPLIT_64x128 ? 4 : 2; }
+enum SplitShape { SPLIT_64x64 = 0, SPLIT_128x64 = 1, SPLIT_64x128 = 2, SPLIT_128x128 = 3, SPLIT_SHAPES = 4 };
+constexpr int shape_wm(int c) { return (c == SPLIT_128x64 || c == SPLIT_128x128) ? 4 : 2; }
+constexpr int shape_wn(int c) { return (c == SPLIT_64x128 || c == SPLIT_128x128) ? 4 : 2; }
 
 struct SplitTables {
   SplitJob* jobs = nullptr;
   GTile* tiles = nullptr;
   int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid
-  uint32_t tile_start[SPLIT_SHAPES] = {0, 0, 0}, tile_count[SPLIT_SHAPES] = {0, 0, 0};
+  uint32_t tile_start[SPLIT_SHAPES] = {}, tile_count[SPLIT_SHAPES] = {};
   uint32_t items_a = 0, items_b = 0;
   int64_t frags_a = 0, frags_b = 0, a_rows = 0, b_cols = 0, precs = 0;
 };
@@ -473,7 +473,8 @@ static void launch_split_s(const SplitTables& t, const SplitGemmArgs& a0, hipStr
     } else {
       if (c == SPLIT_64x64) hipLaunchKernelGGL((split_gemm_kernel<S, 2, 2>), grid, dim3(256), 0, s, a);
       else if (c == SPLIT_128x64) hipLaunchKernelGGL((split_gemm_kernel<S, 4, 2>), grid, dim3(256), 0, s, a);
-      else hipLaunchKernelGGL((split_gemm_kernel<S, 2, 4>), grid, dim3(256), 0, s, a);
+      else if (c == SPLIT_64x128) hipLaunchKernelGGL((split_gemm_kernel<S, 2, 4>), grid, dim3(256), 0, s, a);
+      else if constexpr (S <= 5) hipLaunchKernelGGL((split_gemm_kernel<S, 4, 4>), grid, dim3(256), 0, s, a);
     }
   }
 }
@@ -497,6 +498,7 @@ static void launch_digits_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
 // (measured: K ~ 7000 +12%, K ~ 400 -15%), and only where the job fills them.
 static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
   if (S >= 6 || total_64x64_tiles < 512 || j.k < 1024) return SPLIT_64x64;
+  if (j.m >= 256 && j.n >= 256) return SPLIT_128x128;
   if (j.m >= 128 && j.m >= j.n) return SPLIT_128x64;
   if (j.n >= 128) return SPLIT_64x128;
   return SPLIT_64x64;
