@@ -1,13 +1,14 @@
 // Split-BF16 (Ozaki-style) block GEMM for gfx950.
 //
-// FP64 C = op(A) op(B) from BF16 matrix-core products:
-//   1. per row i of op(A) an exponent e_i with max_k |A_ik| < 2^e_i, per
-//      column j of op(B) an exponent f_j;
-//   2. a = A_ik 2^-e_i in (-1, 1) is cut into S base-256 digits by repeated
+// FP64 C = op(A) op(B) from BF16 matrix-core products, K cut into chunks of
+// 256 (one chunk = one exact FP32 window of the GEMM):
+//   1. per row i of op(A) and chunk c an exponent e_ic with max_{k in c}
+//      |A_ik| < 2^e_ic, per column j of op(B) and chunk c an exponent f_jc;
+//   2. a = A_ik 2^-e_ic in (-1, 1) is cut into S base-256 digits by repeated
 //      round-to-nearest: d_1 in [-256, 256], d_p in [-128, 128] for p > 1 --
 //      small integers, exact in BF16.  Plane p stores d_p 2^-8(p-1) (an exact
-//      exponent shift), so a = 2^(e_i - 8) sum_p plane_p;
-//   3. C_ij = 2^(e_i + f_j - 16) sum_{p+q <= S-1 (0-based)} plane_p(A) plane_q(B).
+//      exponent shift), so a = 2^(e_ic - 8) sum_p plane_p;
+//   3. C_ij = sum_c 2^(e_ic + f_jc - 16) sum_{p+q <= S-1 (0-based)} plane_p(A) plane_q(B).
 //      Products are summed on v_mfma_f32_16x16x32_bf16 into FP32 accumulator
 //      groups by level l = p + q: l = 0 and l = 1 each have their own group;
 //      within a K window of 256 every partial sum of those two groups is a
@@ -15,7 +16,8 @@
 //      |d_p| <= 2^7), so FP32 holds it exactly.
 //      All remaining levels share one group (two for S = 6); their FP32
 //      rounding is 2^-24 relative to a level-2 sum, i.e. about 2^-40 of the
-//      result.  After each window the groups are added into FP64.
+//      result.  After each window the groups are scaled by the window's
+//      2^(e_ic + f_jc - 16) and added into FP64.
 // Truncating the digit expansion (relative 2^-8S per operand) and dropping
 // the levels l > S-1 are the only other errors: S = 5 gives relative
 // Frobenius errors of a few 1e-12 on normal data.
@@ -51,13 +53,12 @@ constexpr int WINDOW_STEPS = 8;   // 256 k per exact FP32 window
 struct SplitJob {
   int64_t d_off;     // element offset of C from dbase
   int64_t fa0, fb0;  // first fragment of this job's A / B planes
-  int64_t ea0, eb0;  // first exponent of this job's A rows / B columns
+  int64_t ea0, eb0;  // first exponent of this job's A rows / B columns (chunk-major: ea0 + c * m + i)
   int32_t m, n, ns;  // ns = number of 32-k steps
   int32_t ldd;
   int32_t lba, lbb;  // line blocks of 16 (A rows, B columns)
   int32_t cfg;
-  int32_t nkc;       // K chunks of the pre-pass
-  int64_t pa0, pb0;  // first partial-exponent record (16 ints) of this job's A / B items
+  int32_t nkc;       // K chunks (= exact windows)
 };
 
 // Tile shapes: wave tile (16 WM) x (16 WN), 2 x 2 waves per workgroup.
@@ -73,11 +74,11 @@ struct SplitTables {
   int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid
   uint32_t tile_start[SPLIT_SHAPES] = {}, tile_count[SPLIT_SHAPES] = {};
   uint32_t items_a = 0, items_b = 0;
-  int64_t frags_a = 0, frags_b = 0, a_rows = 0, b_cols = 0, precs = 0;
+  int64_t frags_a = 0, frags_b = 0, a_exps = 0, b_exps = 0;
 };
 
 struct SplitWorkspace {
-  void* planes = nullptr;  // A planes | B planes (bf16) | A exps | B exps | partial exps (int32)
+  void* planes = nullptr;  // A planes | B planes (bf16) | A exps | B exps (int32)
   size_t bytes = 0;
   std::unordered_map<std::string, SplitTables> tables;
   std::vector<void*> table_allocs;
@@ -91,21 +92,19 @@ SplitWorkspace* split_workspace_create() { return new SplitWorkspace(); }
 void split_workspace_destroy(SplitWorkspace* ws) { delete ws; }
 
 // ---------------------------------------------------------------------------
-// Pre-pass: two launches over items (operand side, job, block of 16 lines,
-// chunk of 256 k), both operands in one grid.
+// Pre-pass: one launch over items (operand side, job, block of 16 lines,
+// chunk of 256 k), both operands in one grid.  An item reads its 16 x 256
+// elements once into registers, takes the 16 line maxima of the chunk,
+// stores their exponents and writes the digit fragments.
 //   element (line, kk): side 0 -> op(A)[line][kk], side 1 -> op(B)[kk][line]
-// split_maxexp writes, per item, the exponents of the 16 line maxima of its
-// chunk (read coalesced along whichever index is contiguous in the source);
-// split_digits combines the chunk exponents of its line block, stores the
-// line exponents (chunk 0) and cuts its chunk into digit fragments.
 // ---------------------------------------------------------------------------
-constexpr int KCHUNK = 256;
-constexpr int EXP_NONE = -4096;  // exponent record of an all-zero line chunk
+constexpr int KCHUNK = KSTEP * WINDOW_STEPS;  // 256: pre-pass chunk = exact window
+constexpr int CHUNK_STEPS = KCHUNK / KSTEP;    // 8 fragments of k per chunk
+constexpr int EXP_NONE = -4096;                // exponent of an all-zero line chunk
 
 struct PrepSide {
   uintptr_t base;          // operand base
-  int32_t* exps;           // line exponents of this side
-  int32_t* pexp;           // partial exponents (16 per item)
+  int32_t* exps;           // line x chunk exponents of this side
   __bf16* planes;          // fragments of this side
   const int64_t* prefix;   // n_jobs + 1 entries: first item of each job
 };
@@ -125,11 +124,6 @@ __device__ __forceinline__ int find_job(const int64_t* prefix, int n, int64_t b)
     if (prefix[mid] <= b) lo = mid; else hi = mid;
   }
   return lo;
-}
-
-__device__ __forceinline__ bool line_contiguous(const GJob& g, int side) {
-  // identity op(A) (column-major rows) and adjoint op(B) are contiguous along lines
-  return side == 0 ? !(g.ops & 1) : ((g.ops & 2) != 0);
 }
 
 __device__ __forceinline__ int64_t elem_index(const GJob& g, int side, int line, int kk) {
@@ -153,37 +147,14 @@ __device__ __forceinline__ PrepItem prep_item(const PrepArgs& x) {
   return it;
 }
 
-__global__ __launch_bounds__(256) void split_maxexp(PrepArgs x) {
-  __shared__ double red[16][17];
-  const PrepItem it = prep_item(x);
-  const GJob g = x.jobs[it.job];
-  const SplitJob s = x.sj[it.job];
-  const PrepSide& sd = x.side[it.side];
-  const int lines = it.side == 0 ? g.m : g.n;
-  const double* M = reinterpret_cast<const double*>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
-  const int t = threadIdx.x;
-  const bool lc = line_contiguous(g, it.side);
-  const int li = lc ? (t & 15) : (t >> 4), k0 = lc ? (t >> 4) : (t & 15);
-  const int line = it.lb * 16 + li;
-  const int kend = min(g.k, (it.kc + 1) * KCHUNK);
-  double mx = 0.0;
-  if (line < lines)
-    for (int kk = it.kc * KCHUNK + k0; kk < kend; kk += 16) mx = fmax(mx, fabs(M[elem_index(g, it.side, line, kk)]));
-  red[li][k0] = mx;
-  __syncthreads();
-  if (t < 16) {
-    double m2 = 0.0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) m2 = fmax(m2, red[t][c]);
-    int e = EXP_NONE;
-    if (m2 > 0.0) frexp(m2, &e);  // m2 = f 2^e, f in [0.5, 1): |x| < 2^e
-    const int64_t rec = (it.side == 0 ? s.pa0 : s.pb0) + (int64_t)it.lb * s.nkc + it.kc;
-    sd.pexp[rec * 16 + t] = e;
-  }
-}
-
+// Wave w handles the chunk's k steps w and w + 4; lane l holds line l % 16
+// and k = 8 (l / 16) .. + 7 of a step -- the fragment order, so each lane
+// writes 16 bytes per plane.  For sources contiguous along lines, the 16 lanes
+// of one k read 128 contiguous bytes.
 template <int S>
 __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
+  static_assert(CHUNK_STEPS == 8, "two k steps per wave");
+  __shared__ double red[4][4][16];
   __shared__ int sexp[16];
   const PrepItem it = prep_item(x);
   const GJob g = x.jobs[it.job];
@@ -192,38 +163,53 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   const int lines = it.side == 0 ? g.m : g.n;
   const int K = g.k;
   const double* M = reinterpret_cast<const double*>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
-  const int t = threadIdx.x;
-  if (t < 16) {
-    const int64_t rec0 = (it.side == 0 ? s.pa0 : s.pb0) + (int64_t)it.lb * s.nkc;
-    int e = EXP_NONE;
-    for (int c = 0; c < s.nkc; ++c) e = max(e, sd.pexp[(rec0 + c) * 16 + t]);
-    if (e == EXP_NONE) e = 0;  // all-zero line: digits are zero whatever e is
-    sexp[t] = e;
-    const int line = it.lb * 16 + t;
-    if (it.kc == 0 && line < lines) sd.exps[(it.side == 0 ? s.ea0 : s.eb0) + line] = e;
-  }
-  __syncthreads();
-  const int lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
-  const int e = sexp[li];
-  const int64_t f0 = (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)it.lb * s.ns * S;
-  const int st_end = min(s.ns, (it.kc + 1) * (KCHUNK / KSTEP));
-  for (int st = it.kc * (KCHUNK / KSTEP) + wave; st < st_end; st += 4) {
-    double v[8];
+  double v[2][8];
+  double mx = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int kk = st * KSTEP + kg * 8 + q;
-      v[q] = (line < lines && kk < K) ? ldexp(M[elem_index(g, it.side, line, kk)], -e) : 0.0;
+      v[h][q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
+      mx = fmax(mx, fabs(v[h][q]));
     }
+  }
+  red[wave][kg][li] = mx;
+  __syncthreads();
+  if (t < 16) {
+    double m2 = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[w][c][t]);
+    int e = EXP_NONE;
+    if (m2 > 0.0) frexp(m2, &e);  // m2 = f 2^e, f in [0.5, 1): |x| < 2^e
+    if (e == EXP_NONE) e = 0;     // all-zero line chunk: its digits are zero
+    sexp[t] = e;
+    if (it.lb * 16 + t < lines)
+      sd.exps[(it.side == 0 ? s.ea0 : s.eb0) + (int64_t)it.kc * lines + it.lb * 16 + t] = e;
+  }
+  __syncthreads();
+  const int e = sexp[li];
+  const int64_t f0 = (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)it.lb * s.ns * S;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
+    if (st >= s.ns) continue;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[h][q] = ldexp(v[h][q], -e);
 #pragma unroll
     for (int p = 0; p < S; ++p) {
       bf16x8 o;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        v[q] *= 256.0;
-        const double d = rint(v[q]);
-        v[q] -= d;
+        v[h][q] *= 256.0;
+        const double d = rint(v[h][q]);
+        v[h][q] -= d;
         o[q] = (__bf16)(float)ldexp(d, -8 * p);
       }
       *reinterpret_cast<bf16x8*>(sd.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
@@ -366,7 +352,10 @@ struct SplitKernel {
         for (int j = 0; j < WN; ++j) acc[l][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
 
-  static __device__ __forceinline__ void flush(f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4]) {
+  // acc64 += 2^(e_row + f_col - 16) (sum of the window's groups); ea/eb are the
+  // window's exponents of this lane's rows and columns.
+  static __device__ __forceinline__ void flush(f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4],
+                                               const int (&ea)[WM][4], const int (&eb)[WN]) {
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
@@ -376,7 +365,7 @@ struct SplitKernel {
           double s = 0.0;
 #pragma unroll
           for (int l = NACC - 1; l >= 0; --l) s += (double)acc[l][i][j][r];
-          acc64[i][j][r] += s;
+          acc64[i][j][r] = fma(s, ldexp(1.0, ea[i][r] + eb[j] - 16), acc64[i][j][r]);
         }
   }
 
@@ -424,32 +413,47 @@ __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs 
   const uint32_t a0 = lds_address(&A0[0][0]), b0 = lds_address(&B0[0][0]);
   const uint32_t a1 = lds_address(&A1[0][0]), b1 = lds_address(&B1[0][0]);
   K::load_stage(cp, 0, a0, b0, wave, lane * 16);
-  // windows of WINDOW_STEPS (even) steps: every window starts on stage buffer 0
-  for (int w0 = 0; w0 < ns; w0 += WINDOW_STEPS) {
+  const int m0 = tile.tm * K::LBM * 16 + wm * WM * 16, n0 = tile.tn * K::LBN * 16 + wn * WN * 16;
+  // windows of WINDOW_STEPS (even) steps = pre-pass chunks: every window starts
+  // on stage buffer 0 and has its own row / column exponents
+  for (int w0 = 0, kc = 0; w0 < ns; w0 += WINDOW_STEPS, ++kc) {
     const int w1 = min(ns, w0 + WINDOW_STEPS);
     K::zero(acc);
     for (int s = w0; s < w1; s += 2) {
       K::step(cp, ns, s, A0, B0, a1, b1, wave, lane, acc);
       if (s + 1 < w1) K::step(cp, ns, s + 1, A1, B1, a0, b0, wave, lane, acc);
     }
-    K::flush(acc, acc64);
+    int ea[WM][4], eb[WN];
+    const int32_t* ra = g.a_exp + job.ea0 + (int64_t)kc * job.m;
+    const int32_t* rb = g.b_exp + job.eb0 + (int64_t)kc * job.n;
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + i * 16 + (lane >> 4) * 4 + r;
+        ea[i][r] = row < job.m ? ra[row] : 0;
+      }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int col = n0 + j * 16 + (lane & 15);
+      eb[j] = col < job.n ? rb[col] : 0;
+    }
+    K::flush(acc, acc64, ea, eb);
   }
 
-  // epilogue: C = alpha 2^(e_i + f_j - 16) acc64 (+ beta C)
+  // epilogue: C = alpha acc64 (+ beta C)
   double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
-  const int m0 = tile.tm * K::LBM * 16 + wm * WM * 16, n0 = tile.tn * K::LBN * 16 + wn * WN * 16;
 #pragma unroll
   for (int j = 0; j < WN; ++j) {
     const int col = n0 + j * 16 + (lane & 15);
     if (col >= job.n) continue;
-    const int fcol = g.b_exp[job.eb0 + col] - 16;
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + i * 16 + (lane >> 4) * 4 + r;
         if (row < job.m) {
-          const double v = g.alpha * ldexp(acc64[i][j][r], g.a_exp[job.ea0 + row] + fcol);
+          const double v = g.alpha * acc64[i][j][r];
           double* p = D + row + (int64_t)col * job.ldd;
           *p = (g.beta != 0.0) ? v + g.beta * *p : v;
         }
@@ -543,18 +547,14 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       x.fb0 = t.frags_b;
       t.frags_a += (int64_t)x.lba * x.ns * S;
       t.frags_b += (int64_t)x.lbb * x.ns * S;
-      x.ea0 = t.a_rows;
-      x.eb0 = t.b_cols;
-      t.a_rows += j.m;
-      t.b_cols += j.n;
-      x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
       x.nkc = (std::max(j.k, 1) + KCHUNK - 1) / KCHUNK;
+      x.ea0 = t.a_exps;
+      x.eb0 = t.b_exps;
+      t.a_exps += (int64_t)j.m * x.nkc;
+      t.b_exps += (int64_t)j.n * x.nkc;
+      x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
       // operands of a job with an empty output are never read: no pre-pass items
       const int64_t ia = (j.n > 0) ? (int64_t)x.lba * x.nkc : 0, ib = (j.m > 0) ? (int64_t)x.lbb * x.nkc : 0;
-      x.pa0 = t.precs;
-      t.precs += ia;
-      x.pb0 = t.precs;
-      t.precs += ib;
       sj[i] = x;
       pre_a.push_back(pre_a.back() + ia);
       pre_b.push_back(pre_b.back() + ib);
@@ -595,9 +595,8 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     it = ws->tables.emplace(key, t).first;
   }
   const SplitTables& t = it->second;
-  // workspace: A fragments | B fragments (bf16) | A exps | B exps | partial exps (int32)
-  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) +
-                       4 * (size_t)(t.a_rows + t.b_cols + 16 * t.precs) + 256;
+  // workspace: A fragments | B fragments (bf16) | A exps | B exps (int32, line x chunk)
+  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) + 4 * (size_t)(t.a_exps + t.b_exps) + 256;
   if (bytes > ws->bytes) {
     if (capturing) {
       err = "split-BF16 workspace must be sized before graph capture; run the call once first";
@@ -613,13 +612,11 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
   __bf16* ap = static_cast<__bf16*>(ws->planes);
   __bf16* bp = ap + (size_t)FRAG * t.frags_a;
   int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)FRAG * t.frags_b);
-  int32_t* be = ae + t.a_rows;
-  int32_t* pe = be + t.b_cols;
+  int32_t* be = ae + t.a_exps;
   const int n_jobs = (int)host_jobs.size();
   PrepArgs pa{dev_jobs, t.jobs, n_jobs, t.items_a,
-              {{a, ae, pe, ap, t.prefix}, {b, be, pe, bp, t.prefix + (n_jobs + 1)}}};
+              {{a, ae, ap, t.prefix}, {b, be, bp, t.prefix + (n_jobs + 1)}}};
   const uint32_t items = t.items_a + t.items_b;
-  if (items) hipLaunchKernelGGL(split_maxexp, dim3(items), dim3(256), 0, s, pa);
   SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ae, be, alpha, beta};
   switch (S) {
     case 2: launch_digits_s<2>(pa, items, s); launch_split_s<2>(t, g, s); break;
