@@ -64,8 +64,11 @@ static hipEvent_t take_event(tnt_ctx* ctx) {
     ctx->event_pool.pop_back();
     return e;
   }
+  // Timing-only events: no system-scope fence at record time (it writes back
+  // and invalidates caches, which both slows the following kernel and widens
+  // the gap between kernels); stats synchronise the stream before reading.
   hipEvent_t e = nullptr;
-  (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 

@@ -81,7 +81,33 @@ def mps_env_spaces(rule_name="U1", D=4096):
     return rule, V, P, M
 
 
-class MPSEnvUpdate:
+class GraphStep:
+    """Replays a workload step from a hipGraph: the step's launches are
+    recorded once (every table and buffer already exists after one eager run)
+    and each replay is one graph launch, without per-kernel launch gaps."""
+
+    def capture(self):
+        import ctypes as C
+        from . import _native as N
+        self.run()
+        self.rt.synchronize()
+        h = N.hip()
+        self.rt._check(h.tnt_graph_begin(self.rt._ctx))
+        try:
+            self.run()
+        finally:
+            g = C.c_void_p()
+            self.rt._check(h.tnt_graph_end(self.rt._ctx, C.byref(g)))
+        self.graph = g
+        return self
+
+    def replay(self):
+        from . import _native as N
+        self.rt._check(N.hip().tnt_graph_launch(self.rt._ctx, self.graph))
+        return self.out
+
+
+class MPSEnvUpdate(GraphStep):
     """Left-environment update L' = A^dagger . ((L . A) . W) on one runtime."""
 
     def __init__(self, rt, rule_name="U1", D=4096, precision="f64", seed=3):
@@ -126,7 +152,7 @@ def su2_fmove_space(mult=64, twoj_max=8):
     return rule, V
 
 
-class SU2FMoveContract:
+class SU2FMoveContract(GraphStep):
     """C = contract(A, [1, 3], B, [2, 0]) with recoupling on both operands."""
 
     def __init__(self, rt, mult=64, twoj_max=8, precision="f64"):

@@ -115,3 +115,18 @@ def test_network_output_permutation_and_split(rt, rule_name):
     got = net.plan([A, B]).execute([A, B]).to_numpy()
     want = _o(A).contract([2, 3], _o(B), [0, 1], [2, 1, 0, 3], 2).data
     assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["f64", "split_bf16"])
+def test_env_update_graph_replays_match_eager(rt, prec):
+    """Every replay of the captured step equals the eager step bit for bit,
+    with eager work and synchronisations between the replays."""
+    wl = MPSEnvUpdate(rt, "U1", 256, prec)
+    eager = wl.run().to_numpy().copy()
+    wl.capture()
+    for _ in range(3):
+        got = wl.replay().to_numpy()
+        assert np.array_equal(got, eager)
+        wl.run()
+        rt.synchronize()
+        assert np.array_equal(wl.out.to_numpy(), eager)
