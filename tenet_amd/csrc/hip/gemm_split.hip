@@ -506,7 +506,7 @@ static void launch_digits_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
 // (LDS) against two for 64x64; they win only when a tile runs many k steps
 // (measured: K ~ 7000 +12%, K ~ 400 -15%), and only where the job fills them.
 static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
-  if (S >= 6 || total_64x64_tiles < 512 || j.k < 1024) return SPLIT_64x64;
+  if (S >= 6 || total_64x64_tiles < 512 || j.k < 1024) return SPLIT_64x64;  // 128x128 at K >= 256: -12 % (U(1) env)
   if (j.m >= 256 && j.n >= 256) return SPLIT_128x128;
   if (j.m >= 128 && j.m >= j.n) return SPLIT_128x64;
   if (j.n >= 128) return SPLIT_64x128;
