@@ -169,8 +169,9 @@ def run_serve(args):
 
     from tenet_amd.serving import HeffServer, poisson_trace
     t = time.perf_counter()
+    mpo = "heisenberg" if args.rule == "U1" else "random"
     srv = HeffServer(args.rule, n_streams=args.streams, max_inflight=args.inflight,
-                     precision=args.precision)
+                     precision=args.precision, mpo=mpo)
     srv.warm()
     cold = time.perf_counter() - t
     log(f"[bench] serve: {len(srv.buckets)} bond-dimension buckets {srv.buckets}, environments resident, "
@@ -210,8 +211,9 @@ def run_serve(args):
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)"}[args.precision]
                  + ", f64 dot",
-        "data": "synthetic (seeded normal(0,1) environments and states on device)",
-        "config": {"workload": f"serve_two_site_heff_{args.rule.lower()}", "arrivals": f"poisson {args.rate:.0f}/s",
+        "data": "synthetic (seeded normal(0,1) environments and states on device; "
+                + ("spin-1/2 Heisenberg MPO)" if mpo == "heisenberg" else "seeded normal(0,1) MPO tensors)"),
+        "config": {"workload": f"serve_two_site_heff_{args.rule.lower()}", "mpo": mpo, "arrivals": f"poisson {args.rate:.0f}/s",
                    "bond_dims": "lognormal median 512 sigma 0.7 clipped 32..4096, snapped to buckets",
                    "buckets": srv.buckets, "streams": args.streams, "slots": args.inflight,
                    "mean_D": float(ds.mean())},

@@ -26,7 +26,10 @@ from .workloads import mps_env_spaces
 class TwoSiteHeff:
     """Environments of one bond dimension, resident in HBM."""
 
-    def __init__(self, rt, rule_name="U1", D=512, seed=7, precision="f64"):
+    def __init__(self, rt, rule_name="U1", D=512, seed=7, precision="f64", mpo="random"):
+        """Environments L, R of bond dimension D (seeded normal(0,1)); MPO tensors
+        W1 = W2 either seeded normal(0,1) ("random") or the spin-1/2 Heisenberg
+        MPO ("heisenberg", U(1) only)."""
         self.rt = rt
         self.rule, V, P, M = mps_env_spaces(rule_name, D)
         self.V, self.P, self.M = V, P, M
@@ -34,9 +37,26 @@ class TwoSiteHeff:
         self.precision = precision
         self.L = TensorMap.rand(rt, [V, M], [V], seed=seed, normal=True)
         self.R = TensorMap.rand(rt, [V], [V, M], seed=seed + 1, normal=True)
-        self.W1 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 2, normal=True)
-        self.W2 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 3, normal=True)
+        if mpo == "heisenberg":
+            if rule_name != "U1":
+                raise ValueError("the Heisenberg MPO is built in U(1) form only")
+            from .workloads import heisenberg_mpo
+            self.W1 = self.W2 = heisenberg_mpo(rt)
+        else:
+            self.W1 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 2, normal=True)
+            self.W2 = TensorMap.rand(rt, [M, P], [P, M], seed=seed + 3, normal=True)
         self.theta_space = TensorMap.zeros(rt, [V], [P, P, V]).space
+
+    @classmethod
+    def from_tensors(cls, rt, L, W1, W2, R, precision="f64"):
+        """H_eff of given environments L in Vl (x) M <- Vl, R in Vr <- Vr (x) M and
+        MPO tensors W in M (x) P <- P (x) M; theta lives in Vl <- P (x) P (x) Vr."""
+        h = object.__new__(cls)
+        h.rt, h.precision = rt, precision
+        h.L, h.W1, h.W2, h.R = L, W1, W2, R
+        P = W1.space.codomain[1]
+        h.theta_space = TensorMap.zeros(rt, [L.space.codomain[0]], [P, P, R.space.codomain[0]]).space
+        return h
 
     def gram(self, theta, scratch=None):
         """G theta = L^T (L theta): a real symmetric positive semidefinite operator on

@@ -56,7 +56,7 @@ class _Req:
 
 class HeffServer:
     def __init__(self, rule="U1", buckets=None, n_streams=4, max_inflight=16,
-                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7):
+                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7, mpo="random"):
         self.rule = rule
         self.precision = precision
         self.max_inflight = max_inflight
@@ -66,7 +66,7 @@ class HeffServer:
         self.env = {}       # D -> TwoSiteHeff on context 0
         self.views = {}     # (D, ctx index) -> environment views on that context
         for D in self.buckets:
-            self.env[D] = TwoSiteHeff(self.rts[0], rule, D, seed=env_seed + D, precision=precision)
+            self.env[D] = TwoSiteHeff(self.rts[0], rule, D, seed=env_seed + D, precision=precision, mpo=mpo)
         self.rts[0].synchronize()  # the other streams read the environments
         self.flops = {}
         self.fp = {}

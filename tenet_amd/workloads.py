@@ -81,6 +81,41 @@ def mps_env_spaces(rule_name="U1", D=4096):
     return rule, V, P, M
 
 
+def heisenberg_mpo_payload(J=1.0):
+    """Spin-1/2 Heisenberg MPO in U(1) form: (HomSpace, host payload) of
+    W in M (x) P <- P (x) M with legs W[w_r, s_in; s_out, w_l] =
+    <s_out| O(w_l, w_r) |s_in> and the lower-triangular operator matrix
+        O(0,0) = I,  O(1,0) = S+,  O(2,0) = S-,  O(3,0) = Sz,
+        O(4,1) = J/2 S-,  O(4,2) = J/2 S+,  O(4,3) = J Sz,  O(4,4) = I,
+    so that left channel 4 and right channel 0 close the chain to
+    sum over bonds of J S_i . S_(i+1).  Channels on M = {0: 3, -1: 1, +1: 1}:
+    charge 0 holds channels 0, 3, 4 (in degeneracy order), charge -1 channel 1
+    (it carries S+), charge +1 channel 2; P = {0: down, 1: up}."""
+    rule, _, P, M = mps_env_spaces("U1", 64)
+    sp, sm = np.zeros((2, 2)), np.zeros((2, 2))
+    sp[1, 0] = 1.0  # <up| S+ |down>
+    sm[0, 1] = 1.0
+    sz, eye = np.diag([-0.5, 0.5]), np.eye(2)
+    ops = {(0, 0): eye, (1, 0): sp, (2, 0): sm, (3, 0): sz, (4, 1): 0.5 * J * sm, (4, 2): 0.5 * J * sp,
+           (4, 3): J * sz, (4, 4): eye}
+    chan = {0: (0, 3, 4), -1: (1,), 1: (2,)}
+    space = TensorMap._space([M, P], [P, M])
+    data = np.zeros(space.len)
+    for off, shape, strides, _, unc in space.blocks():
+        qr, s_in, s_out, ql = (rule.label(x) for x in unc)
+        for idx in np.ndindex(*shape):
+            op = ops.get((chan[ql][idx[3]], chan[qr][idx[0]]))
+            if op is not None:
+                data[off + sum(i * st for i, st in zip(idx, strides))] = op[s_out, s_in]
+    return space, data
+
+
+def heisenberg_mpo(rt, J=1.0):
+    """The U(1) spin-1/2 Heisenberg MPO tensor resident on the device."""
+    space, data = heisenberg_mpo_payload(J)
+    return TensorMap.from_numpy(rt, space, data)
+
+
 class GraphStep:
     """Replays a workload step from a hipGraph: the step's launches are
     recorded once (every table and buffer already exists after one eager run)

@@ -84,6 +84,39 @@ def test_lanczos_gram_matches_dense_oracle(rt, rule_name, D):
     assert abs(ev - want) <= 1e-9 * max(1.0, abs(want)), (ev, want, res, iters)
 
 
+def _identity_env(rt, V, M, left):
+    """Boundary environment delta(a', a) delta(w, boundary channel): left closes
+    on MPO channel 4 (third slot of charge 0), right on channel 0 (first slot)."""
+    from tenet_amd.tensor import TensorMap as T
+    lab = V.rule.label
+    if left:  # L[a', w; a]
+        return T.from_block_fn(rt, [V, M], [V], lambda u, c, i: float(
+            lab(u[1]) == 0 and i[1] == 2 and u[0] == u[2] and i[0] == i[2]))
+    return T.from_block_fn(rt, [V], [V, M], lambda u, c, i: float(  # R[c; c', w]
+        lab(u[2]) == 0 and i[2] == 0 and u[0] == u[1] and i[0] == i[1]))
+
+
+def test_lanczos_heisenberg_bond_ground_energy(rt):
+    """With identity boundary environments the two-site H_eff of the Heisenberg
+    MPO is 1 (x) J S1.S2 (x) 1: symmetric, and its lowest eigenvalue is the
+    singlet energy -3/4 J whenever a sector holds one up spin."""
+    from tenet_amd.tensor import GradedSpace, Rule
+    from tenet_amd.workloads import heisenberg_mpo, mps_env_spaces
+    _, _, P, M = mps_env_spaces("U1", 64)
+    Vl = GradedSpace(Rule("U1"), [(0, 2), (1, 3)])
+    Vr = GradedSpace(Rule("U1"), [(-1, 2), (0, 1)])
+    W = heisenberg_mpo(rt, J=1.0)
+    H = TwoSiteHeff.from_tensors(rt, _identity_env(rt, Vl, M, True), W, W, _identity_env(rt, Vr, M, False))
+    x = TensorMap.rand(rt, H.theta_space, seed=1, normal=True)
+    y = TensorMap.rand(rt, H.theta_space, seed=2, normal=True)
+    assert rel(H.apply(x).to_numpy(), _oracle_apply(H, _o(x)).data) < 1e-13
+    hx, hy = H.apply(x).copy(), H.apply(y).copy()
+    assert abs(x.inner(hy) - hx.inner(y)) < 1e-12 * abs(x.inner(hy))
+    sc = {}
+    ev, res, iters = lanczos(lambda v: H.apply(v, sc), x, iters=H.theta_space.len, tol=1e-12)
+    assert abs(ev + 0.75) < 1e-10, (ev, res, iters)
+
+
 def test_dot_async_matches_sync_and_events(rt):
     H = TwoSiteHeff(rt, "U1", 64, seed=2)
     a = TensorMap.rand(rt, H.theta_space, seed=1, normal=True)
