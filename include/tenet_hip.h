@@ -226,6 +226,15 @@ int tnt_contract_destroy(tnt_ctx* ctx, tnt_artifact* art);
  * tnt_set_tile_config for the codes; -1 = table default).  The autotuner
  * measures each configuration on the artifact and keeps the fastest. */
 int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* art, int cfg);
+/* Execution route chosen at upload (no reference counterpart).  A narrow
+ * contraction -- every core job with k <= 32 contracted and n <= 16 output
+ * columns, Single-only transforms -- runs as ONE recoupling pass from the lhs
+ * payload to the destination whose coefficients are gathered from the rhs
+ * payload, instead of transform + GEMM + transform (HBM-bound either way; the
+ * fused pass moves each element once).  TENET_AMD_FUSE_NARROW=0 in the
+ * environment at upload keeps the general route. */
+enum tnt_route { TNT_ROUTE_TRANSFORM_GEMM = 0, TNT_ROUTE_FUSED_NARROW = 1 };
+int tnt_contract_route(const tnt_artifact* art, int* route);
 int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_buf* lhs,
                  const tnt_buf* rhs, double alpha, double beta, int precision);
 

@@ -109,6 +109,20 @@ const tnth_space* tnth_contract_dst_space(const tnth_contract* c);
 /* Algorithmic FP64 flops of the core GEMMs (sum 2 m k n) and of the recoupling GEMMs. */
 double tnth_contract_flops(const tnth_contract* c);
 
+/* Fused route of a narrow contraction (every core job k <= 32, n <= 16; no
+ * reference counterpart -- the reference runs such contractions as transform,
+ * per-sector GEMM, transform).  The backend plans the same route inside
+ * tnt_contract_upload; this entry point exposes the plan for inspection and
+ * CPU checks: a transform from the lhs payload to the output payload made of
+ * Multi groups whose coefficients are U[t] = scale[t] * rhs[idx[t]]
+ * (idx[t] < 0: U[t] = 0).  Returns TNT_E_UNSUPPORTED, with the failed
+ * condition in tnth_last_error(), when the contraction is not eligible. */
+typedef struct tnth_fusion tnth_fusion;
+int tnth_narrow_fusion(const tnt_contract_desc* d, tnth_fusion** out);
+void tnth_fusion_destroy(tnth_fusion* f);
+const tnt_transform_desc* tnth_fusion_desc(const tnth_fusion* f);
+size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** idx, const double** scale);
+
 /* ---- snapshots: TeNeT's v1 semantic wire format ----------------------------
  * Replace TensorMap::to_bytes_with / from_bytes_with and GradedSpace's pair
  * (tenet/src/typed/serialization.rs:912-957, :1569-1640, :1700-1740) for dense

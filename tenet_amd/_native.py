@@ -101,6 +101,7 @@ HIP_API = {
     "tnt_contract_destroy": (C.c_int, [VP, VP]),
     "tnt_contract_set_tile_config": (C.c_int, [VP, VP, C.c_int]),
     "tnt_contract": (C.c_int, [VP, VP, VP, VP, VP, C.c_double, C.c_double, C.c_int]),
+    "tnt_contract_route": (C.c_int, [VP, C.POINTER(C.c_int)]),
     "tnt_weighted_dot": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ,
                                    C.POINTER(C.c_double)]),
     "tnt_axpby": (C.c_int, [VP, VP, VP, C.c_double, C.c_double]),
@@ -160,6 +161,10 @@ HOST_API = {
     "tnth_contract_desc": (C.POINTER(ContractDesc), [VP]),
     "tnth_contract_dst_space": (VP, [VP]),
     "tnth_contract_flops": (C.c_double, [VP]),
+    "tnth_narrow_fusion": (C.c_int, [C.POINTER(ContractDesc), PP]),
+    "tnth_fusion_destroy": (None, [VP]),
+    "tnth_fusion_desc": (C.POINTER(TransformDesc), [VP]),
+    "tnth_fusion_gather": (SZ, [VP, C.POINTER(I64P), C.POINTER(C.POINTER(C.c_double))]),
     "tnth_bytes_data": (C.POINTER(C.c_uint8), [VP]),
     "tnth_bytes_size": (C.c_size_t, [VP]),
     "tnth_bytes_free": (None, [VP]),

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../../include/tenet_hip.h"
+#include "../common/narrow_fusion.hpp"
 #include "kernels.hpp"
 #include "split.hpp"
 
@@ -112,6 +114,8 @@ struct DevArrays {  // device copy of a normalised transform plan
   XItem* items = nullptr;
   double* coeffs = nullptr;
   uint32_t n_items = 0;
+  uint32_t cls_begin[XC_COUNT] = {};  // items are grouped by class (one launch each)
+  uint32_t cls_count[XC_COUNT] = {};
 };
 
 struct tnt_plan {
@@ -156,7 +160,20 @@ struct tnt_artifact {
   int64_t* unc_len = nullptr;
   uint32_t n_unc = 0;
   int lhs_op = 0, rhs_op = 0;
+  // fused narrow route: one recoupling pass lhs -> dst with coefficients
+  // gathered from the rhs payload (tnt_fuse::plan_narrow_fusion)
+  tnt_plan* fused = nullptr;
+  int64_t* g_idx = nullptr;
+  double* g_scale = nullptr;
+  uint32_t n_gather = 0;
 };
+
+// The fused narrow route is on unless TENET_AMD_FUSE_NARROW=0 (for A/B runs
+// and for tests that compare both routes).
+static bool fusion_enabled() {
+  const char* e = std::getenv("TENET_AMD_FUSE_NARROW");
+  return !(e && e[0] == '0');
+}
 
 struct tnt_graph {
   hipGraph_t graph = nullptr;
@@ -393,7 +410,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   if (!d) FAIL(ctx, TNT_E_INVALID, "null transform descriptor");
   std::vector<XBlock> blocks;
   std::vector<XLayout> layouts;
-  std::vector<XItem> items;
+  std::vector<XItem> cls_items[XC_COUNT];
   std::string err;
   double bytes = 0;
   auto add_block = [&](int kind, const std::vector<uint32_t>& dl, const std::vector<uint32_t>& sl,
@@ -441,6 +458,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     // Single blocks: kSingleItem-element items (kSingleUnroll per thread) swept in
     // order by a persistent grid; Multi: ~8K element-updates per item, weighted by
     // its layouts
+    std::vector<XItem>& items = cls_items[xclass_of(kind, (int)sl.size())];
     int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
     int64_t target = kind == XK_SINGLE ? kSingleItem : std::max<int64_t>(256, 8192 / weight);
     if (plane >= target) {
@@ -487,6 +505,12 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   p->src_len = d->src_len;
   p->bytes_moved = bytes;
   std::vector<double> coeffs(d->coeffs, d->coeffs + d->n_coeffs);
+  std::vector<XItem> items;
+  for (int c = 0; c < XC_COUNT; ++c) {
+    p->d.cls_begin[c] = (uint32_t)items.size();
+    p->d.cls_count[c] = (uint32_t)cls_items[c].size();
+    items.insert(items.end(), cls_items[c].begin(), cls_items[c].end());
+  }
   int rc = upload_vec(ctx, blocks, &p->d.blocks, p->allocs);
   if (!rc) rc = upload_vec(ctx, layouts, &p->d.layouts, p->allocs);
   if (!rc) rc = upload_vec(ctx, items, &p->d.items, p->allocs);
@@ -505,7 +529,7 @@ static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double*
                      double beta, int overwrite) {
   XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.coeffs, dst, src, alpha, beta, overwrite};
   TimedScope ts(ctx, 0, p->bytes_moved);
-  launch_xform(a, p->d.n_items, ctx->stream);
+  for (int c = 0; c < XC_COUNT; ++c) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -996,6 +1020,21 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
   a->n_unc = (uint32_t)uo.size();
   if (int rc = upload_vec(ctx, uo, &a->unc_off, a->allocs)) return fail(rc);
   if (int rc = upload_vec(ctx, ul, &a->unc_len, a->allocs)) return fail(rc);
+  if (fusion_enabled()) {
+    tnt_fuse::NarrowFusion nf;
+    if (tnt_fuse::plan_narrow_fusion(d, nf)) {
+      if (build_plan(ctx, &nf.desc, &a->fused) != TNT_OK) {
+        a->fused = nullptr;  // not expressible as a plan: keep the general route
+        ctx->err.clear();
+      } else {
+        a->n_gather = (uint32_t)nf.gather_idx.size();
+        if (int rc = upload_vec(ctx, nf.gather_idx, &a->g_idx, a->allocs)) return fail(rc);
+        if (int rc = upload_vec(ctx, nf.gather_scale, &a->g_scale, a->allocs)) return fail(rc);
+        *out = a;  // the fused route needs no scratch payloads
+        return TNT_OK;
+      }
+    }
+  }
   void* p = nullptr;
   if (a->lhs) {
     if (int rc = pool_alloc(ctx, 8 * (size_t)std::max<int64_t>(a->lhs_len, 1), &p)) return fail(rc);
@@ -1045,12 +1084,19 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
   return TNT_OK;
 }
 
+int tnt_contract_route(const tnt_artifact* a, int* route) {
+  if (!a || !route) return TNT_E_INVALID;
+  *route = a->fused ? TNT_ROUTE_FUSED_NARROW : TNT_ROUTE_TRANSFORM_GEMM;
+  return TNT_OK;
+}
+
 int tnt_contract_destroy(tnt_ctx* ctx, tnt_artifact* a) {
   if (!ctx || !a) return TNT_E_INVALID;
   (void)hipStreamSynchronize(ctx->stream);
   if (a->lhs) tnt_plan_destroy(ctx, a->lhs);
   if (a->rhs) tnt_plan_destroy(ctx, a->rhs);
   if (a->out) tnt_plan_destroy(ctx, a->out);
+  if (a->fused) tnt_plan_destroy(ctx, a->fused);
   for (auto p : a->allocs) pool_free(ctx, p);
   delete a;
   return TNT_OK;
@@ -1073,6 +1119,12 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   if (int rc = set_device(ctx)) return rc;
   const double* L = static_cast<const double*>(lhs->ptr);
   const double* R = static_cast<const double*>(rhs->ptr);
+  if (a->fused) {
+    launch_coeff_gather(a->fused->d.coeffs, R, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
+    run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
+    HIPCHK(ctx, hipGetLastError());
+    return TNT_OK;
+  }
   if (a->lhs) {
     run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
     L = a->scratch_l;

@@ -53,7 +53,19 @@ struct XformArgs {
   int overwrite;
 };
 
-void launch_xform(const XformArgs& a, uint32_t n_items, hipStream_t s);
+// Work-item classes of a plan, one kernel each: Single and Scale items, and
+// Multi items by source count (<= 4, <= 16, <= kMaxMultiSrc).
+enum XClass : int32_t { XC_SINGLE = 0, XC_MULTI4 = 1, XC_MULTI16 = 2, XC_MULTI32 = 3, XC_COUNT = 4 };
+inline int xclass_of(int kind, int nsrc) {
+  if (kind != XK_MULTI) return XC_SINGLE;
+  return nsrc <= 4 ? XC_MULTI4 : nsrc <= 16 ? XC_MULTI16 : XC_MULTI32;
+}
+// Items [begin, begin + n) of one class.
+void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s);
+// coeffs[t] = idx[t] >= 0 ? scale[t] * src[idx[t]] : 0 (coefficients of a fused
+// narrow contraction gathered from its small operand).
+void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,
+                         uint32_t n, hipStream_t s);
 
 // Device GEMM job: element offsets from the kernel's base pointers.
 struct GJob {

@@ -32,43 +32,56 @@ __device__ __forceinline__ int64_t base_offset(const XLayout& L, const int64_t* 
   return off;
 }
 
-__device__ void xform_item(const XformArgs& a, const XItem it);
-
-// Persistent grid-stride loop over work items: concurrently resident
-// workgroups process consecutive items, so the waves of the whole GPU sweep
-// one contiguous window of the payloads at a time (DRAM locality), like a
-// plain grid-stride copy.
-__global__ __launch_bounds__(256) void xform_kernel(XformArgs a, uint32_t n_items) {
-  for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) xform_item(a, a.items[i]);
+__device__ __forceinline__ void outer_index(const XBlock& b, uint64_t o, int64_t* idx) {
+  const int rank = b.rank;
+#pragma unroll
+  for (int d = 2; d < kMaxRank; ++d) {
+    if (d < rank) {
+      idx[d] = (int64_t)(o % (uint64_t)b.dims[d]);
+      o /= (uint64_t)b.dims[d];
+    } else {
+      idx[d] = 0;
+    }
+  }
 }
 
-__device__ void xform_item(const XformArgs& a, const XItem it) {
+// Lanes of the workgroup: waves split outer indices when there are enough of
+// them, else the plane.
+struct ItemSweep {
+  uint64_t o_first, o_step, o_end;
+  uint32_t e_lo, e_hi;
+  __device__ ItemSweep(const XItem& it, int wave) {
+    if (it.o_count >= 4) {
+      o_first = it.o_begin + wave;
+      o_step = 4;
+      e_lo = it.e_begin;
+      e_hi = it.e_begin + it.e_count;
+    } else {
+      o_first = it.o_begin;
+      o_step = 1;
+      const uint32_t q = (it.e_count + 3) / 4;
+      e_lo = it.e_begin + min(it.e_count, q * wave);
+      e_hi = it.e_begin + min(it.e_count, q * (wave + 1));
+    }
+    o_end = it.o_begin + it.o_count;
+  }
+};
+
+// Single (scaled strided copy) and Scale (inactive destination) items.
+__device__ void single_item(const XformArgs& a, const XItem it) {
   const XBlock& b = a.blocks[it.block];
   const int rank = b.rank;
-  const int kind = b.kind;
   const int64_t n0 = b.dims[0];
-  const int64_t n1 = rank > 1 ? b.dims[1] : 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 
-  if (it.o_count == 1 && kind == XK_SINGLE) {
+  if (it.o_count == 1 && b.kind == XK_SINGLE) {
     // one outer index, up to kSingleItem plane elements: element
     // e = e_begin + 64 * (4u + wave) + lane, u < kSingleUnroll -- the whole
     // workgroup sweeps the item front to back in 256-element steps, with all
     // kSingleUnroll loads of a lane in flight before its stores
     int64_t idx[kMaxRank];
-    {
-      uint64_t rr = it.o_begin;
-#pragma unroll
-      for (int d = 2; d < kMaxRank; ++d) {
-        if (d < rank) {
-          idx[d] = (int64_t)(rr % (uint64_t)b.dims[d]);
-          rr /= (uint64_t)b.dims[d];
-        } else {
-          idx[d] = 0;
-        }
-      }
-    }
+    outer_index(b, it.o_begin, idx);
     const XLayout& Ld = a.layouts[b.lay];
     const XLayout& Ls = a.layouts[b.lay + 1];
     const int64_t od = base_offset(Ld, idx, rank), os = base_offset(Ls, idx, rank);
@@ -108,45 +121,18 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
     return;
   }
 
-  // Waves split outer indices when there are enough of them, else the plane.
-  uint64_t o_first, o_step;
-  uint32_t e_lo, e_hi;
-  if (it.o_count >= 4) {
-    o_first = it.o_begin + wave;
-    o_step = 4;
-    e_lo = it.e_begin;
-    e_hi = it.e_begin + it.e_count;
-  } else {
-    o_first = it.o_begin;
-    o_step = 1;
-    uint32_t q = (it.e_count + 3) / 4;
-    e_lo = it.e_begin + min(it.e_count, q * wave);
-    e_hi = it.e_begin + min(it.e_count, q * (wave + 1));
-  }
-  const uint64_t o_end = it.o_begin + it.o_count;
+  const ItemSweep w(it, wave);
   // incremental plane walk: e -> (x, y) with e = x + n0 * y; step 64
   const int64_t dx = 64 % n0, dy = 64 / n0;
   const double alpha = a.alpha, beta = a.beta;
-  const bool ow = a.overwrite != 0;
-  const bool readd = !ow && beta != 0.0;
+  const bool readd = a.overwrite == 0 && beta != 0.0;
 
-  for (uint64_t o = o_first; o < o_end; o += o_step) {
+  for (uint64_t o = w.o_first; o < w.o_end; o += w.o_step) {
     int64_t idx[kMaxRank];
-    {
-      uint64_t rr = o;
-#pragma unroll
-      for (int d = 2; d < kMaxRank; ++d) {
-        if (d < rank) {
-          idx[d] = (int64_t)(rr % (uint64_t)b.dims[d]);
-          rr /= (uint64_t)b.dims[d];
-        } else {
-          idx[d] = 0;
-        }
-      }
-    }
-    uint32_t e = e_lo + lane;
+    outer_index(b, o, idx);
+    uint32_t e = w.e_lo + lane;
     int64_t x = e % n0, y = e / n0;
-    if (kind == XK_SINGLE) {
+    if (b.kind == XK_SINGLE) {
       const XLayout& Ld = a.layouts[b.lay];
       const XLayout& Ls = a.layouts[b.lay + 1];
       const int64_t od = base_offset(Ld, idx, rank), os = base_offset(Ls, idx, rank);
@@ -156,13 +142,13 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
       const double* __restrict__ src = a.src;
       double* __restrict__ dst = a.dst;
       // four independent loads in flight per lane before their stores
-      while (e < e_hi) {
+      while (e < w.e_hi) {
         double v[kUnroll];
         int64_t dp[kUnroll];
         int cnt = 0;
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          if (e < e_hi) {
+          if (e < w.e_hi) {
             v[u] = src[os + x * s0 + y * s1];
             dp[u] = od + x * d0 + y * d1;
             cnt = u + 1;
@@ -180,42 +166,11 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
           }
         }
       }
-    } else if (kind == XK_MULTI) {
-      const int nd = b.ndst, ns = b.nsrc;
-      const XLayout* L = a.layouts + b.lay;
-      const double* U = a.coeffs + b.coeff;
-      for (; e < e_hi; e += 64) {
-        double v[kMaxMultiSrc];
-#pragma unroll
-        for (int i = 0; i < kMaxMultiSrc; ++i) {
-          if (i < ns) {
-            const XLayout& Ls = L[nd + i];
-            const int64_t off = base_offset(Ls, idx, rank) + x * Ls.strides[0] +
-                                (rank > 1 ? y * Ls.strides[1] : 0);
-            v[i] = a.src[off];
-          } else {
-            v[i] = 0.0;
-          }
-        }
-        for (int j = 0; j < nd; ++j) {
-          double acc = 0.0;
-#pragma unroll
-          for (int i = 0; i < kMaxMultiSrc; ++i)
-            if (i < ns) acc = fma(U[j * ns + i], v[i], acc);
-          const XLayout& Ld = L[j];
-          double* p = a.dst + base_offset(Ld, idx, rank) + x * Ld.strides[0] +
-                      (rank > 1 ? y * Ld.strides[1] : 0);
-          *p = readd ? alpha * acc + beta * *p : alpha * acc;
-        }
-        x += dx;
-        y += dy;
-        if (x >= n0) { x -= n0; y += 1; }
-      }
     } else {  // XK_SCALE: inactive destination block
       const XLayout& Ld = a.layouts[b.lay];
       const int64_t od = base_offset(Ld, idx, rank);
       const int64_t d0 = Ld.strides[0], d1 = rank > 1 ? Ld.strides[1] : 0;
-      for (; e < e_hi; e += 64) {
+      for (; e < w.e_hi; e += 64) {
         double* p = a.dst + od + x * d0 + y * d1;
         *p = readd ? beta * *p : 0.0;
         x += dx;
@@ -224,13 +179,165 @@ __device__ void xform_item(const XformArgs& a, const XItem it) {
       }
     }
   }
-  (void)n1;
 }
 
-void launch_xform(const XformArgs& a, uint32_t n_items, hipStream_t s) {
-  if (n_items == 0) return;
-  const uint32_t grid = n_items < 4096u ? n_items : 4096u;
-  hipLaunchKernelGGL(xform_kernel, dim3(grid), dim3(256), 0, s, a, n_items);
+// Multi items with at most NS source layouts; dst_j = sum_i U[j, i] src_i is
+// accumulated with fma in source order.  Few sources (NS <= 4, e.g. the fused
+// narrow contractions): the source layouts of the current outer index are
+// held in registers and each lane recouples two plane elements (e, e + 64)
+// per step, so 2 * nsrc loads are in flight before the stores.  More sources:
+// one element per lane per step, layouts read from the (scalar-cached) table,
+// which keeps the register budget -- and the occupancy -- of the wide groups.
+template <int NS>
+__device__ void multi_item(const XformArgs& a, const XItem it) {
+  constexpr bool kTwo = NS <= 4;
+  const XBlock& b = a.blocks[it.block];
+  const int rank = b.rank;
+  const int64_t n0 = b.dims[0];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nd = b.ndst, ns = b.nsrc;
+  const XLayout* __restrict__ L = a.layouts + b.lay;
+  const double* __restrict__ U = a.coeffs + b.coeff;
+  const double* __restrict__ src = a.src;
+  double* __restrict__ dst = a.dst;
+  const double alpha = a.alpha, beta = a.beta;
+  const bool readd = a.overwrite == 0 && beta != 0.0;
+  const ItemSweep w(it, wave);
+  const int64_t dx = 64 % n0, dy = 64 / n0;
+
+  for (uint64_t o = w.o_first; o < w.o_end; o += w.o_step) {
+    int64_t idx[kMaxRank];
+    outer_index(b, o, idx);
+    uint32_t e = w.e_lo + lane;
+    int64_t x = e % n0, y = e / n0;
+    if constexpr (kTwo) {
+      int64_t sb[NS], sx[NS], sy[NS];
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        if (i < ns) {
+          const XLayout& Ls = L[nd + i];
+          sb[i] = base_offset(Ls, idx, rank);
+          sx[i] = Ls.strides[0];
+          sy[i] = rank > 1 ? Ls.strides[1] : 0;
+        } else {
+          sb[i] = sx[i] = sy[i] = 0;
+        }
+      }
+      while (e < w.e_hi) {
+        const bool two = e + 64 < w.e_hi;
+        int64_t x2 = x + dx, y2 = y + dy;
+        if (x2 >= n0) { x2 -= n0; y2 += 1; }
+        double v0[NS], v1[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+          v0[i] = v1[i] = 0.0;
+          if (i < ns) {
+            v0[i] = src[sb[i] + x * sx[i] + y * sy[i]];
+            if (two) v1[i] = src[sb[i] + x2 * sx[i] + y2 * sy[i]];
+          }
+        }
+        for (int j = 0; j < nd; ++j) {
+          const XLayout& Ld = L[j];
+          const int64_t db = base_offset(Ld, idx, rank);
+          const int64_t d0 = Ld.strides[0], d1 = rank > 1 ? Ld.strides[1] : 0;
+          double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            if (i < ns) {
+              const double u = U[j * ns + i];
+              acc0 = fma(u, v0[i], acc0);
+              acc1 = fma(u, v1[i], acc1);
+            }
+          }
+          double* p0 = dst + db + x * d0 + y * d1;
+          *p0 = readd ? alpha * acc0 + beta * *p0 : alpha * acc0;
+          if (two) {
+            double* p1 = dst + db + x2 * d0 + y2 * d1;
+            *p1 = readd ? alpha * acc1 + beta * *p1 : alpha * acc1;
+          }
+        }
+        e += 128;
+        x = x2 + dx;
+        y = y2 + dy;
+        if (x >= n0) { x -= n0; y += 1; }
+      }
+    } else {
+      for (; e < w.e_hi; e += 64) {
+        double v[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+          if (i < ns) {
+            const XLayout& Ls = L[nd + i];
+            v[i] = src[base_offset(Ls, idx, rank) + x * Ls.strides[0] + (rank > 1 ? y * Ls.strides[1] : 0)];
+          } else {
+            v[i] = 0.0;
+          }
+        }
+        for (int j = 0; j < nd; ++j) {
+          double acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < NS; ++i)
+            if (i < ns) acc = fma(U[j * ns + i], v[i], acc);
+          const XLayout& Ld = L[j];
+          double* p = dst + base_offset(Ld, idx, rank) + x * Ld.strides[0] + (rank > 1 ? y * Ld.strides[1] : 0);
+          *p = readd ? alpha * acc + beta * *p : alpha * acc;
+        }
+        x += dx;
+        y += dy;
+        if (x >= n0) { x -= n0; y += 1; }
+      }
+    }
+  }
+}
+
+// One kernel per item class, so each class gets the register budget (and
+// occupancy) of its own path.  Persistent grid-stride loop over the class's
+// items: concurrently resident workgroups process consecutive items, so the
+// waves of the whole GPU sweep one contiguous window of the payloads at a
+// time (DRAM locality), like a plain grid-stride copy.
+__global__ __launch_bounds__(256) void xform_single_kernel(XformArgs a, uint32_t begin, uint32_t n) {
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) single_item(a, a.items[begin + i]);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void xform_multi_kernel(XformArgs a, uint32_t begin, uint32_t n) {
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) multi_item<NS>(a, a.items[begin + i]);
+}
+
+__global__ __launch_bounds__(256) void coeff_gather_kernel(double* coeffs, const double* src,
+                                                          const int64_t* idx, const double* scale,
+                                                          uint32_t n) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t < n) {
+    const int64_t i = idx[t];
+    coeffs[t] = i >= 0 ? scale[t] * src[i] : 0.0;
+  }
+}
+
+void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,
+                         uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(coeff_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, coeffs, src, idx, scale, n);
+}
+
+void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t grid = n < 4096u ? n : 4096u;
+  switch (cls) {
+    case XC_SINGLE:
+      hipLaunchKernelGGL(xform_single_kernel, dim3(grid), dim3(256), 0, s, a, begin, n);
+      break;
+    case XC_MULTI4:
+      hipLaunchKernelGGL((xform_multi_kernel<4>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      break;
+    case XC_MULTI16:
+      hipLaunchKernelGGL((xform_multi_kernel<16>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      break;
+    default:
+      hipLaunchKernelGGL((xform_multi_kernel<kMaxMultiSrc>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      break;
+  }
 }
 
 }  // namespace tnt_dev

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../../include/tenet_host.h"
+#include "../common/narrow_fusion.hpp"
 #include "structure.hpp"
 
 using namespace tnth;
@@ -222,6 +223,33 @@ void tnth_contract_destroy(tnth_contract* c) { delete c; }
 const tnt_contract_desc* tnth_contract_desc(const tnth_contract* c) { return &c->c->desc; }
 const tnth_space* tnth_contract_dst_space(const tnth_contract* c) { return &c->dst; }
 double tnth_contract_flops(const tnth_contract* c) { return c->c->flops; }
+
+struct tnth_fusion {
+  tnt_fuse::NarrowFusion f;
+};
+
+int tnth_narrow_fusion(const tnt_contract_desc* d, tnth_fusion** out) {
+  if (!d || !out) {
+    g_err = "null argument";
+    return TNT_E_INVALID;
+  }
+  auto* h = new tnth_fusion;
+  if (!tnt_fuse::plan_narrow_fusion(d, h->f)) {
+    g_err = h->f.reason;
+    delete h;
+    *out = nullptr;
+    return TNT_E_UNSUPPORTED;
+  }
+  *out = h;
+  return TNT_OK;
+}
+void tnth_fusion_destroy(tnth_fusion* f) { delete f; }
+const tnt_transform_desc* tnth_fusion_desc(const tnth_fusion* f) { return &f->f.desc; }
+size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** idx, const double** scale) {
+  *idx = f->f.gather_idx.data();
+  *scale = f->f.gather_scale.data();
+  return f->f.gather_idx.size();
+}
 
 struct tnth_bytes {
   std::vector<uint8_t> v;

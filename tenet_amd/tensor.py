@@ -323,6 +323,14 @@ class _Contraction:
         self.art = art
         self.rt = rt
 
+    @property
+    def route(self):
+        """"fused_narrow" (one recoupling pass, coefficients gathered from the
+        rhs) or "transform_gemm" (operand transforms, grouped GEMM, output transform)."""
+        r = C.c_int()
+        self.rt._check(N.hip().tnt_contract_route(self.art, C.byref(r)))
+        return "fused_narrow" if r.value == 1 else "transform_gemm"
+
     def __del__(self):
         try:
             N.hip().tnt_contract_destroy(self.rt._ctx, self.art)

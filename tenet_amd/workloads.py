@@ -162,11 +162,13 @@ class MPSEnvUpdate(GraphStep):
         self.contractions = (c1, c2, c3)
         self.flops = sum(c.flops for c in self.contractions)
 
-    # T2 is kept in the order the core GEMM produces it, T2[a', b; w, s'] (no
-    # output recoupling); the last contraction reads it in that order.  L' is the
-    # same tensor, in the same layout, as with T2[a', w; s', b].
-    T2_AXES = ([1, 2], [3, 1], [0, 1, 2, 3], 2)
-    LP_AXES = ([2, 0], [0, 3], [0, 2, 1], 2)
+    # T2 is produced as T2[a', s'; b, w]: the codomain is exactly the pair the
+    # last contraction sums over, so that contraction reads T2 without a
+    # recoupling pass, and the MPO step (narrow: k = n = dim(w) dim(s) <= 10)
+    # runs as one fused pass from T1 to T2.  L' is the same tensor, in the same
+    # layout, as with T2[a', w; s', b].
+    T2_AXES = ([1, 2], [3, 1], [0, 3, 1, 2], 2)
+    LP_AXES = ([2, 0], [0, 1], [0, 2, 1], 2)
 
     def run(self, L=None):
         L = L or self.L

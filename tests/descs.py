@@ -29,13 +29,14 @@ def spaces(rule_name, cod_spec, dom_spec):
     return HomSpace(pr, P[:n], P[n:]), OHomSpace(orule, O[:n], O[n:])
 
 
-def desc_to_specs(desc):
+def desc_to_specs(desc, coeffs=None):
     d = desc.contents
     lay = []
     for i in range(d.n_layouts):
         L = d.layouts[i]
         lay.append((L.offset, tuple(L.shape[:L.rank]), tuple(L.strides[:L.rank])))
-    coeffs = [d.coeffs[i] for i in range(d.n_coeffs)]
+    if coeffs is None:
+        coeffs = [d.coeffs[i] for i in range(d.n_coeffs)]
     specs = []
     for i in range(d.n_blocks):
         b = d.blocks[i]
@@ -50,9 +51,10 @@ def desc_to_specs(desc):
     return lay, specs, inactive, d.dst_len, d.src_len
 
 
-def replay_desc(desc, src, dst=None, alpha=1.0, beta=0.0, overwrite=True):
-    """CPU replay of a compiled tnt_transform_desc (oracle replay semantics)."""
-    lay, specs, inactive, dlen, slen = desc_to_specs(desc)
+def replay_desc(desc, src, dst=None, alpha=1.0, beta=0.0, overwrite=True, coeffs=None):
+    """CPU replay of a compiled tnt_transform_desc (oracle replay semantics);
+    `coeffs` replaces the descriptor's coefficient array."""
+    lay, specs, inactive, dlen, slen = desc_to_specs(desc, coeffs)
     # one layout table: replay() indexes dst/src layouts separately, so pass
     # the same table for both and renumber the scale-only layouts
     if dst is None:

@@ -118,6 +118,31 @@ def test_network_output_permutation_and_split(rt, rule_name):
 
 
 @pytest.mark.parametrize("prec", ["f64", "split_bf16"])
+def test_fused_mpo_step_equals_general_route(rt, prec):
+    """The MPO step of the U(1) environment update runs as one fused pass; the
+    general route (TENET_AMD_FUSE_NARROW=0 at upload) gives the same bits, since
+    U(1) recoupling coefficients are 1 and both sum the k products in order."""
+    import os
+    wl = MPSEnvUpdate(rt, "U1", 256, prec)
+    assert wl.contractions[1].route == "fused_narrow"
+    assert wl.contractions[0].route == wl.contractions[2].route == "transform_gemm"
+    out = wl.run().to_numpy().copy()
+    t2 = wl._tmp["T2"].to_numpy().copy()
+    os.environ["TENET_AMD_FUSE_NARROW"] = "0"
+    rt2 = Runtime(0)
+    wl2 = None
+    try:
+        wl2 = MPSEnvUpdate(rt2, "U1", 256, prec)
+        assert wl2.contractions[1].route == "transform_gemm"
+        assert np.array_equal(wl2._tmp["T2"].to_numpy(), t2)
+        assert np.array_equal(wl2.run().to_numpy(), out)
+    finally:
+        del os.environ["TENET_AMD_FUSE_NARROW"]
+        del wl2
+        rt2.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "split_bf16"])
 def test_env_update_graph_replays_match_eager(rt, prec):
     """Every replay of the captured step equals the eager step bit for bit,
     with eager work and synchronisations between the replays."""

@@ -1,0 +1,124 @@
+"""Host planner of the fused narrow-contraction route.
+
+A narrow contraction (every core job k <= 32, n <= 16) runs on the device as
+one recoupling pass lhs -> dst whose Multi coefficients are gathered from the
+rhs payload.  Checked on the CPU: the planned transform, replayed by the
+oracle's replay with U[t] = scale[t] * rhs[idx[t]], equals the general route
+(lhs transform, per-job GEMM, output transform, replayed the same way) and the
+oracle contraction; ineligible contractions are refused with the failed
+condition named.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from descs import contract_desc_replay, ospace, replay_desc
+from oracle.tensor import OTensor
+from tenet_amd import _native as N
+from tenet_amd.tensor import HomSpace, _HostContraction
+from tenet_amd.workloads import MPSEnvUpdate, mps_env_spaces
+
+
+def fused(cdesc):
+    h = N.host()
+    f = C.c_void_p()
+    rc = h.tnth_narrow_fusion(cdesc, C.byref(f))
+    if rc != N.OK:
+        return None, h.tnth_last_error().decode()
+    return f, None
+
+
+def fused_replay(f, lhs, rhs):
+    h = N.host()
+    idx, scale = N.I64P(), C.POINTER(C.c_double)()
+    n = h.tnth_fusion_gather(f, C.byref(idx), C.byref(scale))
+    ix = np.ctypeslib.as_array(idx, (n,)).copy() if n else np.zeros(0, np.int64)
+    sc = np.ctypeslib.as_array(scale, (n,)).copy() if n else np.zeros(0)
+    U = np.where(ix >= 0, sc * rhs[np.maximum(ix, 0)], 0.0)
+    return replay_desc(h.tnth_fusion_desc(f), lhs, coeffs=list(U)), ix, sc
+
+
+def env_spaces(rule, D):
+    r, V, P, M = mps_env_spaces(rule, D)
+    return r, HomSpace(r, [V, M], [V]), HomSpace(r, [V], [P, V]), HomSpace(r, [M, P], [P, M])
+
+
+@pytest.mark.parametrize("out_perm", [[0, 1, 2, 3], [0, 3, 1, 2], [3, 0, 2, 1]])
+@pytest.mark.parametrize("nout", [2, 1])
+def test_env_mpo_step_fuses_and_matches(out_perm, nout):
+    r, L, A, W = env_spaces("U1", 48)
+    T1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False).dst_space
+    c = _HostContraction(T1, W, [1, 2], [3, 1], out_perm, nout, False, False)
+    f, why = fused(c.desc)
+    assert f is not None, why
+    try:
+        rng = np.random.default_rng(1)
+        lhs = rng.normal(size=T1.len)
+        rhs = rng.normal(size=W.len)
+        got, ix, sc = fused_replay(f, lhs, rhs)
+        want = contract_desc_replay(c.desc, lhs, rhs, c.desc.contents.core_len)
+        assert got.shape == want.shape == (c.dst_space.len,)
+        assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+        # U(1) recoupling coefficients are 1: every gathered scale is 1 or a structural 0
+        assert set(np.unique(sc)) <= {0.0, 1.0}
+        assert np.all((ix >= 0) == (sc != 0))
+        ref = OTensor(ospace(T1), lhs).contract([1, 2], OTensor(ospace(W), rhs), [3, 1], out_perm, nout)
+        assert np.max(np.abs(got - ref.data)) <= 1e-13 * np.max(np.abs(ref.data))
+    finally:
+        N.host().tnth_fusion_destroy(f)
+
+
+def test_heff_mpo_steps_fuse():
+    r, V, P, M = mps_env_spaces("U1", 40)
+    L = HomSpace(r, [V, M], [V])
+    th = HomSpace(r, [V], [P, P, V])
+    W = HomSpace(r, [M, P], [P, M])
+    x1 = _HostContraction(L, th, [2], [0], [0, 1, 2, 3, 4], 2, False, False)
+    x2 = _HostContraction(x1.dst_space, W, [1, 2], [3, 1], [0, 3, 4, 1, 2], 2, False, False)
+    x3 = _HostContraction(x2.dst_space, W, [1, 3], [3, 1], [0, 3, 1, 4, 2], 2, False, False)
+    rng = np.random.default_rng(2)
+    for c, lhs_space in ((x2, x1.dst_space), (x3, x2.dst_space)):
+        f, why = fused(c.desc)
+        assert f is not None, why
+        try:
+            lhs, rhs = rng.normal(size=lhs_space.len), rng.normal(size=W.len)
+            got, _, _ = fused_replay(f, lhs, rhs)
+            want = contract_desc_replay(c.desc, lhs, rhs, c.desc.contents.core_len)
+            assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+        finally:
+            N.host().tnth_fusion_destroy(f)
+
+
+def test_wide_contractions_keep_the_general_route():
+    r, L, A, W = env_spaces("U1", 48)
+    c1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False)
+    f, why = fused(c1.desc)
+    assert f is None and "plain GEMM" in why
+    # A^dagger . T2 contracts k up to hundreds: general route
+    T1 = c1.dst_space
+    T2 = _HostContraction(T1, W, *MPSEnvUpdate.T2_AXES, False, False).dst_space
+    c3 = _HostContraction(A, T2, *MPSEnvUpdate.LP_AXES, True, False)
+    f, why = fused(c3.desc)
+    assert f is None and "not narrow" in why
+
+
+@pytest.mark.parametrize("D", [32, 64])
+def test_su2_mpo_step_fused_or_refused_correctly(D):
+    """SU(2) recouplings of the MPO step may need Multi blocks: then the
+    planner refuses; whatever it plans must equal the general route."""
+    r, L, A, W = env_spaces("SU2", D)
+    T1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False).dst_space
+    c = _HostContraction(T1, W, [1, 2], [3, 1], [0, 3, 1, 2], 2, False, False)
+    f, why = fused(c.desc)
+    if f is None:
+        assert why
+        return
+    try:
+        rng = np.random.default_rng(3)
+        lhs, rhs = rng.normal(size=T1.len), rng.normal(size=W.len)
+        got, _, _ = fused_replay(f, lhs, rhs)
+        want = contract_desc_replay(c.desc, lhs, rhs, c.desc.contents.core_len)
+        assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    finally:
+        N.host().tnth_fusion_destroy(f)
