@@ -228,13 +228,101 @@ def run_serve(args):
     srv.close()
 
 
+def run_gemm_vs_vendor(args):
+    """The core GEMM jobs of the U(1) environment update (L.A and Adag.T2 at
+    bond dimension D) on two FP64 routes over the same operand sizes: this
+    build's grouped MFMA kernel (one tnt_gemm_grouped call per contraction,
+    tile shape picked by measurement as the autotuner does) and the vendor
+    library behind torch.mm (one call per job).  One JSON line."""
+    import torch
+
+    from tenet_amd import _native as N
+    from tenet_amd.tensor import HomSpace, Runtime, _HostContraction
+    from tenet_amd.workloads import MPSEnvUpdate, mps_env_spaces
+    r, V, P, M = mps_env_spaces("U1", args.D)
+    L, A, W = HomSpace(r, [V, M], [V]), HomSpace(r, [V], [P, V]), HomSpace(r, [M, P], [P, M])
+    c1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False)
+    c2 = _HostContraction(c1.dst_space, W, *MPSEnvUpdate.T2_AXES, False, False)
+    c3 = _HostContraction(A, c2.dst_space, *MPSEnvUpdate.LP_AXES, True, False)
+    groups = []
+    for c in (c1, c3):
+        d = c.desc.contents
+        groups.append([(d.jobs[i].rows, d.jobs[i].contracted, d.jobs[i].cols) for i in range(d.n_jobs)])
+    shapes = [s_ for g_ in groups for s_ in g_]
+    flops = float(sum(2 * m * k * n for m, k, n in shapes))
+    rt = Runtime(0)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(1)
+    # column-major (m x k) A, (k x n) B, (m x n) C are row-major (k, m), (n, k),
+    # (n, m) tensors: the grouped kernel runs the jobs as the environment update
+    # does, and torch.mm(B^T, A^T) = C^T computes the same products
+    ops = [[(torch.randn(k, m, dtype=torch.float64, device=dev, generator=g),
+             torch.randn(n, k, dtype=torch.float64, device=dev, generator=g),
+             torch.empty(n, m, dtype=torch.float64, device=dev)) for m, k, n in grp] for grp in groups]
+    jobs = [(N.GemmPtrJob * len(grp))(*[
+        N.GemmPtrJob(cm.data_ptr(), a.data_ptr(), b.data_ptr(), m, k, n, m, m, k, 0, 0)
+        for (m, k, n), (a, b, cm) in zip(grp, o)]) for grp, o in zip(groups, ops)]
+    h = N.hip()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_ours, cfgs = 0.0, []
+    for jb, grp in zip(jobs, groups):
+        best = None
+        for cfg in (-1, 0, 1, 2, 3):
+            h.tnt_set_tile_config(cfg)
+            t = timed(lambda: rt._check(h.tnt_gemm_grouped(rt._ctx, jb, len(grp), 1.0, 0.0, 0)))
+            if best is None or t < best[0]:
+                best = (t, cfg)
+        t_ours += best[0]
+        cfgs.append(GEMM_KERNELS[best[1]].format("double") if best[1] >= 0 else "size rule")
+    h.tnt_set_tile_config(-1)
+
+    def vendor():
+        for o in ops:
+            for a, b, cm in o:
+                torch.mm(b, a, out=cm)
+
+    t_vendor = timed(vendor)
+    ref = [cm.clone() for o in ops for _, _, cm in o]
+    torch.cuda.synchronize()  # the copies (torch stream) before the kernel (backend stream)
+    for jb, grp in zip(jobs, groups):
+        rt._check(h.tnt_gemm_grouped(rt._ctx, jb, len(grp), 1.0, 0.0, 0))
+    torch.cuda.synchronize()
+    outs = [cm for o in ops for _, _, cm in o]
+    err = max(float((cm - r_).abs().max() / r_.abs().max()) for cm, r_ in zip(outs, ref))
+    log(f"[bench] {len(shapes)} FP64 sector GEMMs, {flops / 1e9:.2f} GFLOP: grouped MFMA kernel "
+        f"{t_ours * 1e3:.3f} ms ({flops / t_ours / 1e12:.1f} TFLOP/s; tiles {cfgs}), torch.mm per job "
+        f"{t_vendor * 1e3:.3f} ms ({flops / t_vendor / 1e12:.1f} TFLOP/s); max rel diff {err:.1e}")
+    rec = {"metric": "FP64 sector GEMM TFLOP/s (grouped MFMA kernel vs vendor torch.mm)",
+           "value": flops / t_ours / 1e12, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": t_ours * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (torch normal(0,1) operands)",
+           "config": {"workload": "u1_env_core_gemms", "bond_dim": args.D, "jobs": len(shapes),
+                      "tiles": cfgs},
+           "vendor": {"route": "torch.mm per job (hipBLASLt / rocBLAS)", "tflops": flops / t_vendor / 1e12,
+                      "ms_per_step": t_vendor * 1e3},
+           "max_rel_diff": err}
+    print(json.dumps(rec), flush=True)
+    rt.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="u1_env",
-                    choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration"])
+                    choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
+                             "gemm_vs_vendor"])
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "split_bf16"])
     ap.add_argument("--D", type=int, default=4096)
@@ -258,6 +346,9 @@ def main():
 
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "gemm_vs_vendor":
+        run_gemm_vs_vendor(args)
         return
 
     from tenet_amd import _native as N

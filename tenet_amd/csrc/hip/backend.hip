@@ -434,6 +434,20 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     if (!normalize(ls, dims, err)) FAIL(ctx, TNT_E_SHAPE, err);
     if (dims.empty()) return TNT_OK;  // zero-size block
     if (dims.size() > (size_t)kMaxRank) FAIL(ctx, TNT_E_UNSUPPORTED, "normalised rank exceeds 8");
+    // A Single block contiguous along another axis in its source than in its
+    // destination (dims[0], smallest destination stride) is recoupled through
+    // LDS tiles: that axis becomes dims[1].
+    if (kind == XK_SINGLE && dims.size() >= 2 && ls[0].strides[0] == 1 && ls[1].strides[0] != 1 &&
+        dims[0] >= 16) {
+      for (size_t t = 1; t < dims.size(); ++t) {
+        if (ls[1].strides[t] == 1 && dims[t] >= 16) {
+          std::swap(dims[1], dims[t]);
+          for (auto& l : ls) std::swap(l.strides[1], l.strides[t]);
+          kind = XK_TRANSPOSE;
+          break;
+        }
+      }
+    }
     XBlock b{};
     b.kind = kind;
     b.rank = (int32_t)dims.size();
@@ -459,6 +473,14 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     // order by a persistent grid; Multi: ~8K element-updates per item, weighted by
     // its layouts
     std::vector<XItem>& items = cls_items[xclass_of(kind, (int)sl.size())];
+    if (kind == XK_TRANSPOSE) {  // one item per 64 x 64 tile of every plane
+      const int64_t ntx = (dims[0] + kTransposeTile - 1) / kTransposeTile;
+      const int64_t nty = (dims[1] + kTransposeTile - 1) / kTransposeTile;
+      for (int64_t o = 0; o < outer; ++o)
+        for (int64_t t = 0; t < ntx * nty; ++t)
+          items.push_back({bi, 1u, (uint64_t)o, (uint32_t)t, (uint32_t)kTransposeTile});
+      return TNT_OK;
+    }
     int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
     int64_t target = kind == XK_SINGLE ? kSingleItem : std::max<int64_t>(256, 8192 / weight);
     if (plane >= target) {

@@ -12,7 +12,11 @@ constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in re
 constexpr int kSingleUnroll = 4;  // elements per thread of a Single work item
 constexpr int kSingleItem = 256 * kSingleUnroll;  // plane elements per Single work item
 
-enum XKind : int32_t { XK_SINGLE = 0, XK_MULTI = 1, XK_SCALE = 2 };
+// XK_TRANSPOSE: a Single block whose source is contiguous along another axis
+// than the destination (dims[0] destination-contiguous, dims[1]
+// source-contiguous), recoupled through 64 x 64 LDS tiles.
+enum XKind : int32_t { XK_SINGLE = 0, XK_MULTI = 1, XK_SCALE = 2, XK_TRANSPOSE = 3 };
+constexpr int kTransposeTile = 64;
 
 // One normalised transform block: dims in destination order (extent-1 axes
 // dropped, axes sorted by destination stride, contiguous runs fused).
@@ -53,10 +57,20 @@ struct XformArgs {
   int overwrite;
 };
 
-// Work-item classes of a plan, one kernel each: Single and Scale items, and
-// Multi items by source count (<= 4, <= 16, <= kMaxMultiSrc).
-enum XClass : int32_t { XC_SINGLE = 0, XC_MULTI4 = 1, XC_MULTI16 = 2, XC_MULTI32 = 3, XC_COUNT = 4 };
+// Work-item classes of a plan, one kernel each: Single and Scale items, Multi
+// items by source count (<= 4, <= 16, <= kMaxMultiSrc), transposed Single
+// tiles.  A transpose item is one 64 x 64 tile: e_begin = tile index in the
+// plane (x tiles fastest), o_begin = outer index.
+enum XClass : int32_t {
+  XC_SINGLE = 0,
+  XC_MULTI4 = 1,
+  XC_MULTI16 = 2,
+  XC_MULTI32 = 3,
+  XC_TRANSPOSE = 4,
+  XC_COUNT = 5
+};
 inline int xclass_of(int kind, int nsrc) {
+  if (kind == XK_TRANSPOSE) return XC_TRANSPOSE;
   if (kind != XK_MULTI) return XC_SINGLE;
   return nsrc <= 4 ? XC_MULTI4 : nsrc <= 16 ? XC_MULTI16 : XC_MULTI32;
 }

@@ -291,6 +291,55 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
   }
 }
 
+// Transposed Single tile: read a 64 x 64 tile along the source-contiguous
+// axis (dims[1]) into LDS, write it along the destination-contiguous axis
+// (dims[0]); both global passes are coalesced.  Row pitch 65 keeps the
+// column reads of the write pass free of bank conflicts.
+__device__ void transpose_item(const XformArgs& a, const XItem it, double (&tile)[kTransposeTile][kTransposeTile + 1]) {
+  constexpr int T = kTransposeTile;
+  const XBlock& b = a.blocks[it.block];
+  const int rank = b.rank;
+  const int64_t n0 = b.dims[0], n1 = b.dims[1];
+  const int64_t ntx = (n0 + T - 1) / T;
+  const int64_t x0 = (int64_t)(it.e_begin % ntx) * T, y0 = (int64_t)(it.e_begin / ntx) * T;
+  int64_t idx[kMaxRank];
+  outer_index(b, it.o_begin, idx);
+  const XLayout& Ld = a.layouts[b.lay];
+  const XLayout& Ls = a.layouts[b.lay + 1];
+  const int64_t od = base_offset(Ld, idx, rank), os = base_offset(Ls, idx, rank);
+  const int64_t d0 = Ld.strides[0], d1 = Ld.strides[1];
+  const int64_t s0 = Ls.strides[0], s1 = Ls.strides[1];
+  const double c = a.alpha * a.coeffs[b.coeff];
+  const bool readd = a.overwrite == 0 && a.beta != 0.0;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  {
+    // all T / 4 loads of a lane in flight before the LDS writes
+    const int64_t y = y0 + l;
+    double v[T / 4];
+#pragma unroll
+    for (int r = 0; r < T / 4; ++r) {
+      const int64_t x = x0 + w + 4 * r;
+      v[r] = (x < n0 && y < n1) ? a.src[os + x * s0 + y * s1] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < T / 4; ++r) tile[w + 4 * r][l] = v[r];
+  }
+  __syncthreads();
+  {
+    const int64_t x = x0 + l;
+#pragma unroll
+    for (int r = 0; r < T / 4; ++r) {
+      const int64_t y = y0 + w + 4 * r;
+      if (x < n0 && y < n1) {
+        double* p = a.dst + od + x * d0 + y * d1;
+        const double v = c * tile[l][w + 4 * r];
+        *p = readd ? v + a.beta * *p : v;
+      }
+    }
+  }
+  __syncthreads();  // the tile is refilled by the next item
+}
+
 // One kernel per item class, so each class gets the register budget (and
 // occupancy) of its own path.  Persistent grid-stride loop over the class's
 // items: concurrently resident workgroups process consecutive items, so the
@@ -298,6 +347,11 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
 // time (DRAM locality), like a plain grid-stride copy.
 __global__ __launch_bounds__(256) void xform_single_kernel(XformArgs a, uint32_t begin, uint32_t n) {
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) single_item(a, a.items[begin + i]);
+}
+
+__global__ __launch_bounds__(256) void xform_transpose_kernel(XformArgs a, uint32_t begin, uint32_t n) {
+  __shared__ double tile[kTransposeTile][kTransposeTile + 1];
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) transpose_item(a, a.items[begin + i], tile);
 }
 
 template <int NS>
@@ -333,6 +387,9 @@ void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipSt
       break;
     case XC_MULTI16:
       hipLaunchKernelGGL((xform_multi_kernel<16>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      break;
+    case XC_TRANSPOSE:
+      hipLaunchKernelGGL(xform_transpose_kernel, dim3(grid), dim3(256), 0, s, a, begin, n);
       break;
     default:
       hipLaunchKernelGGL((xform_multi_kernel<kMaxMultiSrc>), dim3(grid), dim3(256), 0, s, a, begin, n);

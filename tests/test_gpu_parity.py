@@ -225,6 +225,25 @@ def test_permute_matches_oracle(rt, rule_name, pairs, p1, p2):
         assert np.array_equal(got, want), "abelian permutation must be bit-exact"
 
 
+BIGV = [(-1, 70), (0, 130), (1, 40)]  # ragged blocks of 16..130 rows: full and partial 64 x 64 tiles
+
+
+@pytest.mark.parametrize("cod,dom,p1,p2", [(1, 1, [1], [0]), (2, 1, [2, 0], [1]), (2, 1, [1], [2, 0]),
+                                           (1, 2, [2, 1], [0])])
+def test_transposed_blocks_through_lds_tiles(rt, cod, dom, p1, p2):
+    """Blocks whose source is contiguous along another axis than the
+    destination go through 64 x 64 LDS tiles: bit-exact, also accumulating."""
+    ps, os_ = spaces("U1", [(BIGV, False)] * cod, [(BIGV, False)] * dom)
+    t = TensorMap.rand(rt, ps, seed=21)
+    src = t.to_numpy()
+    out = t.permute(p1, p2)
+    want = OTensor(os_, src).permute(p1, p2).data
+    assert np.array_equal(out.to_numpy(), want), "abelian permutation must be bit-exact"
+    before = out.to_numpy()
+    t.permute(p1, p2, out=out, alpha=2.0, beta=-0.5)
+    assert np.array_equal(out.to_numpy(), 2.0 * want + -0.5 * before)
+
+
 def test_permute_axpby_into_existing(rt):
     ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, False)], [(SU2V, False), (SU2V, False)])
     t = TensorMap.rand(rt, ps, seed=5)
