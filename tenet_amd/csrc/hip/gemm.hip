@@ -67,9 +67,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
   const int m = job.m, n = job.n, k = job.k;
   const int m0 = tile.tm * BM, n0 = tile.tn * BN;
   const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
-  const double* A = reinterpret_cast<const double*>(g.abase) + job.a_off;
-  const double* B = reinterpret_cast<const double*>(g.bbase) + job.b_off;
-  double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
+  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
+  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
   const int64_t lda = job.lda, ldb = job.ldb, ldd = job.ldd;
 
   const int tid = threadIdx.x;
@@ -93,10 +93,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
   const int b_o = b_adj ? tid / BN : tid / BK;
   const int b_step = b_adj ? 256 / BN : 256 / BK;
   // element (row i, k kk) of op(A): identity A[i + kk*lda], adjoint A[kk + i*lda]
-  const double* pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
+  gptr<const double> pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
   const int64_t a_rstride = (int64_t)a_step * lda;  // between consecutive loads r
   const int64_t a_kadv = a_adj ? BK : (int64_t)BK * lda;
-  const double* pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
+  gptr<const double> pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
   const int64_t b_rstride = (int64_t)b_step * ldb;
   const int64_t b_kadv = b_adj ? (int64_t)BK * ldb : BK;
   // LDS destinations: As[buf][kk][i], Bs[buf][kk][j]
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
         if (row < m) {
-          double* p = D + row + (int64_t)col * ldd;
+          const gptr<double> p = D + row + (int64_t)col * ldd;
           const double v = alpha * (double)acc[i][j][r];
           *p = (beta != 0.0) ? v + beta * *p : v;
         }
@@ -208,9 +208,9 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
   const GJob job = g.jobs[tile.job];
   const int m = job.m, n = job.n, k = job.k;
   const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
-  const double* A = reinterpret_cast<const double*>(g.abase) + job.a_off;
-  const double* B = reinterpret_cast<const double*>(g.bbase) + job.b_off;
-  double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
+  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
+  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
   const int i = tile.tm * 256 + threadIdx.x;
   double acc[NMAX];
 #pragma unroll
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
       if (j < n) {
-        double* p = D + i + (int64_t)j * job.ldd;
+        const gptr<double> p = D + i + (int64_t)j * job.ldd;
         const double v = alpha * acc[j];
         *p = (beta != 0.0) ? v + beta * *p : v;
       }

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   const PrepSide& sd = x.side[it.side];
   const int lines = it.side == 0 ? g.m : g.n;
   const int K = g.k;
-  const double* M = reinterpret_cast<const double*>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
+  const gptr<const double> M = gmem<const double>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs 
   }
 
   // epilogue: C = alpha acc64 (+ beta C)
-  double* D = reinterpret_cast<double*>(g.dbase) + job.d_off;
+  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
 #pragma unroll
   for (int j = 0; j < WN; ++j) {
     const int col = n0 + j * 16 + (lane & 15);
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs 
         const int row = m0 + i * 16 + (lane >> 4) * 4 + r;
         if (row < job.m) {
           const double v = g.alpha * acc64[i][j][r];
-          double* p = D + row + (int64_t)col * job.ldd;
+          const gptr<double> p = D + row + (int64_t)col * job.ldd;
           *p = (g.beta != 0.0) ? v + g.beta * *p : v;
         }
       }

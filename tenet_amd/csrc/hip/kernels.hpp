@@ -8,6 +8,18 @@
 namespace tnt_dev {
 
 constexpr int kMaxRank = 8;
+
+// Pointers in the global address space.  A pointer rebuilt from an integer
+// address is generic (FLAT) to the compiler; FLAT loads count against lgkmcnt
+// as well as vmcnt, so every wait for an LDS read would also wait for all
+// global loads in flight (a GEMM's prefetch of its next K slab).  Kernels that
+// take byte addresses convert them with gmem().
+template <class T>
+using gptr = T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ gptr<T> gmem(uintptr_t addr) {
+  return (gptr<T>)addr;
+}
 constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in registers
 constexpr int kSingleUnroll = 4;  // elements per thread of a Single work item
 constexpr int kSingleItem = 256 * kSingleUnroll;  // plane elements per Single work item
