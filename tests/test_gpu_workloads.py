@@ -155,3 +155,40 @@ def test_env_update_graph_replays_match_eager(rt, prec):
         wl.run()
         rt.synchronize()
         assert np.array_equal(wl.out.to_numpy(), eager)
+
+
+def test_runtime_pool_leases_from_threads(rt):
+    """Operations leased from one pool by four host threads run on separate
+    contexts (streams) and give the same bits as one runtime; a failed lease
+    quarantines its context."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from descs import spaces
+    from tenet_amd.pool import RuntimePool
+    from tenet_amd.tensor import TensorMap
+    leg = [(-1, 12), (0, 20), (1, 10)]
+    ps, _ = spaces("U1", [(leg, False), (leg, False)], [(leg, False), (leg, False)])
+
+    def op(r, seed):
+        A = TensorMap.rand(r, ps, seed=seed, normal=True)
+        B = TensorMap.rand(r, ps, seed=seed + 1, normal=True)
+        return A.contract([2, 3], B, [0, 1]).to_numpy()
+
+    pool = RuntimePool(max_idle=4)
+    try:
+        def leased(seed):
+            with pool.lease() as r:
+                return id(r), op(r, seed)
+
+        with ThreadPoolExecutor(4) as ex:
+            got = list(ex.map(leased, range(0, 32, 2)))
+        assert 1 < pool.minted <= 4 and pool.idle == pool.minted
+        for seed, (_, out) in zip(range(0, 32, 2), got):
+            assert np.array_equal(out, op(rt, seed))
+        with pytest.raises(ValueError):
+            with pool.lease() as r:
+                bad = r
+                TensorMap.from_numpy(r, ps, np.zeros(3))  # wrong payload length: rejected on the host
+        assert pool.quarantined == 1 and bad._ctx is None
+    finally:
+        pool.close()
