@@ -142,6 +142,27 @@ def test_fused_mpo_step_equals_general_route(rt, prec):
         rt2.close()
 
 
+def test_fused_route_accumulates_and_reads_adjoint_mpo(rt):
+    """Fused narrow route on the device: out = alpha X.op(W) + beta out, with W
+    as stored and as a lazy adjoint read with op = adjoint, against the oracle."""
+    from tenet_amd.tensor import HomSpace, TensorMap
+    from tenet_amd.workloads import mps_env_spaces
+    r, V, P, M = mps_env_spaces("U1", 200)
+    X = TensorMap.rand(rt, HomSpace(r, [V], [P, M]), seed=31, normal=True)
+    W = TensorMap.rand(rt, HomSpace(r, [M, P], [P, M]), seed=32, normal=True)
+    oX, oW = _o(X), _o(W)
+    for adj, axes_b in ((False, [1, 0]), (True, [0, 1])):
+        op = W.adjoint() if adj else W
+        c = rt.contraction(X.space, W.space, [1, 2], axes_b, [0, 2, 1], 2, False, adj)
+        assert c.route == "fused_narrow"
+        out = X.contract([1, 2], op, axes_b, [0, 2, 1], 2)
+        prod = oX.contract([1, 2], oW.adjoint() if adj else oW, axes_b, [0, 2, 1], 2).data
+        assert rel(out.to_numpy(), prod) < 1e-14
+        before = out.to_numpy()
+        X.contract([1, 2], op, axes_b, [0, 2, 1], 2, alpha=2.0, beta=-0.5, out=out)
+        assert rel(out.to_numpy(), 2.0 * prod - 0.5 * before) < 1e-14
+
+
 @pytest.mark.parametrize("prec", ["f64", "split_bf16"])
 def test_env_update_graph_replays_match_eager(rt, prec):
     """Every replay of the captured step equals the eager step bit for bit,

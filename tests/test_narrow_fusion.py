@@ -122,3 +122,25 @@ def test_su2_mpo_step_fused_or_refused_correctly(D):
         assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
     finally:
         N.host().tnth_fusion_destroy(f)
+
+
+@pytest.mark.parametrize("out_perm,nout", [([0, 2, 1], 2), ([2, 0, 1], 1)])
+def test_adjoint_mpo_operand_fuses(out_perm, nout):
+    """X[a; s, w] . W^dagger contracted in W^dagger's own leg order: the core
+    reads the stored W with op = adjoint, and the gather indexes it transposed."""
+    r, V, P, M = mps_env_spaces("U1", 48)
+    X, W = HomSpace(r, [V], [P, M]), HomSpace(r, [M, P], [P, M])
+    c = _HostContraction(X, W, [1, 2], [0, 1], out_perm, nout, False, True)
+    assert c.desc.contents.rhs_op == N.OP_ADJOINT
+    f, why = fused(c.desc)
+    assert f is not None, why
+    try:
+        rng = np.random.default_rng(4)
+        lhs, rhs = rng.normal(size=X.len), rng.normal(size=W.len)
+        got, _, _ = fused_replay(f, lhs, rhs)
+        want = contract_desc_replay(c.desc, lhs, rhs, c.desc.contents.core_len)
+        ref = OTensor(ospace(X), lhs).contract([1, 2], OTensor(ospace(W), rhs).adjoint(), [0, 1], out_perm, nout)
+        assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
+        assert np.max(np.abs(got - ref.data)) <= 1e-14 * np.max(np.abs(ref.data))
+    finally:
+        N.host().tnth_fusion_destroy(f)
