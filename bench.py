@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 GEMM_KERNELS = ["gemm_kernel<{}, 128, 128>", "gemm_kernel<{}, 64, 64>", "gemm_kernel<{}, 128, 64>",
-                "gemm_kernel<{}, 64, 128>", "gemm_skinny_kernel<{}>"]
+                "gemm_kernel<{}, 64, 128>", "gemm_skinny_kernel<{}>", "gemm_kernel<{}, 128, 128, 2, 4>"]
 SPLIT_PRODUCTS = 15  # bf16 MFMA products per FP64 product at five digit planes
 
 
@@ -277,7 +277,7 @@ def run_gemm_vs_vendor(args):
     t_ours, cfgs = 0.0, []
     for jb, grp in zip(jobs, groups):
         best = None
-        for cfg in (-1, 0, 1, 2, 3):
+        for cfg in (-1, 0, 1, 2, 3, 5):
             h.tnt_set_tile_config(cfg)
             t = timed(lambda: rt._check(h.tnt_gemm_grouped(rt._ctx, jb, len(grp), 1.0, 0.0, 0)))
             if best is None or t < best[0]:
@@ -472,7 +472,7 @@ def main():
         dom = {"kernel": "split-BF16 table run (digit pre-pass + bf16 MFMA GEMM; FP64 kernels for narrow jobs)",
                "launches": gemm_launches, "ms": gemm_ms, "flops": gemm_work}
     else:
-        per = [(stats(8 + 8 * prec_id + c), c) for c in range(5)]
+        per = [(stats(8 + 8 * prec_id + c), c) for c in range(len(GEMM_KERNELS))]
         (dl, dms, dw), dc = max(per, key=lambda x: x[0][1])
         dom = {"kernel": GEMM_KERNELS[dc].format("double" if prec_id == 0 else "float"),
                "launches": dl, "ms": dms, "flops": dw}

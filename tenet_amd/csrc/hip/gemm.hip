@@ -9,8 +9,8 @@
 //   FP32: operands rounded to FP32 while staging into LDS,
 //         v_mfma_f32_16x16x4f32, FP32 accumulate, FP64 epilogue.
 //
-// Workgroup = 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
-// (BM/2)x(BN/2) sub-tile made of 16x16 MFMA tiles.  K advances in steps of
+// Workgroup = 4 wave64s in a 2x2 grid (8 in a 2x4 grid for TILE_128x128_W8);
+// each wave owns a (BM/GM)x(BN/GN) sub-tile made of 16x16 MFMA tiles.  K advances in steps of
 // BK=16 through LDS with a register prefetch of the next K slab, so global
 // loads of slab k+1 overlap the MFMAs of slab k.
 //
@@ -51,14 +51,19 @@ struct MfmaOp<float> {
 
 constexpr int BK = 16;
 
-// Register budget: at most 256 VGPRs+AGPRs per lane, so two waves share each
-// SIMD (one wave per SIMD leaves LDS and global latency exposed).
-template <typename T, int BM, int BN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(GemmArgs g) {
-  constexpr int WM = BM / 2, WN = BN / 2;
+// Workgroup = GM x GN waves (wave w at row w % GM, column w / GM of the tile).
+// Register budget: 4-wave tiles at most 256 VGPRs+AGPRs per lane, so two
+// waves share each SIMD (one wave per SIMD leaves LDS and global latency
+// exposed); 8-wave tiles at most 128, four waves per SIMD.
+template <typename T, int BM, int BN, int GM = 2, int GN = 2>
+__global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 ? 4 : 2))) void gemm_kernel(
+    GemmArgs g) {
+  constexpr int NTH = 64 * GM * GN;
+  constexpr int WM = BM / GM, WN = BN / GN;
   constexpr int MT = WM / 16, NT = WN / 16;
   constexpr int PA = 2, PB = 2;  // LDS row padding (elements)
-  constexpr int A_LOADS = BM * BK / 256, B_LOADS = BN * BK / 256;
+  constexpr int A_LOADS = BM * BK / NTH, B_LOADS = BN * BK / NTH;
+  static_assert(A_LOADS * NTH == BM * BK && B_LOADS * NTH == BN * BK, "tile loads must split evenly");
   __shared__ T As[2][BK][BM + PA];
   __shared__ T Bs[2][BK][BN + PB];
 
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+  const int wm = (wave % GM) * WM, wn = (wave / GM) * WN;
 
   using acc_t = typename MfmaOp<T>::acc_t;
   acc_t acc[MT][NT];
@@ -88,10 +93,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
   // B identity: contiguous along k (C = BK, O = BN); adjoint: along j (C = BN, O = BK)
   const int a_c = a_adj ? tid % BK : tid % BM;
   const int a_o = a_adj ? tid / BK : tid / BM;
-  const int a_step = a_adj ? 256 / BK : 256 / BM;
+  const int a_step = a_adj ? NTH / BK : NTH / BM;
   const int b_c = b_adj ? tid % BN : tid % BK;
   const int b_o = b_adj ? tid / BN : tid / BK;
-  const int b_step = b_adj ? 256 / BN : 256 / BK;
+  const int b_step = b_adj ? NTH / BN : NTH / BK;
   // element (row i, k kk) of op(A): identity A[i + kk*lda], adjoint A[kk + i*lda]
   gptr<const double> pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
   const int64_t a_rstride = (int64_t)a_step * lda;  // between consecutive loads r
@@ -262,6 +267,9 @@ static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t
       break;
     case TILE_64x128:
       hipLaunchKernelGGL((gemm_kernel<T, 64, 128>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_128x128_W8:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4>), dim3(n_tiles), dim3(512), 0, s, a);
       break;
     default:
       hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), dim3(n_tiles), dim3(256), 0, s, a);

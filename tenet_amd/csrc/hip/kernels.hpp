@@ -119,19 +119,22 @@ struct GemmArgs {
 // Tile shapes (rows x cols per workgroup) available to the autotuner.
 // TILE_SKINNY: narrow outputs (n <= 16), 256 rows x all columns per workgroup
 // on the vector units.
+// TILE_128x128_W8: 128 x 128 on 8 waves (2 x 4, each 64 x 32), two waves of
+// the workgroup per SIMD.
 enum TileCfg : int32_t {
   TILE_128x128 = 0,
   TILE_64x64 = 1,
   TILE_128x64 = 2,
   TILE_64x128 = 3,
   TILE_SKINNY = 4,
-  TILE_CFG_COUNT = 5
+  TILE_128x128_W8 = 5,
+  TILE_CFG_COUNT = 6
 };
 inline int tile_rows(int cfg) {
-  return cfg == TILE_SKINNY ? 256 : (cfg == TILE_128x128 || cfg == TILE_128x64) ? 128 : 64;
+  return cfg == TILE_SKINNY ? 256 : (cfg == TILE_128x128 || cfg == TILE_128x64 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
 inline int tile_cols(int cfg) {
-  return cfg == TILE_SKINNY ? 16 : (cfg == TILE_128x128 || cfg == TILE_64x128) ? 128 : 64;
+  return cfg == TILE_SKINNY ? 16 : (cfg == TILE_128x128 || cfg == TILE_64x128 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
 
 // Launch one grouped GEMM over the tile table.  All tiles of one launch must

@@ -99,6 +99,35 @@ def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     assert rel(got, w) < tol
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("lop,rop", [(0, 0), (1, 1)])
+def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop):
+    """Every tile configuration (forced) on ragged FP64 jobs, both operand ops."""
+    rng = np.random.default_rng(cfg)
+    shapes = [(130, 70, 129), (300, 17, 65), (64, 64, 64), (257, 33, 200)]
+    jobs, lhs, rhs, want = [], [], [], []
+    lo = ro = do = 0
+    for m, k, n in shapes:
+        a, b = rng.normal(size=(m, k)), rng.normal(size=(k, n))
+        lhs.append((a.T if lop else a).reshape(-1, order="F"))
+        rhs.append((b.T if rop else b).reshape(-1, order="F"))
+        jobs.append(N.GemmJob(do, lo, ro, m, k, n))
+        want.append((a @ b).reshape(-1, order="F"))
+        lo, ro, do = lo + m * k, ro + k * n, do + m * n
+    L, R, D = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs)), _buf(rt, np.zeros(do))
+    h = N.hip()
+    try:
+        h.tnt_set_tile_config(cfg)
+        arr = (N.GemmJob * len(jobs))(*jobs)
+        rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, 1.0, 0.0, 0))
+        got = _get(rt, D, do)
+    finally:
+        h.tnt_set_tile_config(-1)
+        _free(rt, L, R, D)
+    w = np.concatenate(want)
+    assert rel(got, w) < 1e-14
+
+
 def test_gemm_batch_rejects_out_of_bounds_before_mutating(rt):
     D = np.arange(16.0)
     bd, bl, br = _buf(rt, D), _buf(rt, np.ones(16)), _buf(rt, np.ones(16))
