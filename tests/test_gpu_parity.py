@@ -302,6 +302,40 @@ def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
     assert rel(got, want) < tol, (rule_name, ci, prec, rel(got, want))
 
 
+@pytest.mark.parametrize("axb", [[0, 1], [1, 0]])
+def test_quickstart_scale_compose_and_swap(rt, axb):
+    """U(1) legs -2..2 x 8 (dim 40), uniform fills with seeds 1 and 2:
+    A[a b; c d] . B[c d; g h] (no recoupling) and B[d c; g h] (rhs recoupled)."""
+    V = [(q, 8) for q in range(-2, 3)]
+    ps, os_ = spaces("U1", [(V, False), (V, False)], [(V, False), (V, False)])
+    A, B = TensorMap.rand(rt, ps, seed=1), TensorMap.rand(rt, ps, seed=2)
+    got = A.contract([2, 3], B, axb).to_numpy()
+    want = OTensor(os_, A.to_numpy()).contract([2, 3], OTensor(os_, B.to_numpy()), axb, [0, 1, 2, 3], 2).data
+    assert rel(got, want) < 1e-13
+
+
+def test_contract_rejects_short_operands_before_mutating(rt):
+    """Validate before mutate: an operand shorter than the artifact expects is
+    an error and the destination keeps its contents."""
+    V = [(q, 4) for q in range(-1, 2)]
+    ps, _ = spaces("U1", [(V, False), (V, False)], [(V, False), (V, False)])
+    A, B = TensorMap.rand(rt, ps, seed=1), TensorMap.rand(rt, ps, seed=2)
+    c = rt.contraction(A.space, B.space, [2, 3], [1, 0], [0, 1, 2, 3], 2)
+    D = TensorMap.rand(rt, c.dst_space, seed=3)
+    before = D.to_numpy()
+    short = _buf(rt, np.zeros(5))
+    try:
+        rc = N.hip().tnt_contract(rt._ctx, c.art, D._buf, A._buf, short, 1.0, 0.0, 0)
+        assert rc == N.E_SHAPE
+        rc = N.hip().tnt_contract(rt._ctx, c.art, short, A._buf, B._buf, 1.0, 0.0, 0)
+        assert rc == N.E_SHAPE
+        rc = N.hip().tnt_contract(rt._ctx, c.art, D._buf, A._buf, B._buf, 1.0, 0.0, 7)
+        assert rc == N.E_INVALID
+    finally:
+        _free(rt, short)
+    assert np.array_equal(D.to_numpy(), before)
+
+
 def _oracle_fill_fn(rule, c0):
     def fn(unc, coupled, idx):
         lab = [rule.label(s) for s in unc] + [rule.label(coupled)]

@@ -17,7 +17,7 @@ from descs import contract_desc_replay, replay_desc, spaces
 from oracle.symmetry import SU2 as OSU2
 from oracle.tensor import OTensor
 from tenet_amd import _native as N
-from tenet_amd.tensor import Runtime  # noqa: F401  (import check only)
+from tenet_amd.tensor import GradedSpace, HomSpace, Rule, Runtime, _HostContraction  # noqa: F401
 
 U1V = [(-1, 2), (0, 3), (1, 2)]
 U1W = [(0, 1), (1, 2), (2, 2), (-2, 1)]
@@ -163,3 +163,22 @@ def test_host_compiler_rejects_bad_input():
     rc = h.tnth_contract_compile(ps._h, ps._h, N.sz_array([0]), N.sz_array([0]), 1,
                                  N.sz_array([0, 1, 2, 3]), 2, 0, 0, C.byref(out))
     assert rc == -1 and b"match" in h.tnth_last_error()
+
+
+def test_quickstart_scale_compose_closed_form():
+    """Two U(1) rank-4 maps, charges -2..2 x degeneracy 8 per leg: coupled
+    charge c has n_c = 64 p(c) rows (p(c) = pairs q1 + q2 = c: 1,2,3,4,5,4,3,2,1),
+    so the payload is sum n_c^2 = 4096 * 85 elements and the compose
+    A[a b; c d] . B[c d; g h] costs sum 2 n_c^3 = 2 * 64^3 * 325 flops
+    (170.4 MFLOP); the swapped order B[d c; g h] adds an rhs recoupling and
+    the same GEMM work."""
+    r = Rule("U1")
+    V = GradedSpace(r, [(q, 8) for q in range(-2, 3)])
+    A = HomSpace(r, [V, V], [V, V])
+    assert A.len == 4096 * 85
+    c = _HostContraction(A, A, [2, 3], [0, 1], [0, 1, 2, 3], 2, False, False)
+    assert c.flops == 2 * 64 ** 3 * 325
+    d = c.desc.contents
+    assert not d.lhs and not d.rhs and not d.out  # Core route: no recoupling
+    sw = _HostContraction(A, A, [2, 3], [1, 0], [0, 1, 2, 3], 2, False, False)
+    assert sw.flops == c.flops and sw.desc.contents.rhs
