@@ -340,8 +340,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (sharding is by coupled "
-              "sector; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the contraction would shard by "
+              "coupled sector, with one all-to-all per leg-moving recoupling). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "serve":

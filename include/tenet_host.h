@@ -5,9 +5,8 @@
  * In TeNeT these steps live in the host compile layer and stay on the host:
  * the backend only consumes their outputs (compiled offsets, strides,
  * recoupling coefficients and GEMM job lists, as tnt_transform_desc /
- * tnt_contract_desc of tenet_hip.h).  Rust is not available in this build
- * environment, so this library restates that layer in C++ to drive the
- * backend end to end:
+ * tnt_contract_desc of tenet_hip.h).  This library restates that layer in
+ * C++ (no Rust toolchain is assumed) to drive the backend end to end:
  *
  *   tnth_space_*         FusionTreeHomSpace + coupled-sector block layout
  *                        (tenet-core/src/fusion_space.rs:2211-2261, :3953-4104;

@@ -32,7 +32,7 @@ def splitmix_normal(seed, n):
 
     Elements 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0, 1]:
     r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).  (The reference has no
-    normal fill; this is the build's definition for the synthetic workloads.)
+    normal fill; this definition serves the synthetic workloads.)
     """
     m = 2 * ((n + 1) // 2)
     k = np.arange(1, m + 1, dtype=np.uint64)

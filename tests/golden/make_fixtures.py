@@ -1,4 +1,4 @@
-"""Extract the golden vectors this build pins against from the reference's data files.
+"""Extract the golden vectors the oracle is pinned against from the reference's data files.
 
 Reads (text only, nothing executed):
   benchmarks/tensorkit_semantic_oracle.out   (TensorKit 0.17 symbol tables,

@@ -446,7 +446,7 @@ def test_split_bf16_error_vs_slices(rt, slices, tol):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("split_bf16", 1e-10)])
 def test_batched_grouped_launch_matches_oracle(rt, prec, tol):
-    """configs[3] shape at small sizes: every merge of one grouped launch vs the FP64 oracle."""
+    """Batched U(1)xZ2 merges at small sizes: every merge of one grouped launch vs the FP64 oracle."""
     from oracle.fusion import HomSpace as OH, Leg as OL
     from tenet_amd.workloads import BatchedMerge, batched_specs
     count, dmin, dmax = 6, 32, 64
