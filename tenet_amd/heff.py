@@ -79,28 +79,35 @@ class TwoSiteHeff:
             setattr(v, name, TensorMap(rt, t.space, _buf=t._buf, _base=t))
         return v
 
+    # The chain (lhs, rhs, lhs axes, rhs axes, output permutation, output
+    # codomain rank, result) of H theta; names index the environments, theta
+    # and the intermediates.
+    CHAIN = (("L", "theta", [2], [0], [0, 1, 2, 3, 4], 2, "x1"),
+             ("x1", "W1", [1, 2], [3, 1], [0, 3, 4, 1, 2], 2, "x2"),
+             ("x2", "W2", [1, 3], [3, 1], [0, 3, 1, 4, 2], 2, "x3"),
+             ("x3", "R", [1, 4], [2, 0], [0, 1, 2, 3], 1, "y"))
+
     def apply(self, theta, scratch=None, precision=None):
         """H_eff theta (four contractions); scratch: dict of reusable intermediates."""
         prec = precision or self.precision
         sc = scratch if scratch is not None else {}
-        x1 = self.L.contract([2], theta, [0], [0, 1, 2, 3, 4], 2, out=sc.get("x1"), precision=prec)
-        sc["x1"] = x1
-        x2 = x1.contract([1, 2], self.W1, [3, 1], [0, 3, 4, 1, 2], 2, out=sc.get("x2"), precision=prec)
-        sc["x2"] = x2
-        x3 = x2.contract([1, 3], self.W2, [3, 1], [0, 3, 1, 4, 2], 2, out=sc.get("x3"), precision=prec)
-        sc["x3"] = x3
-        y = x3.contract([1, 4], self.R, [2, 0], [0, 1, 2, 3], 1, out=sc.get("y"), precision=prec)
-        sc["y"] = y
+        env = {"L": self.L, "W1": self.W1, "W2": self.W2, "R": self.R, "theta": theta}
+        for lhs, rhs, ax, bx, perm, nout, res in self.CHAIN:
+            env[res] = env[lhs].contract(ax, env[rhs], bx, perm, nout, out=sc.get(res), precision=prec)
+            sc[res] = env[res]
         if not hasattr(self, "_flops"):
-            rt = self.rt
-            keys = [
-                (self.L.space, theta.space, [2], [0], [0, 1, 2, 3, 4], 2),
-                (x1.space, self.W1.space, [1, 2], [3, 1], [0, 3, 4, 1, 2], 2),
-                (x2.space, self.W2.space, [1, 3], [3, 1], [0, 3, 1, 4, 2], 2),
-                (x3.space, self.R.space, [1, 4], [2, 0], [0, 1, 2, 3], 1),
-            ]
-            self._flops = sum(rt.contraction(*k).flops for k in keys)
-        return y
+            self._flops = sum(self.rt.contraction(env[l].space, env[r].space, ax, bx, perm, nout).flops
+                              for l, r, ax, bx, perm, nout, _ in self.CHAIN)
+        return env["y"]
+
+    def launch_list(self, theta, scratch):
+        """(artifact, dst, lhs, rhs) device handles of the four tnt_contract calls
+        of apply(theta, scratch), for callers that replay the chain on fixed
+        buffers (apply must have run once with this theta and scratch)."""
+        env = {"L": self.L, "W1": self.W1, "W2": self.W2, "R": self.R, "theta": theta}
+        env.update(scratch)
+        return [(self.rt.contraction(env[l].space, env[r].space, ax, bx, perm, nout).art, env[res]._buf,
+                 env[l]._buf, env[r]._buf) for l, r, ax, bx, perm, nout, res in self.CHAIN]
 
 
 def lanczos(op, v0, iters=30, tol=1e-10):

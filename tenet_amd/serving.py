@@ -15,8 +15,14 @@ Scheduling (host thread):
     while others run overlap on the device;
   * completion is polled with stream events, freeing slot and footprint,
     after which waiting arrivals are admitted.
-Every request uses its own buffers and deterministic kernels, so its result
-is bit-for-bit the same alone or in any mix (tests/test_gpu_serving.py).
+Device buffers: a request holds one buffer set (theta, the three H_eff
+intermediates, the result) of its (stream, bucket) pair from the moment it is
+admitted until its completion event fires; sets are reused by later requests
+on that pair, and the footprint admission counts the bytes of the sets in
+flight.  Issue replays prebuilt argument lists: one fill, four tnt_contract
+calls, one stream-ordered dot and one event record per request.  Every
+request computes on its own buffers with deterministic kernels, so its result
+is bit-for-bit the same alone or in any mix (test in tests/test_gpu_heff.py).
 Latency = completion seen by the scheduler - arrival.
 """
 import ctypes as C
@@ -50,8 +56,14 @@ def default_buckets(dmin=32, dmax=4096, n=15):
 
 
 class _Req:
-    __slots__ = ("rid", "arrival", "D", "seed", "ctx_index", "event", "theta", "y", "scratch",
+    __slots__ = ("rid", "arrival", "D", "seed", "ctx_index", "event", "bufs", "y_copy",
                  "footprint", "t_issue", "t_done", "slot")
+
+
+class _BufSet:
+    """One request's device buffers on one (stream, bucket) pair and the
+    prebuilt tnt_contract argument tuples of the H_eff chain over them."""
+    __slots__ = ("theta", "scratch", "y", "calls", "bytes")
 
 
 class HeffServer:
@@ -71,23 +83,60 @@ class HeffServer:
         self.flops = {}
         self.fp = {}
         self.results = None
+        self._prec = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16}[precision]
+        self._free = {}                       # (stream, D) -> idle buffer sets
+        self._events = [[] for _ in self.rts]  # idle events per stream
+        self._dot = {}                        # D -> (regions, weights, count) of theta's sectors
 
     # -- per-bucket structures (cold: compiled per context once) -------------
     def warm(self):
-        """Compile every (bucket, context) artifact and size the scratch pools."""
+        """Compile every (bucket, stream) artifact and build one buffer set per pair."""
         for D in self.buckets:
             for k, rt in enumerate(self.rts):
-                v = self._view(D, k)
-                th = TensorMap.rand(rt, v.theta_space, seed=1, normal=True)
-                sc = {}
-                y = v.apply(th, sc)
-                self.flops[D] = v._flops
-                fp = 8 * (th.space.len + sum(t.space.len for t in sc.values()))
-                # transform scratch of the four artifacts is held per context; count the
-                # per-request buffers (theta, intermediates, result) in the footprint
-                self.fp[D] = fp
+                bs = self._new_set(D, k)
+                self.flops[D] = self._view(D, k)._flops
+                self.fp[D] = bs.bytes
+                self._free.setdefault((k, D), []).append(bs)
                 rt.synchronize()
-                del th, y, sc
+
+    def _new_set(self, D, k):
+        v = self._view(D, k)
+        bs = _BufSet()
+        bs.theta = TensorMap.zeros(self.rts[k], v.theta_space)
+        bs.scratch = {}
+        bs.y = v.apply(bs.theta, bs.scratch)
+        bs.calls = v.launch_list(bs.theta, bs.scratch)
+        bs.bytes = 8 * (bs.theta.space.len + sum(t.space.len for t in bs.scratch.values()))
+        if D not in self._dot:
+            secs = v.theta_space.sectors
+            regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * c) for _, o, r, c, _ in secs])
+            ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
+            self._dot[D] = (regs, ws, len(secs))
+        return bs
+
+    def _issue(self, r, out_buf, keep):
+        """Queue one request on its stream: fill theta, the four contractions,
+        the dot into the request's result slot, then a completion event."""
+        h = N.hip()
+        k, D = r.ctx_index, r.D
+        rt = self.rts[k]
+        free = self._free.setdefault((k, D), [])
+        bs = free.pop() if free else self._new_set(D, k)
+        ctx = rt._ctx
+        rt._check(h.tnt_buf_fill_random(ctx, bs.theta._buf, r.seed & ((1 << 64) - 1), 1))
+        for art, d, a, b in bs.calls:
+            rt._check(h.tnt_contract(ctx, art, d, a, b, 1.0, 0.0, self._prec))
+        regs, ws, ns = self._dot[D]
+        rt._check(h.tnt_weighted_dot_async(ctx, bs.theta._buf, bs.y._buf, regs, ws, ns, out_buf._buf, r.rid))
+        if keep:  # a copy of the result vector, read at completion (tests)
+            r.y_copy = TensorMap.zeros(rt, bs.y.space)
+            rt._check(h.tnt_axpby(ctx, r.y_copy._buf, bs.y._buf, 1.0, 0.0))
+        ev = self._events[k].pop() if self._events[k] else None
+        if ev is None:
+            ev = C.c_void_p()
+            rt._check(h.tnt_event_create(ctx, C.byref(ev)))
+        rt._check(h.tnt_event_record(ctx, ev))
+        r.bufs, r.event = bs, ev
 
     def _view(self, D, k):
         key = (D, k)
@@ -129,17 +178,8 @@ class HeffServer:
                 r.slot = slots.popleft()
                 r.ctx_index = next_ctx
                 next_ctx = (next_ctx + 1) % len(self.rts)
-                rt = self.rts[r.ctx_index]
-                v = self._view(D, r.ctx_index)
                 r.t_issue = time.perf_counter()
-                r.theta = TensorMap.rand(rt, v.theta_space, seed=seed, normal=True)
-                r.scratch = {}
-                r.y = v.apply(r.theta, r.scratch)
-                self._dot_async(rt, r.theta, r.y, out_buf[r.ctx_index], i)
-                ev = C.c_void_p()
-                rt._check(h.tnt_event_create(rt._ctx, C.byref(ev)))
-                rt._check(h.tnt_event_record(rt._ctx, ev))
-                r.event = ev
+                self._issue(r, out_buf[r.ctx_index], keep_results)
                 issue_s += time.perf_counter() - r.t_issue
                 used += fp
                 inflight.append(r)
@@ -154,17 +194,21 @@ class HeffServer:
                 if q == 1:
                     r.t_done = time.perf_counter() - t0
                     if keep_results:
-                        results[(r.rid, "y")] = r.y.to_numpy()
-                    h.tnt_event_destroy(rt._ctx, r.event)
+                        results[(r.rid, "y")] = r.y_copy.to_numpy()
+                        r.y_copy = None
+                    self._events[r.ctx_index].append(r.event)
+                    self._free[(r.ctx_index, r.D)].append(r.bufs)
                     used -= r.footprint
                     slots.append(r.slot)
                     done.append(r)
-                    r.theta = r.y = r.scratch = None
+                    r.bufs = r.event = None
                 else:
                     still.append(r)
             inflight = still
-            if not inflight and pending and pending[0][1][0] > now:
-                time.sleep(min(1e-4, pending[0][1][0] - now))
+            # idle until the next arrival: sleep through long gaps, spin the last
+            # 0.2 ms (a sleep can overshoot by tens of microseconds)
+            if not inflight and pending and pending[0][1][0] - now > 4e-4:
+                time.sleep(pending[0][1][0] - now - 2e-4)
         wall = time.perf_counter() - t0
         for k, rt in enumerate(self.rts):
             vals = out_buf[k].to_numpy()
@@ -194,13 +238,13 @@ class HeffServer:
         rule = Rule("U1")
         return HomSpace(rule, [GradedSpace(rule, [(0, max(1, n))])], [GradedSpace(rule, [(0, 1)])])
 
-    def _dot_async(self, rt, a, b, out, slot):
-        secs = a.space.sectors
-        regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * k) for _, o, r, k, _ in secs])
-        ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
-        rt._check(N.hip().tnt_weighted_dot_async(rt._ctx, a._buf, b._buf, regs, ws, len(secs), out._buf, slot))
-
     def close(self):
+        h = N.hip()
+        for k, evs in enumerate(self._events):
+            for ev in evs:
+                h.tnt_event_destroy(self.rts[k]._ctx, ev)
+        self._events = [[] for _ in self.rts]
+        self._free.clear()
         self.env.clear()
         self.views.clear()
         for rt in self.rts:
