@@ -124,7 +124,16 @@ struct tnt_plan {
   DevArrays d;
   std::vector<void*> allocs;
   double bytes_moved = 0;  // algorithmic bytes of one replay (read src + write dst)
+  bool one_launch = false;  // items of several classes, small: one launch of every class
 };
+
+// Plans of several item classes that move fewer bytes than this (default
+// kSmallPlanBytes; TENET_AMD_SMALL_PLAN_BYTES overrides at upload) replay in
+// one launch.
+static double small_plan_bytes() {
+  const char* e = std::getenv("TENET_AMD_SMALL_PLAN_BYTES");
+  return e ? std::atof(e) : kSmallPlanBytes;
+}
 
 struct GemmTable {
   GJob* jobs = nullptr;
@@ -543,6 +552,9 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     return rc;
   }
   p->d.n_items = (uint32_t)items.size();
+  int classes = 0;
+  for (int c = 0; c < XC_COUNT; ++c) classes += p->d.cls_count[c] > 0;
+  p->one_launch = classes > 1 && bytes < small_plan_bytes();
   *out = p;
   return TNT_OK;
 }
@@ -551,6 +563,10 @@ static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double*
                      double beta, int overwrite) {
   XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.coeffs, dst, src, alpha, beta, overwrite};
   TimedScope ts(ctx, 0, p->bytes_moved);
+  if (p->one_launch) {
+    launch_xform_any(a, p->d.n_items, ctx->stream);
+    return;
+  }
   for (int c = 0; c < XC_COUNT; ++c) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
 }
 

@@ -81,13 +81,17 @@ enum XClass : int32_t {
   XC_TRANSPOSE = 4,
   XC_COUNT = 5
 };
-inline int xclass_of(int kind, int nsrc) {
+__host__ __device__ inline int xclass_of(int kind, int nsrc) {
   if (kind == XK_TRANSPOSE) return XC_TRANSPOSE;
   if (kind != XK_MULTI) return XC_SINGLE;
   return nsrc <= 4 ? XC_MULTI4 : nsrc <= 16 ? XC_MULTI16 : XC_MULTI32;
 }
 // Items [begin, begin + n) of one class.
 void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s);
+// All items of a plan in one launch (any class).
+void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s);
+// Plans that move fewer bytes than this run as one launch of every class.
+constexpr double kSmallPlanBytes = 1.0 * (1 << 20);
 // coeffs[t] = idx[t] >= 0 ? scale[t] * src[idx[t]] : 0 (coefficients of a fused
 // narrow contraction gathered from its small operand).
 void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,

@@ -359,6 +359,34 @@ __global__ __launch_bounds__(256) void xform_multi_kernel(XformArgs a, uint32_t 
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) multi_item<NS>(a, a.items[begin + i]);
 }
 
+// Every class in one launch (small plans: one launch costs more than the
+// occupancy the per-class kernels buy).  Items of one workgroup are uniform,
+// so the transpose barrier is reached by the whole workgroup.
+__global__ __launch_bounds__(256) void xform_any_kernel(XformArgs a, uint32_t n) {
+  __shared__ double tile[kTransposeTile][kTransposeTile + 1];
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const XItem it = a.items[i];
+    const XBlock& b = a.blocks[it.block];
+    switch (xclass_of(b.kind, b.nsrc)) {
+      case XC_SINGLE:
+        single_item(a, it);
+        break;
+      case XC_TRANSPOSE:
+        transpose_item(a, it, tile);
+        break;
+      case XC_MULTI4:
+        multi_item<4>(a, it);
+        break;
+      case XC_MULTI16:
+        multi_item<16>(a, it);
+        break;
+      default:
+        multi_item<kMaxMultiSrc>(a, it);
+        break;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void coeff_gather_kernel(double* coeffs, const double* src,
                                                           const int64_t* idx, const double* scale,
                                                           uint32_t n) {
@@ -373,6 +401,12 @@ void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, 
                          uint32_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(coeff_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, coeffs, src, idx, scale, n);
+}
+
+void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s) {
+  if (n_items == 0) return;
+  const uint32_t grid = n_items < 4096u ? n_items : 4096u;
+  hipLaunchKernelGGL(xform_any_kernel, dim3(grid), dim3(256), 0, s, a, n_items);
 }
 
 void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s) {

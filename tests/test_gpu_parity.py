@@ -273,6 +273,27 @@ def test_transposed_blocks_through_lds_tiles(rt, cod, dom, p1, p2):
     assert np.array_equal(out.to_numpy(), 2.0 * want + -0.5 * before)
 
 
+@pytest.mark.parametrize("small_plan_bytes", ["0", "1e12"])
+def test_multi_class_plans_per_class_and_in_one_launch(small_plan_bytes):
+    """SU(2) permutations mixing Single, Multi and Scale items: replayed per
+    item class and as one launch of every class, same results."""
+    import os
+    os.environ["TENET_AMD_SMALL_PLAN_BYTES"] = small_plan_bytes
+    r = Runtime(0)
+    try:
+        ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, True)], [(SU2V, False), (SU2V, False)])
+        t = TensorMap.rand(r, ps, seed=17)
+        src = t.to_numpy()
+        for p1, p2 in (([0, 2], [1, 3]), ([3, 0], [2, 1]), ([2], [0, 1, 3])):
+            got = t.permute(p1, p2).to_numpy()
+            want = OTensor(os_, src).permute(p1, p2).data
+            assert np.abs(got - want).max() <= 1e-14 * max(1.0, np.abs(want).max())
+        del t
+    finally:
+        del os.environ["TENET_AMD_SMALL_PLAN_BYTES"]
+        r.close()
+
+
 def test_permute_axpby_into_existing(rt):
     ps, os_ = spaces("SU2", [(SU2V, False), (SU2V, False)], [(SU2V, False), (SU2V, False)])
     t = TensorMap.rand(rt, ps, seed=5)
