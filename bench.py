@@ -34,7 +34,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 GEMM_KERNELS = ["gemm_kernel<{}, 128, 128>", "gemm_kernel<{}, 64, 64>", "gemm_kernel<{}, 128, 64>",
-                "gemm_kernel<{}, 64, 128>", "gemm_skinny_kernel<{}>", "gemm_kernel<{}, 128, 128, 2, 4>"]
+                "gemm_kernel<{}, 64, 128>", "gemm_skinny_kernel<{}>", "gemm_kernel<{}, 128, 128, 2, 4>",
+                "gemm_small_kernel<{}>"]
 SPLIT_PRODUCTS = 15  # bf16 MFMA products per FP64 product at five digit planes
 
 
@@ -47,6 +48,12 @@ def roofline(precision, dom):
     rec = {"bound": "mfma", "unit": "TFLOP/s", "traffic": None, "kernel": dom["kernel"],
            "launches": dom["launches"], "avg_launch_ms": avg_s * 1e3,
            "flops_per_launch": per_launch}
+    if dom.get("bytes"):
+        # memory-bound batch of tiny GEMMs: operands read once, results written once
+        gbs = dom["bytes"] / max(1, dom["launches"]) / avg_s / 1e9 if avg_s else 0.0
+        rec.update(bound="hbm", unit="GB/s", achieved=gbs, peak=HBM_PEAK_GBS, frac=gbs / HBM_PEAK_GBS,
+                   bytes_per_launch=dom["bytes"] / max(1, dom["launches"]), tflops=f64_eq)
+        return rec
     if precision == "split_bf16":
         # bf16 MFMA work actually issued: SPLIT_PRODUCTS digit-plane products per FP64 product
         ach = f64_eq * SPLIT_PRODUCTS
@@ -277,7 +284,7 @@ def run_gemm_vs_vendor(args):
     t_ours, cfgs = 0.0, []
     for jb, grp in zip(jobs, groups):
         best = None
-        for cfg in (-1, 0, 1, 2, 3, 5):
+        for cfg in (-1, 0, 1, 2, 3, 5, 6):
             h.tnt_set_tile_config(cfg)
             t = timed(lambda: rt._check(h.tnt_gemm_grouped(rt._ctx, jb, len(grp), 1.0, 0.0, 0)))
             if best is None or t < best[0]:
@@ -358,6 +365,7 @@ def main():
 
     rt = Runtime(0)
     batched_err = None
+    batched_cfg = None
     if args.workload == "copy_calibration":
         # y = x over 2^27 doubles (1 GiB read + 1 GiB written per launch): the
         # reference point that scales FETCH_SIZE / WRITE_SIZE to bytes
@@ -407,7 +415,33 @@ def main():
         num = sum(float(np.linalg.norm(g - r) ** 2) for g, r in zip(got, ref))
         den = sum(float(np.linalg.norm(r) ** 2) for r in ref)
         batched_err = (num / den) ** 0.5
-        wl.capture()
+        if args.autotune == "on":
+            # tile shape of the grouped launch by measurement of graph replays: the
+            # size rule, all 64 x 64 workgroup tiles, or all one-wave 32 x 32 tiles
+            hh = N.hip()
+            trials = []
+            for c in (-1, 1, 6):
+                hh.tnt_set_tile_config(c)
+                wl.graph = None
+                wl.capture()
+                for _ in range(3):
+                    wl.run()
+                rt.synchronize()
+                t_ = time.perf_counter()
+                for _ in range(10):
+                    wl.run()
+                rt.synchronize()
+                trials.append(((time.perf_counter() - t_) / 10, c, wl.graph))
+                log(f"[autotune] batched cfg {c}: {trials[-1][0] * 1e3:.3f} ms per graph replay")
+            best = min(trials, key=lambda x: x[0])
+            for _, c, g in trials:
+                if g is not best[2]:
+                    rt._check(hh.tnt_graph_destroy(rt._ctx, g))
+            batched_cfg = best[1]
+            wl.graph = best[2]
+            hh.tnt_set_tile_config(batched_cfg)  # kept for the event-timed eager region
+        else:
+            wl.capture()
         log(f"[bench] batched: {wl.n_jobs} sector GEMMs, {args.precision} rel. Frobenius error vs FP64 {batched_err:.3e}")
     else:
         wl = SU2FMoveContract(rt, args.mult, 8, args.precision)
@@ -419,7 +453,7 @@ def main():
         wl.run()
     rt.synchronize()
     h = N.hip()
-    tuned = None
+    tuned = {"batched": batched_cfg} if batched_cfg is not None else None
     if args.autotune == "on" and args.workload != "batched":
         from tenet_amd.autotune import autotune
         conts = list(wl.contractions) if hasattr(wl, "contractions") else [wl.contraction]
@@ -468,7 +502,7 @@ def main():
         # the step is one hipGraph replay of the grouped launch(es); graph replays are
         # not bracketed per kernel, so the step time (device-synchronised) stands in
         dom = {"kernel": f"grouped GEMM hipGraph replay ({args.precision})", "launches": args.steps,
-               "ms": ms * args.steps, "flops": wl.flops * args.steps}
+               "ms": ms * args.steps, "flops": wl.flops * args.steps, "bytes": wl.bytes * args.steps}
     elif args.precision == "split_bf16":
         dom = {"kernel": "split-BF16 table run (digit pre-pass + bf16 MFMA GEMM; FP64 kernels for narrow jobs)",
                "launches": gemm_launches, "ms": gemm_ms, "flops": gemm_work}

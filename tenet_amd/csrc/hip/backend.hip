@@ -112,6 +112,7 @@ struct DevArrays {  // device copy of a normalised transform plan
   XBlock* blocks = nullptr;
   XLayout* layouts = nullptr;
   XItem* items = nullptr;
+  XFlat* flat = nullptr;  // records of the XC_FLAT class, in item order
   double* coeffs = nullptr;
   uint32_t n_items = 0;
   uint32_t cls_begin[XC_COUNT] = {};  // items are grouped by class (one launch each)
@@ -420,6 +421,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   std::vector<XBlock> blocks;
   std::vector<XLayout> layouts;
   std::vector<XItem> cls_items[XC_COUNT];
+  std::vector<XFlat> flat;
   std::string err;
   double bytes = 0;
   auto add_block = [&](int kind, const std::vector<uint32_t>& dl, const std::vector<uint32_t>& sl,
@@ -480,8 +482,43 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
     // Single blocks: kSingleItem-element items (kSingleUnroll per thread) swept in
     // order by a persistent grid; Multi: ~8K element-updates per item, weighted by
-    // its layouts
-    std::vector<XItem>& items = cls_items[xclass_of(kind, (int)sl.size())];
+    // its layouts.  Single items of one outer index go to the flattened class.
+    struct ItemList {
+      std::vector<XItem>& items;
+      std::vector<XItem>& flat_items;
+      std::vector<XFlat>& flat;
+      const XBlock& b;
+      const std::vector<NLayout>& ls;
+      double c;
+      void push_back(const XItem& it) {
+        if (b.kind != XK_SINGLE || it.o_count != 1) {
+          items.push_back(it);
+          return;
+        }
+        int64_t od = ls[0].offset, os = ls[1].offset;
+        uint64_t o = it.o_begin;
+        for (int d = 2; d < b.rank; ++d) {
+          const int64_t i = (int64_t)(o % (uint64_t)b.dims[d]);
+          o /= (uint64_t)b.dims[d];
+          od += i * ls[0].strides[d];
+          os += i * ls[1].strides[d];
+        }
+        XFlat f{};
+        f.od = od;
+        f.os = os;
+        f.d0 = ls[0].strides[0];
+        f.s0 = ls[1].strides[0];
+        f.d1 = b.rank > 1 ? ls[0].strides[1] : 0;
+        f.s1 = b.rank > 1 ? ls[1].strides[1] : 0;
+        f.c = c;
+        f.n0 = (uint32_t)b.dims[0];
+        f.e_begin = it.e_begin;
+        f.e_count = it.e_count;
+        flat_items.push_back(it);
+        flat.push_back(f);
+      }
+    } items{cls_items[xclass_of(kind, (int)sl.size())], cls_items[XC_FLAT], flat, blocks.back(), ls,
+            kind == XK_SINGLE ? d->coeffs[coeff] : 0.0};
     if (kind == XK_TRANSPOSE) {  // one item per 64 x 64 tile of every plane
       const int64_t ntx = (dims[0] + kTransposeTile - 1) / kTransposeTile;
       const int64_t nty = (dims[1] + kTransposeTile - 1) / kTransposeTile;
@@ -545,6 +582,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   int rc = upload_vec(ctx, blocks, &p->d.blocks, p->allocs);
   if (!rc) rc = upload_vec(ctx, layouts, &p->d.layouts, p->allocs);
   if (!rc) rc = upload_vec(ctx, items, &p->d.items, p->allocs);
+  if (!rc && !flat.empty()) rc = upload_vec(ctx, flat, &p->d.flat, p->allocs);
   if (!rc) rc = upload_vec(ctx, coeffs, &p->d.coeffs, p->allocs);
   if (rc) {
     for (auto a : p->allocs) pool_free(ctx, a);
@@ -552,8 +590,9 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     return rc;
   }
   p->d.n_items = (uint32_t)items.size();
-  int classes = 0;
-  for (int c = 0; c < XC_COUNT; ++c) classes += p->d.cls_count[c] > 0;
+  int classes = 0;  // launches of a replay (flattened and other Single items share one)
+  for (int c = 0; c < XC_COUNT; ++c)
+    if (c != XC_FLAT) classes += p->d.cls_count[c] > 0 || (c == XC_SINGLE && p->d.cls_count[XC_FLAT] > 0);
   p->one_launch = classes > 1 && bytes < small_plan_bytes();
   *out = p;
   return TNT_OK;
@@ -561,13 +600,15 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
 
 static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double* src, double alpha,
                      double beta, int overwrite) {
-  XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.coeffs, dst, src, alpha, beta, overwrite};
+  XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.flat, p->d.coeffs, dst, src, alpha, beta, overwrite};
   TimedScope ts(ctx, 0, p->bytes_moved);
   if (p->one_launch) {
     launch_xform_any(a, p->d.n_items, ctx->stream);
     return;
   }
-  for (int c = 0; c < XC_COUNT; ++c) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
+  launch_xform_single(a, p->d.cls_begin[XC_SINGLE], p->d.cls_count[XC_SINGLE], p->d.cls_count[XC_FLAT], ctx->stream);
+  for (int c = 0; c < XC_COUNT; ++c)
+    if (c != XC_SINGLE && c != XC_FLAT) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,19 +625,30 @@ static std::vector<TileRule> tile_table;  // tuned per-shape rules (tnt_set_tile
 static int tile_override = -1;  // set by tnt_set_tile_config (autotuner sweeps)
 
 // Table-level default: the largest tile shape that still gives every CU at
-// least ~2 (128x128) or 1 (64x128 / 128x64) workgroups; narrow outputs go to
-// the vector-unit kernel.  A per-artifact override (autotuner) wins.
+// least ~2 (128x128) or 1 (64x128 / 128x64) workgroups and covers the jobs
+// with little padding (>= 80 % of its area used), counted over the jobs that
+// take a workgroup tile; narrow outputs go to the vector-unit kernel.  The
+// one-wave 32 x 32 tiles (TILE_SMALL) are chosen by measurement only (an
+// extra tile shape is an extra launch).  A per-artifact override (autotuner)
+// wins.
 static int default_family(const std::vector<GJob>& jobs) {
-  auto count = [&](int cfg) {
+  auto count = [&](int cfg, double* used_frac) {
     int64_t t = 0;
+    double used = 0, padded = 0;
+    const int bm = tile_rows(cfg), bn = tile_cols(cfg);
     for (const auto& j : jobs) {
       if (j.m <= 0 || j.n <= 0 || j.n <= 16) continue;
-      t += (int64_t)((j.m + tile_rows(cfg) - 1) / tile_rows(cfg)) * ((j.n + tile_cols(cfg) - 1) / tile_cols(cfg));
+      const int64_t tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
+      t += tm * tn;
+      used += double(j.m) * double(j.n);
+      padded += double(tm * bm) * double(tn * bn);
     }
+    *used_frac = padded > 0 ? used / padded : 0.0;
     return t;
   };
-  if (count(TILE_128x128) >= 4 * 256) return TILE_128x128;
-  if (count(TILE_64x128) >= 2 * 256) return TILE_64x128;
+  double f128 = 0, f64x128 = 0;
+  if (count(TILE_128x128, &f128) >= 4 * 256 && f128 >= 0.8) return TILE_128x128;
+  if (count(TILE_64x128, &f64x128) >= 2 * 256 && f64x128 >= 0.8) return TILE_64x128;
   return TILE_64x64;
 }
 

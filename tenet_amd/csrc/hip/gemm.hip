@@ -23,6 +23,8 @@
 //            FP32 (v_mfma_f32_16x16x4f32):  i = 4 * (lane / 16) + r
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace tnt_dev {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -252,6 +254,106 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
   }
 }
 
+// Small-job GEMM: each wave of the workgroup computes its own 32 x 32 tile
+// (2 x 2 MFMA fragments) with operands loaded straight from global memory into
+// registers -- no LDS, no barriers, waves independent.  For the many tiny
+// sector GEMMs of batched contractions (median 30 x 16 x 30), where a 64 x 64
+// workgroup tile is mostly padding and the launch is bound by memory latency.
+// Inside a block of 8 k the lanes of group q = lane / 16 supply k = k0 + q + 4p
+// to the p-th MFMA.  Loads are unconditional (indices clamped into the job's
+// operands); out-of-range k is zeroed by select, so the loop body is
+// straight-line code and each stage keeps its own registers.
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_tiles) {
+  constexpr int MT = 2, NT = 2;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, q4 = lane >> 4;
+  const uint32_t ti = blockIdx.x * 4u + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ti >= n_tiles) return;
+  const GTile tile = g.tiles[ti];
+  const GJob job = g.jobs[tile.job];
+  const int m = job.m, n = job.n, k = job.k;
+  const int m0 = tile.tm * 32, n0 = tile.tn * 32;
+  const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
+  const int64_t sAi = a_adj ? job.lda : 1, sAk = a_adj ? 1 : job.lda;
+  const int64_t sBj = b_adj ? 1 : job.ldb, sBk = b_adj ? job.ldb : 1;
+  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
+  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
+  gptr<const double> pa[MT], pb[NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) pa[i] = A + (int64_t)min(m0 + 16 * i + l16, m - 1) * sAi;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) pb[j] = B + (int64_t)min(n0 + 16 * j + l16, n - 1) * sBj;
+  using acc_t = typename MfmaOp<T>::acc_t;
+  acc_t acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  double ra[2][2][MT], rb[2][2][NT];  // [stage][p][fragment]
+  const int nb = (k + 7) / 8;
+  auto load = [&](auto sc, int blk) {
+    constexpr int s = decltype(sc)::value;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int kk = min(blk * 8 + q4 + 4 * p, k - 1);
+#pragma unroll
+      for (int i = 0; i < MT; ++i) ra[s][p][i] = pa[i][kk * sAk];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) rb[s][p][j] = pb[j][kk * sBk];
+    }
+  };
+  auto compute = [&](auto sc, int blk) {
+    constexpr int s = decltype(sc)::value;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const bool ok = blk * 8 + q4 + 4 * p < k;
+      T af[MT], bf[NT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) af[i] = (T)(ok ? ra[s][p][i] : 0.0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bf[j] = (T)(ok ? rb[s][p][j] : 0.0);
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+    }
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  // two register stages: block t + 1 is in flight while block t computes; the
+  // scheduling barriers keep the compiler from hoisting the next stage's loads
+  // above the current compute (which would make it wait for them)
+  load(S0{}, 0);
+  for (int t = 0; t < nb; t += 2) {
+    load(S1{}, t + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(S0{}, t);
+    __builtin_amdgcn_sched_barrier(0);
+    load(S0{}, t + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(S1{}, t + 1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
+  const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int col = n0 + 16 * j + l16;
+      if (col >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + 16 * i + MfmaOp<T>::row(lane, r);
+        if (row < m) {
+          const gptr<double> p = D + row + (int64_t)col * job.ldd;
+          const double v = alpha * (double)acc[i][j][r];
+          *p = (beta != 0.0) ? v + beta * *p : v;
+        }
+      }
+    }
+}
+
 template <typename T>
 static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
   if (n_tiles == 0) return;
@@ -270,6 +372,9 @@ static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t
       break;
     case TILE_128x128_W8:
       hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4>), dim3(n_tiles), dim3(512), 0, s, a);
+      break;
+    case TILE_SMALL:  // four independent one-wave tiles per workgroup
+      hipLaunchKernelGGL((gemm_small_kernel<T>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
       break;
     default:
       hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), dim3(n_tiles), dim3(256), 0, s, a);

@@ -21,7 +21,7 @@ __device__ __forceinline__ gptr<T> gmem(uintptr_t addr) {
   return (gptr<T>)addr;
 }
 constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in registers
-constexpr int kSingleUnroll = 4;  // elements per thread of a Single work item
+constexpr int kSingleUnroll = 8;  // elements per thread of a Single work item
 constexpr int kSingleItem = 256 * kSingleUnroll;  // plane elements per Single work item
 
 // XK_TRANSPOSE: a Single block whose source is contiguous along another axis
@@ -57,10 +57,23 @@ struct XItem {
   uint32_t e_count;
 };
 
+// A Single item of one outer index, flattened at upload: base offsets of the
+// item's plane and the plane strides, so that the kernel reads one record per
+// item (no block / layout lookups, no outer-index division).
+struct XFlat {
+  int64_t od, os;          // destination / source offset of plane element (0, 0)
+  int64_t d0, d1, s0, s1;  // plane strides (dims 0 and 1)
+  double c;                // block coefficient (alpha applied in the kernel)
+  uint32_t n0;             // plane extent along dims[0]
+  uint32_t e_begin, e_count;
+  uint32_t pad;
+};
+
 struct XformArgs {
   const XBlock* blocks;
   const XLayout* layouts;
   const XItem* items;
+  const XFlat* flat;  // records of the XC_FLAT items, in item order
   const double* coeffs;
   double* dst;
   const double* src;
@@ -69,25 +82,31 @@ struct XformArgs {
   int overwrite;
 };
 
-// Work-item classes of a plan, one kernel each: Single and Scale items, Multi
-// items by source count (<= 4, <= 16, <= kMaxMultiSrc), transposed Single
-// tiles.  A transpose item is one 64 x 64 tile: e_begin = tile index in the
-// plane (x tiles fastest), o_begin = outer index.
+// Work-item classes of a plan: Single and Scale items, Multi items by source
+// count (<= 4, <= 16, <= kMaxMultiSrc), transposed Single tiles, and Single
+// items of one outer index (XC_FLAT, flattened records).  One kernel per
+// class, except that XC_FLAT and XC_SINGLE share one launch.  A transpose
+// item is one 64 x 64 tile: e_begin = tile index in the plane (x tiles
+// fastest), o_begin = outer index.
 enum XClass : int32_t {
   XC_SINGLE = 0,
   XC_MULTI4 = 1,
   XC_MULTI16 = 2,
   XC_MULTI32 = 3,
   XC_TRANSPOSE = 4,
-  XC_COUNT = 5
+  XC_FLAT = 5,
+  XC_COUNT = 6
 };
 __host__ __device__ inline int xclass_of(int kind, int nsrc) {
   if (kind == XK_TRANSPOSE) return XC_TRANSPOSE;
   if (kind != XK_MULTI) return XC_SINGLE;
   return nsrc <= 4 ? XC_MULTI4 : nsrc <= 16 ? XC_MULTI16 : XC_MULTI32;
 }
-// Items [begin, begin + n) of one class.
+// Items [begin, begin + n) of one class (not XC_SINGLE / XC_FLAT).
 void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s);
+// The Single class in one launch: n_flat flattened records (XformArgs::flat),
+// then the XC_SINGLE items [begin, begin + n).
+void launch_xform_single(const XformArgs& a, uint32_t begin, uint32_t n, uint32_t n_flat, hipStream_t s);
 // All items of a plan in one launch (any class).
 void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s);
 // Plans that move fewer bytes than this run as one launch of every class.
@@ -125,6 +144,8 @@ struct GemmArgs {
 // on the vector units.
 // TILE_128x128_W8: 128 x 128 on 8 waves (2 x 4, each 64 x 32), two waves of
 // the workgroup per SIMD.
+// TILE_SMALL: 32 x 32 per wave, four independent waves per workgroup, operands
+// loaded straight into registers (tiny sector GEMMs of batched contractions).
 enum TileCfg : int32_t {
   TILE_128x128 = 0,
   TILE_64x64 = 1,
@@ -132,12 +153,15 @@ enum TileCfg : int32_t {
   TILE_64x128 = 3,
   TILE_SKINNY = 4,
   TILE_128x128_W8 = 5,
-  TILE_CFG_COUNT = 6
+  TILE_SMALL = 6,
+  TILE_CFG_COUNT = 7
 };
 inline int tile_rows(int cfg) {
+  if (cfg == TILE_SMALL) return 32;
   return cfg == TILE_SKINNY ? 256 : (cfg == TILE_128x128 || cfg == TILE_128x64 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
 inline int tile_cols(int cfg) {
+  if (cfg == TILE_SMALL) return 32;
   return cfg == TILE_SKINNY ? 16 : (cfg == TILE_128x128 || cfg == TILE_64x128 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
 

@@ -15,6 +15,10 @@
 // indices [o_begin, o_begin + o_count).  The outer index is wave-uniform, so
 // per-layout base offsets live in scalar registers; lanes sweep the plane
 // destination-fastest (coalesced writes) with an incremental (x, y) walk.
+// Single items of one outer index are also flattened at upload into XFlat
+// records (plane base offsets and strides), read with one scalar load per
+// item; the Single-class kernel runs them before the Single items of several
+// outer indices, in the same launch.
 #include "kernels.hpp"
 
 namespace tnt_dev {
@@ -96,13 +100,12 @@ __device__ void single_item(const XformArgs& a, const XItem it) {
     const uint32_t sx = 256u % n0u, sy = 256u / n0u;
     double v[kSingleUnroll];
     int64_t dp[kSingleUnroll];
+    bool ok[kSingleUnroll];
 #pragma unroll
     for (int u = 0; u < kSingleUnroll; ++u) {
-      const uint32_t f = f0 + 256u * u;
-      if (f < it.e_count) {
-        v[u] = a.src[os + x * s0 + y * s1];
-        dp[u] = od + x * d0 + y * d1;
-      }
+      ok[u] = f0 + 256u * u < it.e_count;  // unconditional loads (see flat_item)
+      v[u] = a.src[ok[u] ? os + x * s0 + y * s1 : os];
+      dp[u] = od + x * d0 + y * d1;
       x += sx;
       y += sy;
       if (x >= n0u) {
@@ -112,8 +115,7 @@ __device__ void single_item(const XformArgs& a, const XItem it) {
     }
 #pragma unroll
     for (int u = 0; u < kSingleUnroll; ++u) {
-      const uint32_t f = f0 + 256u * u;
-      if (f < it.e_count) {
+      if (ok[u]) {
         double* p = a.dst + dp[u];
         *p = readd ? c * v[u] + a.beta * *p : c * v[u];
       }
@@ -141,26 +143,25 @@ __device__ void single_item(const XformArgs& a, const XItem it) {
       const double c = alpha * a.coeffs[b.coeff];
       const double* __restrict__ src = a.src;
       double* __restrict__ dst = a.dst;
-      // four independent loads in flight per lane before their stores
+      // four independent loads in flight per lane before their stores; the
+      // loads are unconditional (past the end a lane re-reads element (0, 0))
       while (e < w.e_hi) {
         double v[kUnroll];
         int64_t dp[kUnroll];
-        int cnt = 0;
+        bool ok[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          if (e < w.e_hi) {
-            v[u] = src[os + x * s0 + y * s1];
-            dp[u] = od + x * d0 + y * d1;
-            cnt = u + 1;
-            e += 64;
-            x += dx;
-            y += dy;
-            if (x >= n0) { x -= n0; y += 1; }
-          }
+          ok[u] = e < w.e_hi;
+          v[u] = src[ok[u] ? os + x * s0 + y * s1 : os];
+          dp[u] = od + x * d0 + y * d1;
+          e += 64;
+          x += dx;
+          y += dy;
+          if (x >= n0) { x -= n0; y += 1; }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          if (u < cnt) {
+          if (ok[u]) {
             double* p = dst + dp[u];
             *p = readd ? c * v[u] + beta * *p : c * v[u];
           }
@@ -178,6 +179,49 @@ __device__ void single_item(const XformArgs& a, const XItem it) {
         if (x >= n0) { x -= n0; y += 1; }
       }
     }
+  }
+}
+
+// One flattened Single item: element e = e_begin + 64 * (4u + wave) + lane,
+// u < kSingleUnroll.  The loads are unconditional (a lane past the item reads
+// plane element (0, 0) and discards it), so all of a lane's loads are in
+// flight before its first store; a predicated load would be waited for at
+// the branch merge.
+__device__ __forceinline__ void flat_item(const XformArgs& a, const XFlat& f, double alpha, double beta,
+                                          bool readd) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const double c = alpha * f.c;
+  const uint32_t f0 = 64u * wave + lane;
+  uint32_t x = (f.e_begin + f0) % f.n0, y = (f.e_begin + f0) / f.n0;
+  const uint32_t sx = 256u % f.n0, sy = 256u / f.n0;
+  double v[kSingleUnroll];
+  int64_t dp[kSingleUnroll];
+  bool ok[kSingleUnroll];
+#pragma unroll
+  for (int u = 0; u < kSingleUnroll; ++u) {
+    ok[u] = f0 + 256u * u < f.e_count;
+    const int64_t sp = ok[u] ? f.os + x * f.s0 + y * f.s1 : f.os;
+    dp[u] = ok[u] ? f.od + x * f.d0 + y * f.d1 : f.od;
+    v[u] = a.src[sp];
+    x += sx;
+    y += sy;
+    if (x >= f.n0) {
+      x -= f.n0;
+      y += 1;
+    }
+  }
+  if (readd) {
+    double w[kSingleUnroll];
+#pragma unroll
+    for (int u = 0; u < kSingleUnroll; ++u) w[u] = a.dst[dp[u]];
+#pragma unroll
+    for (int u = 0; u < kSingleUnroll; ++u)
+      if (ok[u]) a.dst[dp[u]] = c * v[u] + beta * w[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < kSingleUnroll; ++u)
+      if (ok[u]) a.dst[dp[u]] = c * v[u];
   }
 }
 
@@ -233,8 +277,9 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
         for (int i = 0; i < NS; ++i) {
           v0[i] = v1[i] = 0.0;
           if (i < ns) {
+            // unconditional: past the end the second load repeats the first
             v0[i] = src[sb[i] + x * sx[i] + y * sy[i]];
-            if (two) v1[i] = src[sb[i] + x2 * sx[i] + y2 * sy[i]];
+            v1[i] = src[two ? sb[i] + x2 * sx[i] + y2 * sy[i] : sb[i] + x * sx[i] + y * sy[i]];
           }
         }
         for (int j = 0; j < nd; ++j) {
@@ -349,6 +394,20 @@ __global__ __launch_bounds__(256) void xform_single_kernel(XformArgs a, uint32_t
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) single_item(a, a.items[begin + i]);
 }
 
+// The whole Single class in one launch: the n_flat flattened items (one
+// record each, no block / layout lookups) first, then the n Single / Scale
+// items of several outer indices from [begin, begin + n) of the item table.
+__global__ __launch_bounds__(256) void xform_single_all_kernel(XformArgs a, uint32_t begin, uint32_t n,
+                                                               uint32_t n_flat) {
+  const double alpha = a.alpha, beta = a.beta;
+  const bool readd = a.overwrite == 0 && beta != 0.0;
+  const uint32_t total = n_flat + n;
+  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+    if (i < n_flat) flat_item(a, a.flat[i], alpha, beta, readd);
+    else single_item(a, a.items[begin + (i - n_flat)]);
+  }
+}
+
 __global__ __launch_bounds__(256) void xform_transpose_kernel(XformArgs a, uint32_t begin, uint32_t n) {
   __shared__ double tile[kTransposeTile][kTransposeTile + 1];
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) transpose_item(a, a.items[begin + i], tile);
@@ -407,6 +466,13 @@ void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s) {
   if (n_items == 0) return;
   const uint32_t grid = n_items < 4096u ? n_items : 4096u;
   hipLaunchKernelGGL(xform_any_kernel, dim3(grid), dim3(256), 0, s, a, n_items);
+}
+
+void launch_xform_single(const XformArgs& a, uint32_t begin, uint32_t n, uint32_t n_flat, hipStream_t s) {
+  const uint32_t total = n + n_flat;
+  if (total == 0) return;
+  const uint32_t grid = total < 4096u ? total : 4096u;
+  hipLaunchKernelGGL(xform_single_all_kernel, dim3(grid), dim3(256), 0, s, a, begin, n, n_flat);
 }
 
 void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipStream_t s) {

@@ -269,6 +269,8 @@ class BatchedMerge:
             self.pairs.append((A, B, Cm, c))
         self.jobs = (N.GemmPtrJob * len(jobs))(*jobs)
         self.n_jobs = len(jobs)
+        # algorithmic bytes of one launch: every operand read once, every result written once
+        self.bytes = float(sum(8 * (j.rows * j.contracted + j.contracted * j.cols + j.rows * j.cols) for j in jobs))
         self.graph = None
 
     def run(self, precision=None):

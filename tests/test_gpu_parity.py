@@ -99,7 +99,7 @@ def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     assert rel(got, w) < tol
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 5, 6])
 @pytest.mark.parametrize("lop,rop", [(0, 0), (1, 1)])
 def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop):
     """Every tile configuration (forced) on ragged FP64 jobs, both operand ops."""
@@ -126,6 +126,41 @@ def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop):
         _free(rt, L, R, D)
     w = np.concatenate(want)
     assert rel(got, w) < 1e-14
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("lop,rop", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_gemm_small_tiles_bit_identical_to_workgroup_tiles(rt, prec, lop, rop):
+    """The one-wave 32 x 32 kernel (registers only) and the 64 x 64 LDS kernel
+    issue the same MFMA sequence over the same k groups of four, so their
+    results agree bit for bit (ragged jobs, k tails, beta != 0)."""
+    rng = np.random.default_rng(100 + 4 * lop + 2 * rop + prec)
+    shapes = [(30, 16, 30), (1, 7, 33), (33, 1, 17), (64, 64, 64), (65, 9, 31), (100, 37, 90)]
+    jobs, lhs, rhs, dst = [], [], [], []
+    lo = ro = do = 0
+    for m, k, n in shapes:
+        a, b = rng.normal(size=(m, k)), rng.normal(size=(k, n))
+        lhs.append((a.T if lop else a).reshape(-1, order="F"))
+        rhs.append((b.T if rop else b).reshape(-1, order="F"))
+        dst.append(rng.normal(size=m * n))
+        jobs.append(N.GemmJob(do, lo, ro, m, k, n))
+        lo, ro, do = lo + m * k, ro + k * n, do + m * n
+    L, R = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs))
+    D0 = np.concatenate(dst)
+    h = N.hip()
+    out = {}
+    try:
+        for cfg in (1, 6):
+            D = _buf(rt, D0)
+            h.tnt_set_tile_config(cfg)
+            arr = (N.GemmJob * len(jobs))(*jobs)
+            rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, 0.75, -1.5, prec))
+            out[cfg] = _get(rt, D, do)
+            _free(rt, D)
+    finally:
+        h.tnt_set_tile_config(-1)
+        _free(rt, L, R)
+    assert np.array_equal(out[1], out[6])
 
 
 def test_gemm_batch_rejects_out_of_bounds_before_mutating(rt):
