@@ -681,7 +681,10 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int fo
     const GJob& j = jobs[oi];
     if (j.m <= 0 || j.n <= 0) continue;
     int cfg = job_cfg(j, family);
-    if (ovr >= 0 && cfg != TILE_SKINNY) cfg = ovr;
+    // a forced shape leaves narrow outputs on the vector kernel, except that
+    // forced one-wave tiles also take narrow jobs of few rows (one launch for
+    // a batch of tiny GEMMs instead of two)
+    if (ovr >= 0 && (cfg != TILE_SKINNY || (ovr == TILE_SMALL && j.m <= 256))) cfg = ovr;
     if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
     int bm = tile_rows(cfg), bn = tile_cols(cfg);
     int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
