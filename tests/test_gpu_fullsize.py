@@ -1,0 +1,102 @@
+"""GPU properties of the benchmark workloads at their full sizes.
+
+The oracle is too slow at these sizes, so these tests check properties that
+hold at any size (the small-size parity against the oracle lives in
+test_gpu_workloads.py):
+
+* U(1) environment update, D = 4096: linear in the environment; split-BF16
+  within 1e-10 of FP64; an abelian permutation followed by its inverse gives
+  back the payload bit for bit.
+* SU(2) F-move contraction, multiplicity floor(64/(2j+1)) (2.74 GB per
+  operand): the contraction with both operands recoupled inside it equals
+  explicit F-move permutations followed by a canonical compose (the
+  reference's route invariance, contract_fusion.rs:532-700, at 1e-10 there);
+  a permutation followed by its inverse returns the payload within 1e-14.
+
+Norms are taken on the device (FP64 weighted inner products), so nothing of
+gigabyte size crosses PCIe.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tenet_amd.tensor import Runtime, TensorMap
+from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rt():
+    r = Runtime(0)
+    yield r
+    r.close()
+
+
+def dev_rel(a, b):
+    """||a - b|| / ||b|| on the device."""
+    return a.add(b, 1.0, -1.0).norm() / b.norm()
+
+
+@pytest.fixture(scope="module")
+def u1_env(rt):
+    return MPSEnvUpdate(rt, "U1", 4096, "f64")
+
+
+def test_u1_env_update_full_size_is_linear_in_environment(rt, u1_env):
+    wl = u1_env
+    L1 = wl.L
+    L2 = TensorMap.rand(rt, [wl.V, wl.M], [wl.V], seed=99, normal=True)
+    f1 = wl.run(L1).copy()
+    f2 = wl.run(L2).copy()
+    f12 = wl.run(L1.add(L2, 0.75, -1.25)).copy()
+    want = f1.add(f2, 0.75, -1.25)
+    err = dev_rel(f12, want)
+    assert err < 1e-13, err
+    assert math.isfinite(f1.norm()) and f1.norm() > 0
+
+
+def test_u1_env_update_full_size_split_bf16_within_1e10(rt, u1_env):
+    ref = u1_env.run().copy()
+    split = MPSEnvUpdate(rt, "U1", 4096, "split_bf16")
+    got = split.run()
+    assert np.array_equal(split.L.to_numpy(), u1_env.L.to_numpy())  # same seeded inputs
+    err = dev_rel(got, ref)
+    assert err < 1e-10, err
+
+
+def test_u1_permutation_roundtrip_full_size_bit_exact(rt, u1_env):
+    T = u1_env.L  # L[a, w; b], 5.97 M elements
+    P = T.permute([2, 0], [1])  # codomain (b, a), domain (w): crosses codomain / domain
+    Q = P.permute([1, 2], [0])
+    assert Q.space == T.space
+    assert np.array_equal(Q.to_numpy().view(np.uint64), T.to_numpy().view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def su2_fmove(rt):
+    return SU2FMoveContract(rt, mult=64, twoj_max=8, precision="f64")
+
+
+def test_su2_fmove_full_size_route_invariance(rt, su2_fmove):
+    wl = su2_fmove
+    c1 = wl.run().copy()
+    # explicit F-moves: A' = A(0, 2 <- 1, 3), B' = B(2, 0 <- 1, 3), then C = A' B'
+    Ap = wl.A.permute([0, 2], [1, 3])
+    Bp = wl.B.permute([2, 0], [1, 3])
+    c2 = Ap.compose(Bp)
+    assert c2.space == c1.space
+    del Ap, Bp
+    err = dev_rel(c2, c1)
+    assert err < 1e-12, err
+
+
+def test_su2_permutation_roundtrip_full_size(rt, su2_fmove):
+    A = su2_fmove.A
+    P = A.permute([0, 2], [1, 3])
+    Q = P.permute([0, 2], [1, 3])
+    assert Q.space == A.space
+    del P
+    err = dev_rel(Q, A)
+    assert err < 1e-14, err
