@@ -178,7 +178,7 @@ def run_serve(args):
     t = time.perf_counter()
     mpo = "heisenberg" if args.rule == "U1" else "random"
     srv = HeffServer(args.rule, n_streams=args.streams, max_inflight=args.inflight,
-                     precision=args.precision, mpo=mpo)
+                     precision=args.precision, mpo=mpo, graphs=args.serve_graphs == "on")
     srv.warm()
     cold = time.perf_counter() - t
     log(f"[bench] serve: {len(srv.buckets)} bond-dimension buckets {srv.buckets}, environments resident, "
@@ -192,6 +192,14 @@ def run_serve(args):
         f"(offered {stats['offered_rate']:.0f}/s): {stats['matvecs_per_s']:.0f} matvecs/s, "
         f"p50 {stats['p50_ms']:.3f} ms, p99 {stats['p99_ms']:.3f} ms, {stats['tflops']:.2f} TFLOP/s; "
         f"host issue {stats['host_issue_ms_per_request']:.3f} ms/request, peak in flight {stats['max_inflight']}")
+    # capacity: the same request mix offered far above what the server sustains
+    sat = None
+    if args.saturate > 0:
+        sat_trace = poisson_trace(args.requests, rate=args.saturate, seed=8, buckets=srv.buckets)
+        sat = srv.serve(sat_trace)
+        log(f"[bench] serve capacity: {sat['matvecs_per_s']:.0f} matvecs/s at offered {sat['offered_rate']:.0f}/s, "
+            f"{sat['tflops']:.2f} TFLOP/s, p50 {sat['p50_ms']:.3f} ms (queueing included), "
+            f"host issue {sat['host_issue_ms_per_request']:.3f} ms/request")
     cpu = None
     if args.cpu_baseline == "on":
         rate, n = cpu_serve_baseline(args.rule, trace)
@@ -228,6 +236,12 @@ def run_serve(args):
         "offered_rate": stats["offered_rate"],
         "tflops": stats["tflops"],
         "host_issue_ms_per_request": stats["host_issue_ms_per_request"],
+        "issue": "hipGraph replay of the four contractions per request" if args.serve_graphs == "on"
+                 else "prebuilt launch lists",
+        "capacity": None if sat is None else {
+            "matvecs_per_s": sat["matvecs_per_s"], "offered_rate": sat["offered_rate"], "tflops": sat["tflops"],
+            "latency_ms_incl_queueing": {"p50": sat["p50_ms"], "p99": sat["p99_ms"]},
+            "host_issue_ms_per_request": sat["host_issue_ms_per_request"]},
         "cpu_baseline": cpu,
         "cold_compile_s": cold,
     }
@@ -331,7 +345,8 @@ def main():
                     choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
                              "gemm_vs_vendor"])
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "split_bf16"])
+    ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16"],
+                    help="GEMM precision (default: split_bf16 for serve, f64 otherwise)")
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
@@ -340,11 +355,18 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: mean arrival rate (req/s)")
     ap.add_argument("--streams", type=int, default=4, help="serve: backend contexts (HIP streams)")
     ap.add_argument("--inflight", type=int, default=16, help="serve: request slots")
+    ap.add_argument("--serve-graphs", default="on", choices=["on", "off"],
+                    help="serve: issue each request's contractions as one hipGraph replay")
+    ap.add_argument("--saturate", type=float, default=64000.0,
+                    help="serve: arrival rate of a second trace that measures capacity (0: skip)")
     ap.add_argument("--graph", default="off", choices=["on", "off"],
                     help="replay the step from a hipGraph in the throughput region (env / F-move workloads)")
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
                     help="pick GEMM tile shapes per contraction by measurement before timing")
     args = ap.parse_args()
+    if args.precision is None:
+        # the serving configuration runs split-BF16 GEMMs with FP64 dot products
+        args.precision = "split_bf16" if args.workload == "serve" else "f64"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the contraction would shard by "

@@ -19,10 +19,14 @@ Device buffers: a request holds one buffer set (theta, the three H_eff
 intermediates, the result) of its (stream, bucket) pair from the moment it is
 admitted until its completion event fires; sets are reused by later requests
 on that pair, and the footprint admission counts the bytes of the sets in
-flight.  Issue replays prebuilt argument lists: one fill, four tnt_contract
-calls, one stream-ordered dot and one event record per request.  Every
-request computes on its own buffers with deterministic kernels, so its result
-is bit-for-bit the same alone or in any mix (test in tests/test_gpu_heff.py).
+flight.  The four tnt_contract calls of a buffer set are captured once, when
+the server warms up, as a hipGraph on its stream, so a request issues one fill
+(its seed), one graph launch, one stream-ordered dot (its result slot) and
+one event record (`graphs=False`, and sets built later while serving, replay
+the prebuilt argument lists of the four calls instead).  Every request
+computes on its own buffers with deterministic kernels, so its result is
+bit-for-bit the same alone or in any mix, and with or without graphs (tests
+in tests/test_gpu_heff.py).
 Latency = completion seen by the scheduler - arrival.
 """
 import ctypes as C
@@ -63,13 +67,15 @@ class _Req:
 class _BufSet:
     """One request's device buffers on one (stream, bucket) pair and the
     prebuilt tnt_contract argument tuples of the H_eff chain over them."""
-    __slots__ = ("theta", "scratch", "y", "calls", "bytes")
+    __slots__ = ("theta", "scratch", "y", "calls", "bytes", "graph")
 
 
 class HeffServer:
     def __init__(self, rule="U1", buckets=None, n_streams=4, max_inflight=16,
-                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7, mpo="random"):
+                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7, mpo="random",
+                 graphs=True):
         self.rule = rule
+        self.graphs = graphs
         self.precision = precision
         self.max_inflight = max_inflight
         self.budget = hbm_budget_bytes
@@ -90,7 +96,11 @@ class HeffServer:
 
     # -- per-bucket structures (cold: compiled per context once) -------------
     def warm(self):
-        """Compile every (bucket, stream) artifact and build one buffer set per pair."""
+        """Compile every (bucket, stream) artifact, build one buffer set per pair
+        and capture its graph.  The graphs are captured after every bucket has
+        run once on every stream: a stream's split-BF16 workspace grows (and is
+        reallocated) while larger buckets warm up, and a graph keeps the
+        addresses it was captured with."""
         for D in self.buckets:
             for k, rt in enumerate(self.rts):
                 bs = self._new_set(D, k)
@@ -98,6 +108,12 @@ class HeffServer:
                 self.fp[D] = bs.bytes
                 self._free.setdefault((k, D), []).append(bs)
                 rt.synchronize()
+        if self.graphs:
+            for (k, _), sets in self._free.items():
+                for bs in sets:
+                    if bs.graph is None:
+                        bs.graph = self._capture(k, bs.calls)
+                self.rts[k].synchronize()
 
     def _new_set(self, D, k):
         v = self._view(D, k)
@@ -107,12 +123,32 @@ class HeffServer:
         bs.y = v.apply(bs.theta, bs.scratch)
         bs.calls = v.launch_list(bs.theta, bs.scratch)
         bs.bytes = 8 * (bs.theta.space.len + sum(t.space.len for t in bs.scratch.values()))
+        bs.graph = None  # captured by warm(); sets built later, while serving, issue eagerly
         if D not in self._dot:
             secs = v.theta_space.sectors
             regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * c) for _, o, r, c, _ in secs])
             ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
             self._dot[D] = (regs, ws, len(secs))
         return bs
+
+    def _capture(self, k, calls):
+        """hipGraph of the four contractions of one buffer set on stream k (the
+        chain ran once eagerly on these buffers, so every table and workspace
+        it uses exists; the capture allocates and uploads nothing)."""
+        h = N.hip()
+        rt = self.rts[k]
+        ctx = rt._ctx
+        rt._check(h.tnt_graph_begin(ctx))
+        rc = 0
+        for art, d, a, b in calls:
+            rc = h.tnt_contract(ctx, art, d, a, b, 1.0, 0.0, self._prec)
+            if rc:
+                break
+        g = C.c_void_p()
+        rc_end = h.tnt_graph_end(ctx, C.byref(g))
+        rt._check(rc)
+        rt._check(rc_end)
+        return g
 
     def _issue(self, r, out_buf, keep):
         """Queue one request on its stream: fill theta, the four contractions,
@@ -124,8 +160,11 @@ class HeffServer:
         bs = free.pop() if free else self._new_set(D, k)
         ctx = rt._ctx
         rt._check(h.tnt_buf_fill_random(ctx, bs.theta._buf, r.seed & ((1 << 64) - 1), 1))
-        for art, d, a, b in bs.calls:
-            rt._check(h.tnt_contract(ctx, art, d, a, b, 1.0, 0.0, self._prec))
+        if bs.graph is not None:
+            rt._check(h.tnt_graph_launch(ctx, bs.graph))
+        else:
+            for art, d, a, b in bs.calls:
+                rt._check(h.tnt_contract(ctx, art, d, a, b, 1.0, 0.0, self._prec))
         regs, ws, ns = self._dot[D]
         rt._check(h.tnt_weighted_dot_async(ctx, bs.theta._buf, bs.y._buf, regs, ws, ns, out_buf._buf, r.rid))
         if keep:  # a copy of the result vector, read at completion (tests)
@@ -244,6 +283,11 @@ class HeffServer:
             for ev in evs:
                 h.tnt_event_destroy(self.rts[k]._ctx, ev)
         self._events = [[] for _ in self.rts]
+        for (k, _), sets in self._free.items():
+            for bs in sets:
+                if bs.graph is not None:
+                    h.tnt_graph_destroy(self.rts[k]._ctx, bs.graph)
+                    bs.graph = None
         self._free.clear()
         self.env.clear()
         self.views.clear()

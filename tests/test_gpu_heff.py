@@ -165,3 +165,26 @@ def test_serving_same_alone_and_in_mix():
         assert abs(mix[0] - want) <= 1e-12 * max(1.0, abs(want))
     finally:
         srv.close()
+
+
+def test_serving_graph_issue_matches_eager_issue():
+    """Requests issued as hipGraph replays (four contractions captured per
+    buffer set) give bit-for-bit the eager results; each graph is replayed
+    several times with eager fills, dots and copies between the replays."""
+    buckets = [32, 64]
+    trace = poisson_trace(12, rate=1e6, median_d=48, sigma=0.5, dmin=32, dmax=64, seed=9, buckets=buckets)
+    res = {}
+    for graphs in (False, True):
+        srv = HeffServer("U1", buckets=buckets, n_streams=2, max_inflight=4, precision="split_bf16",
+                         graphs=graphs)
+        try:
+            srv.warm()
+            if graphs:
+                assert all(bs.graph is not None for sets in srv._free.values() for bs in sets)
+            srv.serve(trace, keep_results=True)
+            res[graphs] = dict(srv.results)
+        finally:
+            srv.close()
+    for i in range(len(trace)):
+        assert res[True][i] == res[False][i], i
+        assert np.array_equal(res[True][(i, "y")], res[False][(i, "y")]), i
