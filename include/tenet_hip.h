@@ -273,8 +273,11 @@ int tnt_ctx_reset_stats(tnt_ctx* ctx);
 
 /* ---- tile-shape autotuner --------------------------------------------------- */
 /* Force one GEMM tile configuration for tables built after this call
- * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128, 4: narrow-output vector kernel;
- * -1: per-job choice).  Used by the autotuner sweep; no reference counterpart. */
+ * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128, 4: narrow-output vector kernel,
+ * 5: 128x128 on 8 waves, 6: 32x32 per wave with operands in registers (tiny
+ * jobs; also takes narrow jobs of <= 256 rows); -1: per-job choice).  Narrow
+ * outputs otherwise keep the vector kernel.  Used by the autotuner sweep; no
+ * reference counterpart. */
 int tnt_set_tile_config(int cfg);
 /* Install the tuned per-shape table: entries (m_max, n_max, cfg) scanned in
  * order, first match wins; n = 0 clears it. */
