@@ -308,6 +308,27 @@ def test_transposed_blocks_through_lds_tiles(rt, cod, dom, p1, p2):
     assert np.array_equal(out.to_numpy(), 2.0 * want + -0.5 * before)
 
 
+SMALLV = [(-1, 3), (0, 5), (1, 2)]
+
+
+@pytest.mark.parametrize("legs,cod,p1,p2", [(BIGV, 2, [0, 2], [1]), (BIGV, 1, [0, 2], [1]),
+                                            (SMALLV, 2, [0, 3], [2, 1]), (SMALLV, 2, [0, 2, 1], [3])])
+def test_single_class_flattened_and_table_items_accumulate(rt, legs, cod, p1, p2):
+    """Single-only U(1) permutations (one launch of the Single class: items of
+    one outer index as flattened records, items of several outer indices
+    through the block table), overwriting and accumulating: bit-exact."""
+    rank = 3 if legs is BIGV else 4
+    ps, os_ = spaces("U1", [(legs, False)] * cod, [(legs, False)] * (rank - cod))
+    t = TensorMap.rand(rt, ps, seed=31)
+    src = t.to_numpy()
+    want = OTensor(os_, src).permute(p1, p2).data
+    out = t.permute(p1, p2)
+    assert np.array_equal(out.to_numpy(), want)
+    before = out.to_numpy()
+    t.permute(p1, p2, out=out, alpha=2.0, beta=-0.5)
+    assert np.array_equal(out.to_numpy(), 2.0 * want + -0.5 * before)
+
+
 @pytest.mark.parametrize("small_plan_bytes", ["0", "1e12"])
 def test_multi_class_plans_per_class_and_in_one_launch(small_plan_bytes):
     """SU(2) permutations mixing Single, Multi and Scale items: replayed per
