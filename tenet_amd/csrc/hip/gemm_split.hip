@@ -204,13 +204,17 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
     for (int q = 0; q < 8; ++q) v[h][q] = ldexp(v[h][q], -e);
 #pragma unroll
     for (int p = 0; p < S; ++p) {
+      // digit d is an integer with |d| <= 256: int -> float -> x 2^-8p -> bf16 is
+      // exact at every step (a direct double -> bf16 conversion would be
+      // compiled as a round-to-odd sequence of eight instructions)
+      const float scale = __builtin_amdgcn_ldexpf(1.0f, -8 * p);
       bf16x8 o;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         v[h][q] *= 256.0;
         const double d = rint(v[h][q]);
         v[h][q] -= d;
-        o[q] = (__bf16)(float)ldexp(d, -8 * p);
+        o[q] = (__bf16)((float)(int)d * scale);
       }
       *reinterpret_cast<bf16x8*>(sd.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
     }
