@@ -57,7 +57,7 @@ constexpr int BK = 16;
 // Register budget: 4-wave tiles at most 256 VGPRs+AGPRs per lane, so two
 // waves share each SIMD (one wave per SIMD leaves LDS and global latency
 // exposed); 8-wave tiles at most 128, four waves per SIMD.
-template <typename T, int BM, int BN, int GM = 2, int GN = 2>
+template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false>
 __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 ? 4 : 2))) void gemm_kernel(
     GemmArgs g) {
   constexpr int NTH = 64 * GM * GN;
@@ -183,10 +183,24 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
     __syncthreads();
   }
 
-  // Epilogue: D = alpha * acc + beta * D  (D not read when beta == 0).
+  // Epilogue: D = alpha * acc (+ beta * D in the BETA instance, launched for
+  // beta != 0).  There the D values of one row of fragments are read
+  // unconditionally (indices clamped into the job) before any is stored, so
+  // the wave waits once per fragment row rather than once per element; the
+  // beta == 0 instance keeps its registers for the main loop.
   const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
+    double old[BETA ? NT : 1][4];
+    if constexpr (BETA) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int col = min(n0 + wn + j * 16 + (lane & 15), n - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          old[j][r] = D[min(m0 + wm + i * 16 + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * ldd];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       const int col = n0 + wn + j * 16 + (lane & 15);
@@ -195,9 +209,9 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
         if (row < m) {
-          const gptr<double> p = D + row + (int64_t)col * ldd;
           const double v = alpha * (double)acc[i][j][r];
-          *p = (beta != 0.0) ? v + beta * *p : v;
+          if constexpr (BETA) D[row + (int64_t)col * ldd] = v + beta * old[j][r];
+          else D[row + (int64_t)col * ldd] = v;
         }
       }
     }
@@ -242,13 +256,18 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
     }
   }
   if (i < m) {
+    // For beta != 0 the row of D is read (columns clamped) before any store.
     const double alpha = g.alpha, beta = g.beta;
+    double old[NMAX] = {};
+    if (beta != 0.0) {
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) old[j] = D[i + (int64_t)min(j, n - 1) * job.ldd];
+    }
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
       if (j < n) {
-        const gptr<double> p = D + i + (int64_t)j * job.ldd;
         const double v = alpha * acc[j];
-        *p = (beta != 0.0) ? v + beta * *p : v;
+        D[i + (int64_t)j * job.ldd] = (beta != 0.0) ? v + beta * old[j] : v;
       }
     }
   }
@@ -263,7 +282,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
 // to the p-th MFMA.  Loads are unconditional (indices clamped into the job's
 // operands); out-of-range k is zeroed by select, so the loop body is
 // straight-line code and each stage keeps its own registers.
-template <typename T>
+template <typename T, bool BETA = false>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_tiles) {
   constexpr int MT = 2, NT = 2;
   const int lane = threadIdx.x & 63, l16 = lane & 15, q4 = lane >> 4;
@@ -334,8 +353,23 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
     compute(S1{}, t + 1);
     __builtin_amdgcn_sched_barrier(0);
   }
+  // Epilogue.  The BETA instance (launched for beta != 0) reads the whole
+  // tile of D first, indices clamped into the job, so the wave waits once
+  // before the stores instead of once per element.
   const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
   const double alpha = g.alpha, beta = g.beta;
+  double old[BETA ? MT : 1][NT][4];
+  if constexpr (BETA) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int col = min(n0 + 16 * j + l16, n - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          old[i][j][r] = D[min(m0 + 16 * i + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * job.ldd];
+      }
+  }
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -346,12 +380,33 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + 16 * i + MfmaOp<T>::row(lane, r);
         if (row < m) {
-          const gptr<double> p = D + row + (int64_t)col * job.ldd;
           const double v = alpha * (double)acc[i][j][r];
-          *p = (beta != 0.0) ? v + beta * *p : v;
+          if constexpr (BETA) D[row + (int64_t)col * job.ldd] = v + beta * old[i][j][r];
+          else D[row + (int64_t)col * job.ldd] = v;
         }
       }
     }
+}
+
+template <typename T, bool BETA>
+static void launch_tiles(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  switch (cfg) {
+    case TILE_128x128:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_128x64:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 64, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_64x128:
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 128, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+    case TILE_128x128_W8:
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4, BETA>), dim3(n_tiles), dim3(512), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      break;
+  }
 }
 
 template <typename T>
@@ -361,23 +416,15 @@ static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t
     case TILE_SKINNY:
       hipLaunchKernelGGL((gemm_skinny_kernel<T>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
-    case TILE_128x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128>), dim3(n_tiles), dim3(256), 0, s, a);
-      break;
-    case TILE_128x64:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 64>), dim3(n_tiles), dim3(256), 0, s, a);
-      break;
-    case TILE_64x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 128>), dim3(n_tiles), dim3(256), 0, s, a);
-      break;
-    case TILE_128x128_W8:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4>), dim3(n_tiles), dim3(512), 0, s, a);
-      break;
     case TILE_SMALL:  // four independent one-wave tiles per workgroup
-      hipLaunchKernelGGL((gemm_small_kernel<T>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
+      if (a.beta != 0.0)
+        hipLaunchKernelGGL((gemm_small_kernel<T, true>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
+      else
+        hipLaunchKernelGGL((gemm_small_kernel<T, false>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
       break;
     default:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), dim3(n_tiles), dim3(256), 0, s, a);
+      if (a.beta != 0.0) launch_tiles<T, true>(a, cfg, n_tiles, s);
+      else launch_tiles<T, false>(a, cfg, n_tiles, s);
       break;
   }
 }

@@ -101,8 +101,10 @@ def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 5, 6])
 @pytest.mark.parametrize("lop,rop", [(0, 0), (1, 1)])
-def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop):
-    """Every tile configuration (forced) on ragged FP64 jobs, both operand ops."""
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.5, -1.25)])
+def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop, alpha, beta):
+    """Every tile configuration (forced) on ragged FP64 jobs, both operand
+    ops, overwriting (beta = 0) and accumulating into a nonzero D."""
     rng = np.random.default_rng(cfg)
     shapes = [(130, 70, 129), (300, 17, 65), (64, 64, 64), (257, 33, 200)]
     jobs, lhs, rhs, want = [], [], [], []
@@ -114,17 +116,18 @@ def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop):
         jobs.append(N.GemmJob(do, lo, ro, m, k, n))
         want.append((a @ b).reshape(-1, order="F"))
         lo, ro, do = lo + m * k, ro + k * n, do + m * n
-    L, R, D = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs)), _buf(rt, np.zeros(do))
+    d0 = rng.normal(size=do)
+    L, R, D = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs)), _buf(rt, d0)
     h = N.hip()
     try:
         h.tnt_set_tile_config(cfg)
         arr = (N.GemmJob * len(jobs))(*jobs)
-        rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, 1.0, 0.0, 0))
+        rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, alpha, beta, 0))
         got = _get(rt, D, do)
     finally:
         h.tnt_set_tile_config(-1)
         _free(rt, L, R, D)
-    w = np.concatenate(want)
+    w = alpha * np.concatenate(want) + beta * d0
     assert rel(got, w) < 1e-14
 
 
