@@ -29,7 +29,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"f64": 78.6, "f32": 157.3, "split_bf16": 2500.0}  # dense MFMA TFLOP/s
+PEAKS = {"f64": 78.6, "f32": 157.3, "split_bf16": 2500.0, "split_i8": 5000.0}  # dense MFMA TFLOP/s (TOP/s for int8)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -54,10 +54,12 @@ def roofline(precision, dom):
         rec.update(bound="hbm", unit="GB/s", achieved=gbs, peak=HBM_PEAK_GBS, frac=gbs / HBM_PEAK_GBS,
                    bytes_per_launch=dom["bytes"] / max(1, dom["launches"]), tflops=f64_eq)
         return rec
-    if precision == "split_bf16":
-        # bf16 MFMA work actually issued: SPLIT_PRODUCTS digit-plane products per FP64 product
+    if precision in ("split_bf16", "split_i8"):
+        # bf16 / int8 MFMA work actually issued: SPLIT_PRODUCTS digit-plane products per FP64 product
         ach = f64_eq * SPLIT_PRODUCTS
-        rec.update(achieved=ach, peak=PEAKS["split_bf16"], frac=ach / PEAKS["split_bf16"], f64_equivalent=f64_eq)
+        rec.update(achieved=ach, peak=PEAKS[precision], frac=ach / PEAKS[precision], f64_equivalent=f64_eq)
+        if precision == "split_i8":
+            rec["unit"] = "TOP/s"
     else:
         rec.update(achieved=f64_eq, peak=PEAKS[precision], frac=f64_eq / PEAKS[precision])
     return rec
@@ -224,7 +226,8 @@ def run_serve(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)"}[args.precision]
+        "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)",
+                  "split_i8": "split-int8 (f64 in/out)"}[args.precision]
                  + ", f64 dot",
         "data": "synthetic (seeded normal(0,1) environments and states on device; "
                 + ("spin-1/2 Heisenberg MPO)" if mpo == "heisenberg" else "seeded normal(0,1) MPO tensors)"),
@@ -345,7 +348,7 @@ def main():
                     choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
                              "gemm_vs_vendor"])
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
-    ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16"],
+    ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16", "split_i8"],
                     help="GEMM precision (default: split_bf16 for serve, f64 otherwise)")
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
@@ -509,7 +512,7 @@ def main():
     ms_events = (time.perf_counter() - t2) * 1e3 / args.steps
     rt._check(h.tnt_ctx_set_timing(rt._ctx, 0))
     value = wl.flops / (ms * 1e-3) / 1e12
-    prec_id = {"f64": 0, "f32": 1, "split_bf16": 2}[args.precision]
+    prec_id = {"f64": 0, "f32": 1, "split_bf16": 2, "split_i8": 3}[args.precision]
     n, tms, work = C.c_size_t(), C.c_double(), C.c_double()
 
     def stats(kind):
@@ -525,8 +528,10 @@ def main():
         # not bracketed per kernel, so the step time (device-synchronised) stands in
         dom = {"kernel": f"grouped GEMM hipGraph replay ({args.precision})", "launches": args.steps,
                "ms": ms * args.steps, "flops": wl.flops * args.steps, "bytes": wl.bytes * args.steps}
-    elif args.precision == "split_bf16":
-        dom = {"kernel": "split-BF16 table run (digit pre-pass + bf16 MFMA GEMM; FP64 kernels for narrow jobs)",
+    elif args.precision in ("split_bf16", "split_i8"):
+        kind = "split-BF16 table run (digit pre-pass + bf16 MFMA GEMM" if args.precision == "split_bf16" else \
+            "split-INT8 table run (exponent + digit pre-passes + int8 MFMA GEMM"
+        dom = {"kernel": kind + "; FP64 kernels for narrow jobs)",
                "launches": gemm_launches, "ms": gemm_ms, "flops": gemm_work}
     else:
         per = [(stats(8 + 8 * prec_id + c), c) for c in range(len(GEMM_KERNELS))]
@@ -581,7 +586,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)"}[args.precision],
+        "dtype": {"f64": "f64", "f32": "f32 (f64 in/out)", "split_bf16": "split-bf16 (f64 in/out)",
+                  "split_i8": "split-int8 (f64 in/out)"}[args.precision],
         "data": "synthetic (seeded normal(0,1) on device)",
         "config": cfg,
         "roofline": roofline(args.precision, dom),

@@ -79,7 +79,8 @@ enum tnt_op { TNT_OP_IDENTITY = 0, TNT_OP_ADJOINT = 1 };
 enum tnt_precision {
   TNT_PREC_F64 = 0,        /* FP64 MFMA, FP64 accumulate */
   TNT_PREC_F32 = 1,        /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
-  TNT_PREC_SPLIT_BF16 = 2  /* Ozaki split into BF16 slices, BF16 MFMA, exact FP32 chunk sums, FP64 accumulate */
+  TNT_PREC_SPLIT_BF16 = 2, /* Ozaki split into BF16 slices, BF16 MFMA, exact FP32 chunk sums, FP64 accumulate */
+  TNT_PREC_SPLIT_I8 = 3    /* Ozaki split into INT8 slices, INT8 MFMA, exact INT32 level sums, FP64 accumulate */
 };
 
 typedef struct tnt_ctx tnt_ctx;
@@ -283,9 +284,10 @@ int tnt_set_tile_config(int cfg);
  * order, first match wins; n = 0 clears it. */
 int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n);
 
-/* Number of BF16 digit planes per operand of TNT_PREC_SPLIT_BF16 (2..6,
- * default 5: relative error ~1e-12 on random data; 4 trades accuracy for
- * speed).  No reference counterpart. */
+/* Number of digit planes per operand of TNT_PREC_SPLIT_BF16 and
+ * TNT_PREC_SPLIT_I8 (2..6, default 5: relative error ~1e-12 (BF16) and
+ * ~1e-11 (INT8) on random data; 4 trades accuracy for speed).  No reference
+ * counterpart. */
 int tnt_set_split_slices(int slices);
 
 /* ---- hipGraph batching ----------------------------------------------------- */

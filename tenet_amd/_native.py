@@ -17,7 +17,7 @@ E_INVALID, E_UNSUPPORTED, E_SHAPE, E_OOB, E_DEVICE_MISMATCH, E_HIP, E_OOM = -1, 
 ERRORS = {-1: "INVALID", -2: "UNSUPPORTED", -3: "SHAPE", -4: "OOB", -5: "DEVICE_MISMATCH",
           -6: "HIP", -7: "OOM"}
 
-PREC_F64, PREC_F32, PREC_SPLIT_BF16 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_SPLIT_BF16, PREC_SPLIT_I8 = 0, 1, 2, 3
 OP_IDENTITY, OP_ADJOINT = 0, 1
 RULES = {"U1": 0, "SU2": 1, "Z2": 2, "U1xZ2": 3}
 

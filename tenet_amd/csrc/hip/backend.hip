@@ -731,7 +731,7 @@ static double table_flops(const GemmTable& t) {
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
                           double alpha, double beta, int precision) {
   TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
-  if (precision == TNT_PREC_SPLIT_BF16) {
+  if (precision == TNT_PREC_SPLIT_BF16 || precision == TNT_PREC_SPLIT_I8) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
       if (!t.n_narrow_cfg[c]) continue;
       const uint32_t first = t.tile_start_cfg[c] + t.n_tiles_cfg[c] - t.n_narrow_cfg[c];
@@ -740,7 +740,8 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
       launch_gemm_f64(g, c, t.n_narrow_cfg[c], ctx->stream);
     }
     if (!ctx->split) ctx->split.reset(split_workspace_create());
-    int rc = split_gemm_run(ctx->split.get(), t.split_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
+    const int scheme = precision == TNT_PREC_SPLIT_I8 ? SPLIT_SCHEME_I8 : SPLIT_SCHEME_BF16;
+    int rc = split_gemm_run(ctx->split.get(), scheme, t.split_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
                             alpha, beta, ctx->stream, ctx->capturing, ctx->err);
     if (rc) return rc;
     HIPCHK(ctx, hipGetLastError());
@@ -950,7 +951,7 @@ int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf
   if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
   if ((lhs_op != TNT_OP_IDENTITY && lhs_op != TNT_OP_ADJOINT) || (rhs_op != TNT_OP_IDENTITY && rhs_op != TNT_OP_ADJOINT))
     FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
   std::vector<GJob> gj(n_jobs);
   for (size_t i = 0; i < n_jobs; ++i) {
     const tnt_gemm_job& j = jobs[i];
@@ -987,7 +988,7 @@ int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, 
                      double beta, int precision) {
   if (!ctx) return TNT_E_INVALID;
   if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
   std::vector<GJob> gj(n_jobs);
   for (size_t i = 0; i < n_jobs; ++i) {
     const tnt_gemm_ptr_job& j = jobs[i];
@@ -1202,7 +1203,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   if (int rc = check_buf(ctx, lhs, "lhs")) return rc;
   if (int rc = check_buf(ctx, rhs, "rhs")) return rc;
   if (a->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, "artifact uploaded to another device");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_BF16) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
   int64_t need_l = a->lhs ? a->lhs->src_len : a->lhs_len;
   int64_t need_r = a->rhs ? a->rhs->src_len : a->rhs_len;
   int64_t need_d = a->out ? a->out->dst_len : a->core_len;

@@ -45,15 +45,43 @@ namespace tnt_dev {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int FRAG = 512;         // bf16 per fragment (16 lines x 32 k)
+constexpr int FRAG_BYTES = 1024;  // one fragment of either scheme: 16 bytes per lane
 constexpr int KSTEP = 32;         // k per fragment / MFMA step
 constexpr int WINDOW_STEPS = 8;   // 256 k per exact FP32 window
+
+// INT8 scheme: 16 lines x 64 k per fragment, pre-pass items of 512 k, INT32
+// level sums exact over windows of 256 steps (16384 k; the bound is 26296).
+constexpr int I8_KSTEP = 64;
+constexpr int I8_KCHUNK = 512;
+constexpr int I8_WINDOW_STEPS = 256;
+
+// The two digit schemes as seen by the GEMM's inner loop: fragment and
+// accumulator types, the number of level accumulator groups, the MFMA.
+struct Bf16Scheme {
+  typedef bf16x8 frag_t;
+  typedef f32x4 acc_t;
+  static constexpr int nacc(int S) { return S >= 6 ? 4 : 3; }
+  static __device__ __forceinline__ acc_t mfma(frag_t a, frag_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+struct I8Scheme {
+  typedef i32x4 frag_t;
+  typedef i32x4 acc_t;
+  static constexpr int nacc(int S) { return S; }  // one exact INT32 group per level
+  static __device__ __forceinline__ acc_t mfma(frag_t a, frag_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+  }
+};
 
 struct SplitJob {
   int64_t d_off;     // element offset of C from dbase
   int64_t fa0, fb0;  // first fragment of this job's A / B planes
   int64_t ea0, eb0;  // first exponent of this job's A rows / B columns (chunk-major: ea0 + c * m + i)
+  int64_t ra0, rb0;  // INT8 scheme: first whole-K line exponent of this job's A rows / B columns
   int32_t m, n, ns;  // ns = number of 32-k steps
   int32_t ldd;
   int32_t lba, lbb;  // line blocks of 16 (A rows, B columns)
@@ -74,7 +102,7 @@ struct SplitTables {
   int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid
   uint32_t tile_start[SPLIT_SHAPES] = {}, tile_count[SPLIT_SHAPES] = {};
   uint32_t items_a = 0, items_b = 0;
-  int64_t frags_a = 0, frags_b = 0, a_exps = 0, b_exps = 0;
+  int64_t frags_a = 0, frags_b = 0, a_exps = 0, b_exps = 0, a_rows = 0, b_rows = 0;
 };
 
 struct SplitWorkspace {
@@ -105,8 +133,9 @@ constexpr int EXP_NONE = -4096;                // exponent of an all-zero line c
 struct PrepSide {
   uintptr_t base;          // operand base
   int32_t* exps;           // line x chunk exponents of this side
-  __bf16* planes;          // fragments of this side
+  void* planes;            // fragments of this side
   const int64_t* prefix;   // n_jobs + 1 entries: first item of each job
+  int32_t* rows;           // INT8 scheme: whole-K line exponents of this side
 };
 
 struct PrepArgs {
@@ -216,7 +245,143 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
         v[h][q] -= d;
         o[q] = (__bf16)((float)(int)d * scale);
       }
-      *reinterpret_cast<bf16x8*>(sd.planes + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
+      *reinterpret_cast<bf16x8*>(static_cast<__bf16*>(sd.planes) + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// INT8 scheme pre-pass.  Items as above with 512-k chunks; wave w handles the
+// chunk's 64-k steps w and w + 4, lane l line l % 16 and k = 16 (l / 16) .. + 15
+// of a step (the fragment order: 16 bytes per lane and plane).
+//   1. split_i8_chunk_exp: per (line, chunk) the least e with max |A| < 2^e
+//      126.5 / 128 (frexp's exponent, plus one for a maximum in the top 1.2 %
+//      of its binade), EXP_NONE for an all-zero chunk (chunk-major, as the
+//      BF16 exponents);
+//   2. split_i8_digits: the line's exponent over all of K, e = max over its
+//      chunks (0 for an all-zero line), so |a| < 126.5 for a = A 2^(7 - e);
+//      a is cut into d_0 = rint(a) in [-126, 126] and S - 1 base-256 digits by
+//      repeated round-to-nearest (in [-128, 128]); a digit that reaches 128
+//      becomes 128 - 256 and carries one into the digit above, from the last
+//      digit up, so every digit fits INT8 (d_0 <= 127) and
+//        A = 2^(e - 7) sum_p d_p 2^(-8 p)   (truncated after S digits).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int line_count(const GJob& g, int side) { return side == 0 ? g.m : g.n; }
+
+__global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
+  __shared__ double red[4][4][16];
+  const PrepItem it = prep_item(x);
+  const GJob g = x.jobs[it.job];
+  const SplitJob s = x.sj[it.job];
+  const PrepSide& sd = x.side[it.side];
+  const int lines = line_count(g, it.side);
+  const int K = g.k;
+  const gptr<const double> M = gmem<const double>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int line = it.lb * 16 + li;
+  double mx = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int kk = st * I8_KSTEP + kg * 16 + q;
+      v[q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[q]));
+  }
+  red[wave][kg][li] = mx;
+  __syncthreads();
+  if (t < 16 && it.lb * 16 + t < lines) {
+    double m2 = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[w][c][t]);
+    int e = EXP_NONE;
+    if (m2 > 0.0) {
+      frexp(m2, &e);                          // m2 = f 2^e, f in [0.5, 1)
+      if (ldexp(m2, 7 - e) >= 126.5) ++e;     // keeps rint(A 2^(7-e)) <= 126
+    }
+    sd.exps[(it.side == 0 ? s.ea0 : s.eb0) + (int64_t)it.kc * lines + it.lb * 16 + t] = e;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
+  static_assert(S >= 2 && S <= 6, "digit count");
+  __shared__ int sexp[16];
+  const PrepItem it = prep_item(x);
+  const GJob g = x.jobs[it.job];
+  const SplitJob s = x.sj[it.job];
+  const PrepSide& sd = x.side[it.side];
+  const int lines = line_count(g, it.side);
+  const int K = g.k;
+  const gptr<const double> M = gmem<const double>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int line = it.lb * 16 + li;
+  if (t < 16) {
+    const int l = it.lb * 16 + t;
+    int e = EXP_NONE;
+    if (l < lines) {
+      const int32_t* ce = sd.exps + (it.side == 0 ? s.ea0 : s.eb0) + l;
+      for (int c = 0; c < s.nkc; ++c) e = max(e, ce[(int64_t)c * lines]);
+    }
+    if (e == EXP_NONE) e = 0;  // all-zero line: its digits are zero
+    sexp[t] = e;
+    if (it.kc == 0 && l < lines) sd.rows[(it.side == 0 ? s.ra0 : s.rb0) + l] = e;
+  }
+  __syncthreads();
+  const int e = sexp[li];
+  const int64_t f0 = (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)it.lb * s.ns * S;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+    if (st >= s.ns) continue;
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int kk = st * I8_KSTEP + kg * 16 + q;
+      v[q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
+    }
+    uint32_t packed[S][4];
+#pragma unroll
+    for (int p = 0; p < S; ++p)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      double a = ldexp(v[q], 7 - e);  // |a| < 126.5
+      int d[S];
+      double r = rint(a);
+      d[0] = (int)r;
+      a -= r;                         // |a| <= 1/2, exact
+#pragma unroll
+      for (int p = 1; p < S; ++p) {
+        a *= 256.0;
+        r = rint(a);
+        d[p] = (int)r;                // in [-128, 128]
+        a -= r;
+      }
+      int carry = 0;
+#pragma unroll
+      for (int p = S - 1; p >= 1; --p) {
+        d[p] += carry;
+        carry = d[p] > 127 ? 1 : 0;
+        d[p] -= carry << 8;
+      }
+      d[0] += carry;
+#pragma unroll
+      for (int p = 0; p < S; ++p) packed[p][q >> 2] |= (uint32_t)(d[p] & 0xff) << (8 * (q & 3));
+    }
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      const i32x4 o{(int)packed[p][0], (int)packed[p][1], (int)packed[p][2], (int)packed[p][3]};
+      *reinterpret_cast<i32x4*>(static_cast<char*>(sd.planes) + (f0 + (int64_t)st * S + p) * FRAG_BYTES + lane * 16) = o;
     }
   }
 }
@@ -228,9 +393,9 @@ struct SplitGemmArgs {
   const SplitJob* sj;
   const GTile* tiles;
   uintptr_t dbase;
-  const __bf16* a_planes;
-  const __bf16* b_planes;
-  const int32_t* a_exp;
+  const void* a_planes;
+  const void* b_planes;
+  const int32_t* a_exp;  // BF16: line x chunk exponents; INT8: whole-K line exponents
   const int32_t* b_exp;
   double alpha, beta;
 };
@@ -270,9 +435,11 @@ __device__ __forceinline__ void wait_vmcnt() {
 
 // Workgroup = GM x GN waves (wave w at row w % GM, column w / GM), each
 // owning (16 WM) x (16 WN) of the output tile.
-template <int S, int WM, int WN, int GM = 2, int GN = 2>
+template <class Sch, int S, int WM, int WN, int GM = 2, int GN = 2>
 struct SplitKernel {
-  static constexpr int NACC = S >= 6 ? 4 : 3;
+  typedef typename Sch::frag_t frag_t;
+  typedef typename Sch::acc_t acc_t;
+  static constexpr int NACC = Sch::nacc(S);
   static constexpr int NW = GM * GN;                   // waves per workgroup
   static constexpr int LBM = GM * WM, LBN = GN * WN;   // line blocks per workgroup tile
   static constexpr int FA = LBM * S, FB = LBN * S;     // fragments per stage
@@ -305,7 +472,7 @@ struct SplitKernel {
         frag = job.fb0 + (int64_t)lb * job.ns * S + q;
         planes = (uintptr_t)g.b_planes;
       }
-      c.src[r] = uniform64(planes + (uint64_t)frag * (2 * FRAG));
+      c.src[r] = uniform64(planes + (uint64_t)frag * FRAG_BYTES);
     }
   }
 
@@ -313,25 +480,25 @@ struct SplitKernel {
   // into the stage buffers at LDS addresses as_lds (A) and bs_lds (B).
   static __device__ __forceinline__ void load_stage(const Copies& c, int step, uint32_t as_lds, uint32_t bs_lds,
                                                     int wave, uint32_t voff) {
-    const uint64_t step_bytes = (uint64_t)step * (S * 2 * FRAG);
+    const uint64_t step_bytes = (uint64_t)step * (S * FRAG_BYTES);
 #pragma unroll
     for (int r = 0; r < PER_WAVE; ++r) {
       const int f = wave + NW * r;
-      const uint32_t lds = r < FA / NW ? as_lds + f * (2 * FRAG) : bs_lds + (f - FA) * (2 * FRAG);
+      const uint32_t lds = r < FA / NW ? as_lds + f * FRAG_BYTES : bs_lds + (f - FA) * FRAG_BYTES;
       copy_to_lds(c.src[r] + step_bytes, voff, lds);
     }
   }
 
-  static __device__ __forceinline__ void compute(bf16x8 (&As)[FA][64], bf16x8 (&Bs)[FB][64], int wm, int wn,
-                                                 int lane, f32x4 (&acc)[NACC][WM][WN]) {
-    bf16x8 bq[S][WN];
+  static __device__ __forceinline__ void compute(frag_t (&As)[FA][64], frag_t (&Bs)[FB][64], int wm, int wn,
+                                                 int lane, acc_t (&acc)[NACC][WM][WN]) {
+    frag_t bq[S][WN];
 #pragma unroll
     for (int q = 0; q < S; ++q)
 #pragma unroll
       for (int j = 0; j < WN; ++j) bq[q][j] = Bs[(wn * WN + j) * S + q][lane];
 #pragma unroll
     for (int p = 0; p < S; ++p) {
-      bf16x8 ap[WM];
+      frag_t ap[WM];
 #pragma unroll
       for (int i = 0; i < WM; ++i) ap[i] = As[(wm * WM + i) * S + p][lane];
 #pragma unroll
@@ -342,23 +509,23 @@ struct SplitKernel {
         for (int i = 0; i < WM; ++i)
 #pragma unroll
           for (int j = 0; j < WN; ++j)
-            acc[grp][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i], bq[q][j], acc[grp][i][j], 0, 0, 0);
+            acc[grp][i][j] = Sch::mfma(ap[i], bq[q][j], acc[grp][i][j]);
       }
     }
   }
 
-  static __device__ __forceinline__ void zero(f32x4 (&acc)[NACC][WM][WN]) {
+  static __device__ __forceinline__ void zero(acc_t (&acc)[NACC][WM][WN]) {
 #pragma unroll
     for (int l = 0; l < NACC; ++l)
 #pragma unroll
       for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < WN; ++j) acc[l][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < WN; ++j) acc[l][i][j] = acc_t{0, 0, 0, 0};
   }
 
   // acc64 += 2^(e_row + f_col - 16) (sum of the window's groups); ea/eb are the
   // window's exponents of this lane's rows and columns.
-  static __device__ __forceinline__ void flush(f32x4 (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4],
+  static __device__ __forceinline__ void flush(acc_t (&acc)[NACC][WM][WN], double (&acc64)[WM][WN][4],
                                                const int (&ea)[WM][4], const int (&eb)[WN]) {
 #pragma unroll
     for (int i = 0; i < WM; ++i)
@@ -375,9 +542,9 @@ struct SplitKernel {
 
   // One k step: prefetch the next stage into (An, Bn), wait for (Ac, Bc), compute.
   static __device__ __forceinline__ void step(const Copies& c, int ns, int s,
-                                              bf16x8 (&Ac)[FA][64], bf16x8 (&Bc)[FB][64],
+                                              frag_t (&Ac)[FA][64], frag_t (&Bc)[FB][64],
                                               uint32_t an_lds, uint32_t bn_lds, int wave, int lane,
-                                              f32x4 (&acc)[NACC][WM][WN]) {
+                                              acc_t (&acc)[NACC][WM][WN]) {
     if (s + 1 < ns) {
       load_stage(c, s + 1, an_lds, bn_lds, wave, lane * 16);
       wait_vmcnt<PER_WAVE>();
@@ -392,7 +559,7 @@ struct SplitKernel {
 
 template <int S, int WM, int WN, int GM = 2, int GN = 2>
 __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs g) {
-  using K = SplitKernel<S, WM, WN, GM, GN>;
+  using K = SplitKernel<Bf16Scheme, S, WM, WN, GM, GN>;
   __shared__ bf16x8 A0[K::FA][64];
   __shared__ bf16x8 B0[K::FB][64];
   __shared__ bf16x8 A1[K::FA][64];
@@ -466,6 +633,76 @@ __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs 
   }
 }
 
+// INT8 scheme GEMM: the stage pipeline of split_gemm_kernel over 64-k INT8
+// fragments (v_mfma_i32_16x16x64_i8).  Level l = p + q of the digit products
+// has its own INT32 accumulator group; over a window of at most 16384 k every
+// partial sum is bounded by 81664 k < 2^31 (|d_0| <= 127, |d_p| <= 128), so
+// the sums are exact.  At the end of a window
+//   C_ij += 2^(e_i + f_j - 14) sum_l 2^(-8 l) L_l
+// in FP64 (e_i, f_j: whole-K line exponents); a window after the first adds
+// into the D elements that the same lane wrote for the first.
+template <int S, int WM, int WN, int GM = 2, int GN = 2>
+__global__ __launch_bounds__(64 * GM * GN) void split_i8_gemm_kernel(SplitGemmArgs g) {
+  using K = SplitKernel<I8Scheme, S, WM, WN, GM, GN>;
+  __shared__ i32x4 A0[K::FA][64];
+  __shared__ i32x4 B0[K::FB][64];
+  __shared__ i32x4 A1[K::FA][64];
+  __shared__ i32x4 B1[K::FB][64];
+  const GTile tile = g.tiles[blockIdx.x];
+  const SplitJob job = g.sj[tile.job];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave % GM, wn = wave / GM;
+
+  i32x4 acc[S][WM][WN];
+  typename K::Copies cp;
+  K::setup(g, job, tile.tm, tile.tn, wave, cp);
+  const int ns = job.ns;
+  const uint32_t a0 = lds_address(&A0[0][0]), b0 = lds_address(&B0[0][0]);
+  const uint32_t a1 = lds_address(&A1[0][0]), b1 = lds_address(&B1[0][0]);
+  K::load_stage(cp, 0, a0, b0, wave, lane * 16);
+  const int m0 = tile.tm * K::LBM * 16 + wm * WM * 16, n0 = tile.tn * K::LBN * 16 + wn * WN * 16;
+  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
+  for (int w0 = 0; w0 < ns; w0 += I8_WINDOW_STEPS) {  // even window length: each starts on buffer 0
+    const int w1 = min(ns, w0 + I8_WINDOW_STEPS);
+    K::zero(acc);
+    for (int s = w0; s < w1; s += 2) {
+      K::step(cp, ns, s, A0, B0, a1, b1, wave, lane, acc);
+      if (s + 1 < w1) K::step(cp, ns, s + 1, A1, B1, a0, b0, wave, lane, acc);
+    }
+    // line exponents (indices clamped into the job; rows and columns past it
+    // are not stored)
+    int ea[WM][4], eb[WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ea[i][r] = g.a_exp[job.ra0 + min(m0 + i * 16 + (lane >> 4) * 4 + r, job.m - 1)];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) eb[j] = g.b_exp[job.rb0 + min(n0 + j * 16 + (lane & 15), job.n - 1)];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int col = n0 + j * 16 + (lane & 15);
+      if (col >= job.n) continue;
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + i * 16 + (lane >> 4) * 4 + r;
+          if (row < job.m) {
+            double sum = 0.0;
+#pragma unroll
+            for (int l = S - 1; l >= 0; --l) sum += ldexp((double)acc[l][i][j][r], -8 * l);
+            const double v = g.alpha * ldexp(sum, ea[i][r] + eb[j] - 14);
+            const gptr<double> p = D + row + (int64_t)col * job.ldd;
+            if (w0 == 0) *p = (g.beta != 0.0) ? v + g.beta * *p : v;
+            else *p += v;
+          }
+        }
+      }
+    }
+  }
+}
+
 static int split_slices_setting = 5;
 
 int split_set_slices(int slices) {
@@ -497,6 +734,31 @@ static void launch_digits_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
   if (items) hipLaunchKernelGGL(split_digits<S>, dim3(items), dim3(256), 0, s, pa);
 }
 
+template <int S>
+static void launch_split_i8_s(const SplitTables& t, const SplitGemmArgs& a0, hipStream_t s) {
+  SplitGemmArgs a = a0;
+  for (int c = 0; c < SPLIT_SHAPES; ++c) {
+    if (!t.tile_count[c]) continue;
+    a.tiles = t.tiles + t.tile_start[c];
+    const dim3 grid(t.tile_count[c]);
+    if constexpr (S >= 6) {  // the wide shapes would spill at six planes
+      hipLaunchKernelGGL((split_i8_gemm_kernel<S, 2, 2>), grid, dim3(256), 0, s, a);
+    } else {
+      if (c == SPLIT_64x64) hipLaunchKernelGGL((split_i8_gemm_kernel<S, 2, 2>), grid, dim3(256), 0, s, a);
+      else if (c == SPLIT_128x64) hipLaunchKernelGGL((split_i8_gemm_kernel<S, 4, 2>), grid, dim3(256), 0, s, a);
+      else if (c == SPLIT_64x128) hipLaunchKernelGGL((split_i8_gemm_kernel<S, 2, 4>), grid, dim3(256), 0, s, a);
+      else hipLaunchKernelGGL((split_i8_gemm_kernel<S, 4, 2, 2, 4>), grid, dim3(512), 0, s, a);
+    }
+  }
+}
+
+template <int S>
+static void launch_digits_i8_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
+  if (!items) return;
+  hipLaunchKernelGGL(split_i8_chunk_exp, dim3(items), dim3(256), 0, s, pa);
+  hipLaunchKernelGGL(split_i8_digits<S>, dim3(items), dim3(256), 0, s, pa);
+}
+
 #define SPLIT_HIP(expr)                                                    \
   do {                                                                     \
     hipError_t _e = (expr);                                                \
@@ -517,18 +779,21 @@ static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
   return SPLIT_64x64;
 }
 
-int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
+int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
                    const std::vector<GTile>& /*host_tiles*/, const GTile* /*dev_tiles*/, uintptr_t d,
                    uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s, bool capturing,
                    std::string& err) {
   const int S = split_slices_setting;
   if (host_jobs.empty()) return TNT_OK;
+  const bool i8 = scheme == SPLIT_SCHEME_I8;
+  const int kstep = i8 ? I8_KSTEP : KSTEP, kchunk = i8 ? I8_KCHUNK : KCHUNK;
   std::string key(reinterpret_cast<const char*>(host_jobs.data()), host_jobs.size() * sizeof(GJob));
   key.push_back((char)S);
+  key.push_back((char)scheme);
   auto it = ws->tables.find(key);
   if (it == ws->tables.end()) {
     if (capturing) {
-      err = "split-BF16 tables are built on first use; run the call once before capturing";
+      err = "split GEMM tables are built on first use; run the call once before capturing";
       return TNT_E_INVALID;
     }
     SplitTables t;
@@ -543,7 +808,7 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       x.d_off = j.d_off;
       x.m = j.m;
       x.n = j.n;
-      x.ns = (std::max(j.k, 1) + KSTEP - 1) / KSTEP;
+      x.ns = (std::max(j.k, 1) + kstep - 1) / kstep;
       x.ldd = j.ldd;
       x.lba = (j.m + 15) / 16;
       x.lbb = (j.n + 15) / 16;
@@ -551,11 +816,17 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       x.fb0 = t.frags_b;
       t.frags_a += (int64_t)x.lba * x.ns * S;
       t.frags_b += (int64_t)x.lbb * x.ns * S;
-      x.nkc = (std::max(j.k, 1) + KCHUNK - 1) / KCHUNK;
+      x.nkc = (std::max(j.k, 1) + kchunk - 1) / kchunk;
       x.ea0 = t.a_exps;
       x.eb0 = t.b_exps;
       t.a_exps += (int64_t)j.m * x.nkc;
       t.b_exps += (int64_t)j.n * x.nkc;
+      x.ra0 = t.a_rows;
+      x.rb0 = t.b_rows;
+      if (i8) {
+        t.a_rows += j.m;
+        t.b_rows += j.n;
+      }
       x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
       // operands of a job with an empty output are never read: no pre-pass items
       const int64_t ia = (j.n > 0) ? (int64_t)x.lba * x.nkc : 0, ib = (j.m > 0) ? (int64_t)x.lbb * x.nkc : 0;
@@ -574,7 +845,7 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
       tiles.insert(tiles.end(), per[c].begin(), per[c].end());
     }
     if (pre_a.back() + pre_b.back() > (int64_t)UINT32_MAX) {
-      err = "split-BF16 pre-pass grid too large";
+      err = "split pre-pass grid too large";
       return TNT_E_UNSUPPORTED;
     }
     t.items_a = (uint32_t)pre_a.back();
@@ -599,11 +870,13 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     it = ws->tables.emplace(key, t).first;
   }
   const SplitTables& t = it->second;
-  // workspace: A fragments | B fragments (bf16) | A exps | B exps (int32, line x chunk)
-  const size_t bytes = 2 * (size_t)FRAG * (size_t)(t.frags_a + t.frags_b) + 4 * (size_t)(t.a_exps + t.b_exps) + 256;
+  // workspace: A fragments | B fragments (1 KiB each) | A exps | B exps (int32,
+  // line x chunk) | A line exps | B line exps (int32, INT8 scheme)
+  const size_t bytes = (size_t)FRAG_BYTES * (size_t)(t.frags_a + t.frags_b) +
+                       4 * (size_t)(t.a_exps + t.b_exps + t.a_rows + t.b_rows) + 256;
   if (bytes > ws->bytes) {
     if (capturing) {
-      err = "split-BF16 workspace must be sized before graph capture; run the call once first";
+      err = "split GEMM workspace must be sized before graph capture; run the call once first";
       return TNT_E_INVALID;
     }
     SPLIT_HIP(hipStreamSynchronize(s));
@@ -613,14 +886,28 @@ int split_gemm_run(SplitWorkspace* ws, const std::vector<GJob>& host_jobs, const
     SPLIT_HIP(hipMalloc(&ws->planes, bytes));
     ws->bytes = bytes;
   }
-  __bf16* ap = static_cast<__bf16*>(ws->planes);
-  __bf16* bp = ap + (size_t)FRAG * t.frags_a;
-  int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)FRAG * t.frags_b);
+  char* ap = static_cast<char*>(ws->planes);
+  char* bp = ap + (size_t)FRAG_BYTES * t.frags_a;
+  int32_t* ae = reinterpret_cast<int32_t*>(bp + (size_t)FRAG_BYTES * t.frags_b);
   int32_t* be = ae + t.a_exps;
+  int32_t* ar = be + t.b_exps;
+  int32_t* br = ar + t.a_rows;
   const int n_jobs = (int)host_jobs.size();
   PrepArgs pa{dev_jobs, t.jobs, n_jobs, t.items_a,
-              {{a, ae, ap, t.prefix}, {b, be, bp, t.prefix + (n_jobs + 1)}}};
+              {{a, ae, ap, t.prefix, ar}, {b, be, bp, t.prefix + (n_jobs + 1), br}}};
   const uint32_t items = t.items_a + t.items_b;
+  if (i8) {
+    SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ar, br, alpha, beta};
+    switch (S) {
+      case 2: launch_digits_i8_s<2>(pa, items, s); launch_split_i8_s<2>(t, g, s); break;
+      case 3: launch_digits_i8_s<3>(pa, items, s); launch_split_i8_s<3>(t, g, s); break;
+      case 4: launch_digits_i8_s<4>(pa, items, s); launch_split_i8_s<4>(t, g, s); break;
+      case 6: launch_digits_i8_s<6>(pa, items, s); launch_split_i8_s<6>(t, g, s); break;
+      default: launch_digits_i8_s<5>(pa, items, s); launch_split_i8_s<5>(t, g, s); break;
+    }
+    SPLIT_HIP(hipGetLastError());
+    return TNT_OK;
+  }
   SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ae, be, alpha, beta};
   switch (S) {
     case 2: launch_digits_s<2>(pa, items, s); launch_split_s<2>(t, g, s); break;

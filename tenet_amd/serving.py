@@ -89,7 +89,8 @@ class HeffServer:
         self.flops = {}
         self.fp = {}
         self.results = None
-        self._prec = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16}[precision]
+        self._prec = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16,
+                      "split_i8": N.PREC_SPLIT_I8}[precision]
         self._free = {}                       # (stream, D) -> idle buffer sets
         self._events = [[] for _ in self.rts]  # idle events per stream
         self._dot = {}                        # D -> (regions, weights, count) of theta's sectors

@@ -397,7 +397,7 @@ class Runtime:
             self._ctx = None
 
 
-PRECISIONS = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16}
+PRECISIONS = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16, "split_i8": N.PREC_SPLIT_I8}
 
 
 # ---------------------------------------------------------------------------

@@ -5,13 +5,14 @@ hold at any size (the small-size parity against the oracle lives in
 test_gpu_workloads.py):
 
 * U(1) environment update, D = 4096: linear in the environment; split-BF16
-  within 1e-10 of FP64; an abelian permutation followed by its inverse gives
+  and split-INT8 within 1e-10 of FP64; an abelian permutation followed by its inverse gives
   back the payload bit for bit.
 * SU(2) F-move contraction, multiplicity floor(64/(2j+1)) (2.74 GB per
   operand): the contraction with both operands recoupled inside it equals
   explicit F-move permutations followed by a canonical compose (the
   reference's route invariance, contract_fusion.rs:532-700, at 1e-10 there);
-  a permutation followed by its inverse returns the payload within 1e-14.
+  a permutation followed by its inverse returns the payload within 1e-14;
+  split-BF16 and split-INT8 within 1e-10 of FP64.
 
 Norms are taken on the device (FP64 weighted inner products), so nothing of
 gigabyte size crosses PCIe.
@@ -57,9 +58,10 @@ def test_u1_env_update_full_size_is_linear_in_environment(rt, u1_env):
     assert math.isfinite(f1.norm()) and f1.norm() > 0
 
 
-def test_u1_env_update_full_size_split_bf16_within_1e10(rt, u1_env):
+@pytest.mark.parametrize("prec", ["split_bf16", "split_i8"])
+def test_u1_env_update_full_size_split_within_1e10(rt, u1_env, prec):
     ref = u1_env.run().copy()
-    split = MPSEnvUpdate(rt, "U1", 4096, "split_bf16")
+    split = MPSEnvUpdate(rt, "U1", 4096, prec)
     got = split.run()
     assert np.array_equal(split.L.to_numpy(), u1_env.L.to_numpy())  # same seeded inputs
     err = dev_rel(got, ref)
@@ -90,6 +92,18 @@ def test_su2_fmove_full_size_route_invariance(rt, su2_fmove):
     del Ap, Bp
     err = dev_rel(c2, c1)
     assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_i8"])
+def test_su2_fmove_full_size_split_within_1e10(rt, su2_fmove, prec):
+    """The benchmark's split-precision contraction against the FP64 one
+    (1.8 GFLOP per 565-leg sector job, K up to 7197: the 128 x 128 tiles)."""
+    ref = su2_fmove.run().copy()
+    got = su2_fmove.A.contract([1, 3], su2_fmove.B, [2, 0], [0, 1, 2, 3], 2, precision=prec)
+    assert got.space == ref.space
+    err = dev_rel(got, ref)
+    del got
+    assert err < 1e-10, err
 
 
 def test_su2_permutation_roundtrip_full_size(rt, su2_fmove):

@@ -47,14 +47,14 @@ def _oracle_apply(H, theta):
 
 
 @pytest.mark.parametrize("rule_name,D", [("U1", 40), ("SU2", 24)])
-@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16", "split_i8"])
 def test_heff_apply_matches_oracle(rt, rule_name, D, prec):
     H = TwoSiteHeff(rt, rule_name, D, seed=11, precision=prec)
     th = TensorMap.rand(rt, H.theta_space, seed=3, normal=True)
     y = H.apply(th)
     assert y.space == H.theta_space
     want = _oracle_apply(H, _o(th)).data
-    tol = {"f64": 1e-13, "f32": 5e-6, "split_bf16": 1e-10}[prec]
+    tol = {"f64": 1e-13, "f32": 5e-6, "split_bf16": 1e-10, "split_i8": 1e-10}[prec]
     assert rel(y.to_numpy(), want) < tol, (rule_name, prec, rel(y.to_numpy(), want))
     # reuse of the scratch intermediates gives the same bits
     sc = {}

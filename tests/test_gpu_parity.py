@@ -65,7 +65,7 @@ GEMM_SHAPES = [(16, 4, 16), (1, 1, 1), (17, 5, 33), (64, 64, 64), (130, 70, 129)
 
 
 @pytest.mark.parametrize("lop,rop", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     rng = np.random.default_rng(7 + 2 * lop + rop)
     jobs, lhs, rhs, dst_parts, want = [], [], [], [], []
@@ -91,7 +91,7 @@ def test_gemm_batch_matches_numpy(rt, lop, rop, prec):
     got = _get(rt, bd, len(D))
     _free(rt, bl, br, bd)
     w = np.concatenate(want)
-    tol = {0: 1e-13, 1: 1e-5, 2: 1e-10}[prec]
+    tol = {0: 1e-13, 1: 1e-5, 2: 1e-10, 3: 1e-10}[prec]
     o = 0
     for (m, k, n), part in zip(GEMM_SHAPES, want):
         assert rel(got[o:o + m * n], part) < tol, ((m, k, n), lop, rop, prec)
@@ -370,7 +370,7 @@ CONTRACTS = [([2, 3], [0, 1], [0, 1, 2, 3], 2), ([3, 2], [0, 1], [0, 1, 2, 3], 2
 
 @pytest.mark.parametrize("rule_name,pairs", [("U1", U1V), ("SU2", SU2V)])
 @pytest.mark.parametrize("ci", range(len(CONTRACTS)))
-@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16", "split_i8"])
 def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
     axa, axb, perm, nout = CONTRACTS[ci]
     ps, os_ = spaces(rule_name, [(pairs, False), (pairs, False)], [(pairs, False), (pairs, False)])
@@ -378,7 +378,7 @@ def test_contract_matches_oracle(rt, rule_name, pairs, ci, prec):
     B = TensorMap.rand(rt, ps, seed=2)
     got = A.contract(axa, B, axb, perm, nout, precision=prec).to_numpy()
     want = OTensor(os_, A.to_numpy()).contract(axa, OTensor(os_, B.to_numpy()), axb, perm, nout).data
-    tol = {"f64": 1e-13, "f32": 2e-6, "split_bf16": 1e-10}[prec]
+    tol = {"f64": 1e-13, "f32": 2e-6, "split_bf16": 1e-10, "split_i8": 1e-10}[prec]
     assert rel(got, want) < tol, (rule_name, ci, prec, rel(got, want))
 
 
@@ -502,9 +502,11 @@ def test_graph_capture_replays_match_eager(rt):
     h.tnt_graph_destroy(rt._ctx, g)
 
 
-@pytest.mark.parametrize("slices,tol", [(4, 1e-8), (5, 1e-10), (6, 1e-12)])
-def test_split_bf16_error_vs_slices(rt, slices, tol):
-    """Relative Frobenius error of the split-BF16 GEMM against FP64, per slice count."""
+@pytest.mark.parametrize("prec", [2, 3])
+@pytest.mark.parametrize("slices,tol", [(4, 1e-7), (5, 1e-10), (6, 1e-12)])
+def test_split_error_vs_slices(rt, prec, slices, tol):
+    """Relative Frobenius error of the split GEMMs (BF16 and INT8 digit planes)
+    against FP64, per slice count."""
     rng = np.random.default_rng(slices)
     m, k, n = 200, 700, 150
     A = rng.standard_normal((m, k)) * np.exp(rng.uniform(-3, 3, (m, 1)))  # uneven row scales
@@ -514,17 +516,111 @@ def test_split_bf16_error_vs_slices(rt, slices, tol):
     jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
     rt._check(N.hip().tnt_set_split_slices(slices))
     try:
-        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 1.0, 0.0, 2))
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 1.0, 0.0, prec))
         got = _get(rt, bd, m * n).reshape(n, m).T
     finally:
         N.hip().tnt_set_split_slices(5)
         _free(rt, bl, br, bd)
     err = rel(got, A @ B)
-    print(f"split-bf16 slices={slices}: rel err {err:.3e}")
+    print(f"split prec={prec} slices={slices}: rel err {err:.3e}")
     assert err < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("split_bf16", 1e-10)])
+def _carry_digit_matrix(rng, rows, cols, digits=5):
+    """Rows x cols with entries 2^-7 (d_0 + sum_{p=1..digits-1} d_p 2^-8p),
+    |d_0| <= 120 and the lower digits drawn from {-128, -127, 0, 1, 127, 128},
+    column 0 = 0.75 (every row's maximum is in [0.5, 126.5 / 128), exponent 0).
+    `digits` INT8 digit planes hold such entries exactly; with five, their
+    round-to-nearest digits often reach +-128, the case that needs a carry
+    into the digit above.  With more than one digit, every fourth row instead
+    has maximum 1 - 2^-20 (in the top of its binade: the line exponent goes up
+    by one, the first digit shrinks to [-64, 64]) and its last digit even, so
+    it stays exact."""
+    val = rng.integers(-120, 121, (rows, cols)).astype(np.float64) * 2.0 ** -7
+    for p in range(1, digits):
+        d = rng.choice([-128, -127, 0, 1, 127, 128], size=(rows, cols))
+        if p == digits - 1:
+            d[::4] &= ~1
+        val += d * 2.0 ** (-7 - 8 * p)
+    val[:, 0] = 0.75
+    if digits > 1:
+        val[::4, 0] = 1.0 - 2.0 ** -20
+    return val
+
+
+@pytest.mark.parametrize("carry_side", ["lhs", "rhs"])
+@pytest.mark.parametrize("lop,rop", [(0, 0), (1, 1)])
+def test_split_i8_digits_on_the_carry_boundary_are_exact(rt, carry_side, lop, rop):
+    """One operand exact in five INT8 planes with digits on the carry
+    boundary, the other exact in its first plane (multiples of 2^-7): every
+    nonzero digit product then has level p + q <= 4 and is kept, so the
+    split-INT8 GEMM has no truncation error and matches the FP64 product to
+    rounding.  A digit of 128 stored without its carry would be off by ~2^-7."""
+    rng = np.random.default_rng(31 + lop + (2 if carry_side == "rhs" else 0))
+    m, k, n = 70, 300, 50
+    A = _carry_digit_matrix(rng, m, k, 5 if carry_side == "lhs" else 1)
+    B = _carry_digit_matrix(rng, n, k, 5 if carry_side == "rhs" else 1).T  # column maxima as above
+    sa, sb = (A.T if lop else A), (B.T if rop else B)
+    bl, br = _buf(rt, sa.reshape(-1, order="F")), _buf(rt, sb.reshape(-1, order="F"))
+    bd = _buf(rt, np.zeros(m * n))
+    jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
+    try:
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, lop, rop, 1.0, 0.0, 3))
+        got = _get(rt, bd, m * n).reshape(n, m).T
+    finally:
+        _free(rt, bl, br, bd)
+    assert rel(got, A @ B) < 1e-14, rel(got, A @ B)
+
+
+def test_split_i8_long_k_spans_two_exact_windows(rt):
+    """K = 16684 > 16384: the INT32 level sums are flushed after the first
+    window and the second window adds into D (alpha, beta != 0)."""
+    rng = np.random.default_rng(5)
+    m, k, n = 40, 16684, 24
+    A, B, D0 = rng.standard_normal((m, k)), rng.standard_normal((k, n)), rng.standard_normal((m, n))
+    bl, br = _buf(rt, A.reshape(-1, order="F")), _buf(rt, B.reshape(-1, order="F"))
+    bd = _buf(rt, D0.reshape(-1, order="F"))
+    jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
+    try:
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 0.75, -0.5, 3))
+        got = _get(rt, bd, m * n).reshape(n, m).T
+    finally:
+        _free(rt, bl, br, bd)
+    assert rel(got, 0.75 * A @ B - 0.5 * D0) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+def test_split_tile_shapes_match_numpy(rt, prec):
+    """Jobs sized so that the split GEMMs use each of their four tile shapes
+    (128x128 on 8 waves, 128x64, 64x128, 64x64: K >= 1024 and enough tiles),
+    accumulating into a nonzero D."""
+    rng = np.random.default_rng(40 + prec)
+    shapes = [(1100, 1100, 1100), (1050, 1030, 1000), (600, 1030, 200), (200, 1030, 600), (100, 1030, 90)]
+    jobs, lhs, rhs, want = [], [], [], []
+    lo = ro = do = 0
+    d_parts = []
+    for m, k, n in shapes:
+        a, b, d0 = rng.standard_normal((m, k)), rng.standard_normal((k, n)), rng.standard_normal((m, n))
+        lhs.append(a.reshape(-1, order="F"))
+        rhs.append(b.reshape(-1, order="F"))
+        d_parts.append(d0.reshape(-1, order="F"))
+        want.append((1.5 * a @ b + 0.25 * d0).reshape(-1, order="F"))
+        jobs.append(N.GemmJob(do, lo, ro, m, k, n))
+        lo, ro, do = lo + m * k, ro + k * n, do + m * n
+    L, R, D = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs)), _buf(rt, np.concatenate(d_parts))
+    try:
+        arr = (N.GemmJob * len(jobs))(*jobs)
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, 0, 0, 1.5, 0.25, prec))
+        got = _get(rt, D, do)
+    finally:
+        _free(rt, L, R, D)
+    o = 0
+    for (m, k, n), part in zip(shapes, want):
+        assert rel(got[o:o + m * n], part) < 1e-10, ((m, k, n), prec)
+        o += m * n
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("split_bf16", 1e-10), ("split_i8", 1e-10)])
 def test_batched_grouped_launch_matches_oracle(rt, prec, tol):
     """Batched U(1)xZ2 merges at small sizes: every merge of one grouped launch vs the FP64 oracle."""
     from oracle.fusion import HomSpace as OH, Leg as OL

@@ -3,7 +3,8 @@
 The environment update is checked against the oracle's chain written with the
 textbook index order (T2[a', w; s', b]); the workload keeps T2 in the core
 GEMM's output order instead, which must not change L'.
-Bars: FP64 within 1e-13 relative Frobenius error; split-BF16 within 1e-10.
+Bars: FP64 within 1e-13 relative Frobenius error; split-BF16 and split-INT8
+within 1e-10.
 """
 import numpy as np
 import pytest
@@ -32,7 +33,7 @@ def _o(t):
 
 
 @pytest.mark.parametrize("rule_name,D", [("U1", 128), ("SU2", 256)])
-@pytest.mark.parametrize("prec", ["f64", "split_bf16"])
+@pytest.mark.parametrize("prec", ["f64", "split_bf16", "split_i8"])
 def test_env_update_matches_oracle(rt, rule_name, D, prec):
     wl = MPSEnvUpdate(rt, rule_name, D, prec)
     got = wl.run().to_numpy()
@@ -41,7 +42,7 @@ def test_env_update_matches_oracle(rt, rule_name, D, prec):
     t1 = L.contract([2], A, [0], [0, 1, 2, 3], 2)
     t2 = t1.contract([1, 2], W, [3, 1], [0, 2, 3, 1], 2)
     want = A.adjoint().contract([2, 0], t2, [0, 2], [0, 1, 2], 2).data
-    tol = {"f64": 1e-13, "split_bf16": 1e-10}[prec]
+    tol = {"f64": 1e-13, "split_bf16": 1e-10, "split_i8": 1e-10}[prec]
     assert rel(got, want) < tol, (rule_name, prec, rel(got, want))
 
 
@@ -163,7 +164,7 @@ def test_fused_route_accumulates_and_reads_adjoint_mpo(rt):
         assert rel(out.to_numpy(), 2.0 * prod - 0.5 * before) < 1e-14
 
 
-@pytest.mark.parametrize("prec", ["f64", "split_bf16"])
+@pytest.mark.parametrize("prec", ["f64", "split_bf16", "split_i8"])
 def test_env_update_graph_replays_match_eager(rt, prec):
     """Every replay of the captured step equals the eager step bit for bit,
     with eager work and synchronisations between the replays."""
