@@ -160,6 +160,27 @@ __device__ __forceinline__ int64_t elem_index(const GJob& g, int side, int line,
   return (g.ops & 2) ? (line + (int64_t)kk * g.ldb) : (kk + (int64_t)line * g.ldb);
 }
 
+// Element (line, kk) of the item's operand, 0 outside the job.  Lanes outside
+// the job read a zero word instead (the address is selected, not the value),
+// so every load is unconditional and its value used: a load whose value only
+// feeds a select is sunk into a branch and waited for before the next load.
+__device__ double kZeroElem;  // never written: reads as 0
+__device__ __forceinline__ double load_elem(const gptr<const double>& M, const GJob& g, int side, int line,
+                                            int kk, int lines, int K) {
+  const bool ok = line < lines && kk < K;
+  const gptr<const double> p =
+      ok ? M + elem_index(g, side, line, kk) : (gptr<const double>)(uintptr_t)&kZeroElem;
+  return *p;
+}
+
+// Element (line, kk) with both indices clamped into the job (lines, K >= 1):
+// a duplicate of an element of the same line for kk >= K, of another line for
+// line >= lines -- harmless in a line maximum whose padded lines are not stored.
+__device__ __forceinline__ double load_clamped(const gptr<const double>& M, const GJob& g, int side, int line,
+                                               int kk, int lines, int K) {
+  return M[elem_index(g, side, min(line, lines - 1), min(kk, K - 1))];
+}
+
 struct PrepItem {
   int side, job, lb, kc;
 };
@@ -203,7 +224,7 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int kk = st * KSTEP + kg * 8 + q;
-      v[h][q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
+      v[h][q] = load_elem(M, g, it.side, line, kk, lines, K);
       mx = fmax(mx, fabs(v[h][q]));
     }
   }
@@ -259,7 +280,8 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
 //      of its binade), EXP_NONE for an all-zero chunk (chunk-major, as the
 //      BF16 exponents);
 //   2. split_i8_digits: the line's exponent over all of K, e = max over its
-//      chunks (0 for an all-zero line), so |a| < 126.5 for a = A 2^(7 - e);
+//      chunks (0 for an all-zero line; a job of one chunk skips pass 1 and
+//      takes the maximum here), so |a| < 126.5 for a = A 2^(7 - e);
 //      a is cut into d_0 = rint(a) in [-126, 126] and S - 1 base-256 digits by
 //      repeated round-to-nearest (in [-128, 128]); a digit that reaches 128
 //      becomes 128 - 256 and carries one into the digit above, from the last
@@ -271,8 +293,9 @@ __device__ __forceinline__ int line_count(const GJob& g, int side) { return side
 __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
   __shared__ double red[4][4][16];
   const PrepItem it = prep_item(x);
-  const GJob g = x.jobs[it.job];
   const SplitJob s = x.sj[it.job];
+  if (s.nkc == 1) return;  // one chunk: split_i8_digits takes the maximum itself
+  const GJob g = x.jobs[it.job];
   const PrepSide& sd = x.side[it.side];
   const int lines = line_count(g, it.side);
   const int K = g.k;
@@ -286,10 +309,7 @@ __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
     const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int kk = st * I8_KSTEP + kg * 16 + q;
-      v[q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
-    }
+    for (int q = 0; q < 16; ++q) v[q] = load_clamped(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
 #pragma unroll
     for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[q]));
   }
@@ -324,7 +344,39 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
-  if (t < 16) {
+  double v[2][16];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      v[h][q] = load_elem(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
+  }
+  if (s.nkc == 1) {  // the line's exponent from this chunk (no exponent pass ran)
+    __shared__ double red[4][4][16];
+    double mx = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[h][q]));
+    red[wave][kg][li] = mx;
+    __syncthreads();
+    if (t < 16) {
+      double m2 = 0.0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[w][c][t]);
+      int e = 0;  // all-zero line: its digits are zero
+      if (m2 > 0.0) {
+        frexp(m2, &e);
+        if (ldexp(m2, 7 - e) >= 126.5) ++e;
+      }
+      sexp[t] = e;
+      const int l = it.lb * 16 + t;
+      if (l < lines) sd.rows[(it.side == 0 ? s.ra0 : s.rb0) + l] = e;
+    }
+  } else if (t < 16) {
     const int l = it.lb * 16 + t;
     int e = EXP_NONE;
     if (l < lines) {
@@ -342,12 +394,6 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   for (int h = 0; h < 2; ++h) {
     const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
     if (st >= s.ns) continue;
-    double v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int kk = st * I8_KSTEP + kg * 16 + q;
-      v[q] = (line < lines && kk < K) ? M[elem_index(g, it.side, line, kk)] : 0.0;
-    }
     uint32_t packed[S][4];
 #pragma unroll
     for (int p = 0; p < S; ++p)
@@ -355,7 +401,7 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
       for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      double a = ldexp(v[q], 7 - e);  // |a| < 126.5
+      double a = ldexp(v[h][q], 7 - e);  // |a| < 126.5
       int d[S];
       double r = rint(a);
       d[0] = (int)r;
