@@ -99,9 +99,10 @@ constexpr int shape_cols(int c) { return (c == SPLIT_64x128 || c >= SPLIT_128x12
 struct SplitTables {
   SplitJob* jobs = nullptr;
   GTile* tiles = nullptr;
-  int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid
+  int64_t* prefix = nullptr;   // per-side per-job start of its items in the pre-pass grid, then the
+                               // same for the INT8 exponent pass (jobs of one chunk have no items)
   uint32_t tile_start[SPLIT_SHAPES] = {}, tile_count[SPLIT_SHAPES] = {};
-  uint32_t items_a = 0, items_b = 0;
+  uint32_t items_a = 0, items_b = 0, exp_items_a = 0, exp_items_b = 0;
   int64_t frags_a = 0, frags_b = 0, a_exps = 0, b_exps = 0, a_rows = 0, b_rows = 0;
 };
 
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
   __shared__ double red[4][4][16];
   const PrepItem it = prep_item(x);
   const SplitJob s = x.sj[it.job];
-  if (s.nkc == 1) return;  // one chunk: split_i8_digits takes the maximum itself
+  if (s.nkc == 1) return;  // (not in the grid) one chunk: split_i8_digits takes the maximum itself
   const GJob g = x.jobs[it.job];
   const PrepSide& sd = x.side[it.side];
   const int lines = line_count(g, it.side);
@@ -800,9 +801,7 @@ static void launch_split_i8_s(const SplitTables& t, const SplitGemmArgs& a0, hip
 
 template <int S>
 static void launch_digits_i8_s(const PrepArgs& pa, uint32_t items, hipStream_t s) {
-  if (!items) return;
-  hipLaunchKernelGGL(split_i8_chunk_exp, dim3(items), dim3(256), 0, s, pa);
-  hipLaunchKernelGGL(split_i8_digits<S>, dim3(items), dim3(256), 0, s, pa);
+  if (items) hipLaunchKernelGGL(split_i8_digits<S>, dim3(items), dim3(256), 0, s, pa);
 }
 
 #define SPLIT_HIP(expr)                                                    \
@@ -845,7 +844,7 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
     SplitTables t;
     std::vector<SplitJob> sj(host_jobs.size());
     std::vector<GTile> per[SPLIT_SHAPES];
-    std::vector<int64_t> pre_a{0}, pre_b{0};
+    std::vector<int64_t> pre_a{0}, pre_b{0}, xpre_a{0}, xpre_b{0};
     int64_t small_tiles = 0;
     for (const GJob& j : host_jobs) small_tiles += (int64_t)((j.m + 63) / 64) * ((j.n + 63) / 64);
     for (size_t i = 0; i < host_jobs.size(); ++i) {
@@ -879,6 +878,8 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
       sj[i] = x;
       pre_a.push_back(pre_a.back() + ia);
       pre_b.push_back(pre_b.back() + ib);
+      xpre_a.push_back(xpre_a.back() + (x.nkc > 1 ? ia : 0));
+      xpre_b.push_back(xpre_b.back() + (x.nkc > 1 ? ib : 0));
       if (j.m == 0 || j.n == 0) continue;
       const int bm = shape_rows(x.cfg), bn = shape_cols(x.cfg);
       for (int tm = 0; tm < (j.m + bm - 1) / bm; ++tm)
@@ -896,9 +897,13 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
     }
     t.items_a = (uint32_t)pre_a.back();
     t.items_b = (uint32_t)pre_b.back();
+    t.exp_items_a = (uint32_t)xpre_a.back();
+    t.exp_items_b = (uint32_t)xpre_b.back();
     std::vector<int64_t> pre;
     pre.insert(pre.end(), pre_a.begin(), pre_a.end());
     pre.insert(pre.end(), pre_b.begin(), pre_b.end());
+    pre.insert(pre.end(), xpre_a.begin(), xpre_a.end());
+    pre.insert(pre.end(), xpre_b.begin(), xpre_b.end());
     void *pj, *pt, *pp;
     SPLIT_HIP(hipMalloc(&pj, sizeof(SplitJob) * sj.size()));
     ws->table_allocs.push_back(pj);
@@ -943,6 +948,12 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
               {{a, ae, ap, t.prefix, ar}, {b, be, bp, t.prefix + (n_jobs + 1), br}}};
   const uint32_t items = t.items_a + t.items_b;
   if (i8) {
+    PrepArgs px = pa;  // exponent pass: items of the jobs of more than one chunk
+    px.items_a = t.exp_items_a;
+    px.side[0].prefix = t.prefix + 2 * (n_jobs + 1);
+    px.side[1].prefix = t.prefix + 3 * (n_jobs + 1);
+    if (t.exp_items_a + t.exp_items_b)
+      hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a + t.exp_items_b), dim3(256), 0, s, px);
     SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ar, br, alpha, beta};
     switch (S) {
       case 2: launch_digits_i8_s<2>(pa, items, s); launch_split_i8_s<2>(t, g, s); break;
