@@ -349,7 +349,7 @@ def main():
                              "gemm_vs_vendor"])
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16", "split_i8"],
-                    help="GEMM precision (default: split_bf16 for serve, f64 otherwise)")
+                    help="GEMM precision (default: split_i8 for serve, f64 otherwise)")
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
@@ -368,8 +368,8 @@ def main():
                     help="pick GEMM tile shapes per contraction by measurement before timing")
     args = ap.parse_args()
     if args.precision is None:
-        # the serving configuration runs split-BF16 GEMMs with FP64 dot products
-        args.precision = "split_bf16" if args.workload == "serve" else "f64"
+        # the serving configuration runs split-INT8 GEMMs with FP64 dot products
+        args.precision = "split_i8" if args.workload == "serve" else "f64"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the contraction would shard by "

@@ -2,7 +2,7 @@
 
 A request is (arrival time, bond dimension D, seed): build theta (seeded
 normal fill on the device), apply H_eff of its D bucket (four block-sparse
-contractions, split-BF16 GEMMs by default) and take the FP64 Rayleigh
+contractions, split-INT8 GEMMs by default) and take the FP64 Rayleigh
 numerator <theta, H theta> on the device.  Environments (L, R, W) of every D
 bucket stay resident in HBM for the life of the server.
 
@@ -72,7 +72,7 @@ class _BufSet:
 
 class HeffServer:
     def __init__(self, rule="U1", buckets=None, n_streams=4, max_inflight=16,
-                 hbm_budget_bytes=48 << 30, precision="split_bf16", env_seed=7, mpo="random",
+                 hbm_budget_bytes=48 << 30, precision="split_i8", env_seed=7, mpo="random",
                  graphs=True):
         self.rule = rule
         self.graphs = graphs
@@ -99,7 +99,7 @@ class HeffServer:
     def warm(self):
         """Compile every (bucket, stream) artifact, build one buffer set per pair
         and capture its graph.  The graphs are captured after every bucket has
-        run once on every stream: a stream's split-BF16 workspace grows (and is
+        run once on every stream: a stream's split GEMM workspace grows (and is
         reallocated) while larger buckets warm up, and a graph keeps the
         addresses it was captured with."""
         for D in self.buckets:
