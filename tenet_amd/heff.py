@@ -2,8 +2,9 @@
 
 The reference's Krylov layer (tenet-krylov) is a matrix-free solver over
 ``LinearOperator::apply`` and ``KrylovVector::dot_real``
-(tenet-krylov/src/operator.rs:2-15, vector.rs:2-48, cg.rs:64-190); it has no
-Lanczos and no H_eff.  Here ``TwoSiteHeff.apply`` is the operator (a fixed
+(tenet-krylov/src/operator.rs:2-15, vector.rs:2-48, cg.rs:64-190; restated
+over device vectors in tenet_amd.krylov); it has no Lanczos and no H_eff.
+Here ``TwoSiteHeff.apply`` is the operator (a fixed
 chain of four block-sparse contractions over resident environments) and the
 FP64 inner products run on the device (tnt_weighted_dot).
 

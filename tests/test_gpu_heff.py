@@ -63,9 +63,9 @@ def test_heff_apply_matches_oracle(rt, rule_name, D, prec):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("rule_name,D", [("U1", 24), ("SU2", 24)])
-def test_lanczos_gram_matches_dense_oracle(rt, rule_name, D):
-    H = TwoSiteHeff(rt, rule_name, D, seed=5)
+def _dense_gram(H):
+    """The Gram operator L^T L of H's environment as a dense matrix, column by
+    column from the oracle's contractions."""
     osp = ospace(H.theta_space)
     n = H.theta_space.len
     L = _o(H.L)
@@ -76,12 +76,41 @@ def test_lanczos_gram_matches_dense_oracle(rt, rule_name, D):
         e[i] = 1.0
         x = L.contract([2], OTensor(osp, e), [0], [0, 1, 2, 3, 4], 2)
         cols.append(Lad.contract([1, 2], x, [0, 1], [0, 1, 2, 3], 1).data)
-    G = np.array(cols).T
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("rule_name,D", [("U1", 24), ("SU2", 24)])
+def test_lanczos_gram_matches_dense_oracle(rt, rule_name, D):
+    H = TwoSiteHeff(rt, rule_name, D, seed=5)
+    n = H.theta_space.len
+    G = _dense_gram(H)
     want = float(np.min(np.linalg.eigvals(G).real))
     v0 = TensorMap.rand(rt, H.theta_space, seed=9, normal=True)
     sc = {}
     ev, res, iters = lanczos(lambda v: H.gram(v, sc), v0, iters=n, tol=1e-12)
     assert abs(ev - want) <= 1e-9 * max(1.0, abs(want)), (ev, want, res, iters)
+
+
+@pytest.mark.parametrize("rule_name,D", [("U1", 24), ("SU2", 24)])
+def test_cg_damped_gram_solve_matches_dense_oracle(rt, rule_name, D):
+    """The reference's CG (cg.rs:58-190) on device vectors: (L^T L + 0.5 I) x = b
+    with the device Gram operator and weighted FP64 inner products, against a
+    dense solve of the oracle's matrix; the residual is re-checked with a fresh
+    device matvec."""
+    from tenet_amd.krylov import CgOptions, DeviceVector, cg
+    H = TwoSiteHeff(rt, rule_name, D, seed=5)
+    b = TensorMap.rand(rt, H.theta_space, seed=13, normal=True)
+    sc = {}
+    res = cg(lambda v: DeviceVector(H.gram(v.t, sc)), DeviceVector(b), CgOptions(rtol=1e-12, damping=0.5))
+    assert res.stats.converged and res.stats.breakdown is None, res.stats
+    assert res.stats.matvecs == res.stats.iterations <= H.theta_space.len
+    x = res.solution.t
+    r = H.gram(x, {}).add(x, 1.0, 0.5).add(b, 1.0, -1.0)
+    assert r.norm() <= 1e-10 * b.norm(), (r.norm(), b.norm())
+    G = _dense_gram(H)
+    want = np.linalg.solve(G + 0.5 * np.eye(len(G)), b.to_numpy())
+    got = x.to_numpy()
+    assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)
 
 
 def _identity_env(rt, V, M, left):
