@@ -449,6 +449,18 @@ struct SplitGemmArgs {
 
 __device__ __forceinline__ void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
+// Workgroup b of a launch runs on XCD b % 8 (workgroups are dispatched to the
+// XCDs round-robin).  Tile lists are padded to a multiple of 256 and read in
+// super-blocks of 32 consecutive tiles, super-block s on XCD s % 8, so the
+// workgroups one XCD runs side by side cover one block of 4 x 8 tiles of a
+// job: their k steps stream 4 A and 8 B panels through that XCD's L2 instead
+// of one A and 32 B panels.  Bijective for b < 256 n.
+constexpr uint32_t kSuperBlock = 32, kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b) {
+  const uint32_t xcd = b % kXcds, local = b / kXcds;
+  return ((local / kSuperBlock) * kXcds + xcd) * kSuperBlock + local % kSuperBlock;
+}
+
 // 16 bytes per lane from global memory straight into LDS at lds + 16 * lane:
 // global address = base (uniform, SGPR pair) + voff (per lane).  Written as
 // asm so that the compiler's wait insertion does not treat every later LDS
@@ -611,7 +623,8 @@ __global__ __launch_bounds__(64 * GM * GN) void split_gemm_kernel(SplitGemmArgs 
   __shared__ bf16x8 B0[K::FB][64];
   __shared__ bf16x8 A1[K::FA][64];
   __shared__ bf16x8 B1[K::FB][64];
-  const GTile tile = g.tiles[blockIdx.x];
+  const GTile tile = g.tiles[xcd_tile(blockIdx.x)];
+  if (tile.job < 0) return;  // padding
   const SplitJob job = g.sj[tile.job];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -695,7 +708,8 @@ __global__ __launch_bounds__(64 * GM * GN) void split_i8_gemm_kernel(SplitGemmAr
   __shared__ i32x4 B0[K::FB][64];
   __shared__ i32x4 A1[K::FA][64];
   __shared__ i32x4 B1[K::FB][64];
-  const GTile tile = g.tiles[blockIdx.x];
+  const GTile tile = g.tiles[xcd_tile(blockIdx.x)];
+  if (tile.job < 0) return;  // padding
   const SplitJob job = g.sj[tile.job];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -881,12 +895,19 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
       xpre_a.push_back(xpre_a.back() + (x.nkc > 1 ? ia : 0));
       xpre_b.push_back(xpre_b.back() + (x.nkc > 1 ? ib : 0));
       if (j.m == 0 || j.n == 0) continue;
+      // blocks of 4 tile rows x 8 tile columns, consecutive (see xcd_tile)
       const int bm = shape_rows(x.cfg), bn = shape_cols(x.cfg);
-      for (int tm = 0; tm < (j.m + bm - 1) / bm; ++tm)
-        for (int tn = 0; tn < (j.n + bn - 1) / bn; ++tn) per[x.cfg].push_back({(int32_t)i, tm, tn, x.cfg});
+      const int ntm = (j.m + bm - 1) / bm, ntn = (j.n + bn - 1) / bn;
+      for (int r0 = 0; r0 < ntm; r0 += 4)
+        for (int c0 = 0; c0 < ntn; c0 += 8)
+          for (int tm = r0; tm < std::min(ntm, r0 + 4); ++tm)
+            for (int tn = c0; tn < std::min(ntn, c0 + 8); ++tn) per[x.cfg].push_back({(int32_t)i, tm, tn, x.cfg});
     }
     std::vector<GTile> tiles;
     for (int c = 0; c < SPLIT_SHAPES; ++c) {
+      if (!per[c].empty())  // pad to whole rounds of super-blocks over the XCDs (no-op tiles)
+        per[c].resize((per[c].size() + kSuperBlock * kXcds - 1) / (kSuperBlock * kXcds) * (kSuperBlock * kXcds),
+                      GTile{-1, 0, 0, c});
       t.tile_start[c] = (uint32_t)tiles.size();
       t.tile_count[c] = (uint32_t)per[c].size();
       tiles.insert(tiles.end(), per[c].begin(), per[c].end());
