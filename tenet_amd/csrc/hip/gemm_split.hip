@@ -291,8 +291,18 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int line_count(const GJob& g, int side) { return side == 0 ? g.m : g.n; }
 
+// Whether consecutive k of one line are adjacent in memory (see elem_index).
+__device__ __forceinline__ bool k_contiguous(const GJob& g, int side) {
+  return side == 0 ? (g.ops & 1) != 0 : (g.ops & 2) == 0;
+}
+
+// Column of element (row, col) of a 16 x 64 staging tile: the XOR keeps both
+// the coalesced writes (4 rows x 16 columns per instruction) and the fragment
+// order reads (16 rows x 1 column per 16 lanes) at two LDS passes.
+__device__ __forceinline__ int stage_col(int row, int col) { return col ^ (row & 15) ^ ((row & 1) << 4); }
+
 __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
-  __shared__ double red[4][4][16];
+  __shared__ double red[4][1][16];
   const PrepItem it = prep_item(x);
   const SplitJob s = x.sj[it.job];
   if (s.nkc == 1) return;  // (not in the grid) one chunk: split_i8_digits takes the maximum itself
@@ -304,24 +314,52 @@ __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
-  double mx = 0.0;
+  if (k_contiguous(g, it.side)) {
+    // lanes (lane / 16, lane % 16) read k = 16 c + lane % 16 of lines
+    // lane / 16 + 4 r: one load covers four whole 128-byte lines; a line's
+    // maximum is then reduced over its 16 lanes
+    double m4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
-    double v[16];
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+      double v[4][4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = load_clamped(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[q]));
+        for (int c = 0; c < 4; ++c)
+          v[r][c] = load_clamped(M, g, it.side, it.lb * 16 + kg + 4 * r, st * I8_KSTEP + 16 * c + li, lines, K);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m4[r] = fmax(m4[r], fabs(v[r][c]));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) m4[r] = fmax(m4[r], __shfl_xor(m4[r], off, 16));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave][0][kg + 4 * r] = m4[r];  // all 16 lanes of a group write the same value
+  } else {
+    double mx = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+      double v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = load_clamped(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[q]));
+    }
+    // fold the four k groups of a line here so that red[w][0][line] holds it
+    mx = fmax(mx, __shfl_xor(mx, 16));
+    mx = fmax(mx, __shfl_xor(mx, 32));
+    if (kg == 0) red[wave][0][li] = mx;
   }
-  red[wave][kg][li] = mx;
   __syncthreads();
   if (t < 16 && it.lb * 16 + t < lines) {
     double m2 = 0.0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[w][c][t]);
+    for (int w = 0; w < 4; ++w) m2 = fmax(m2, red[w][0][t]);
     int e = EXP_NONE;
     if (m2 > 0.0) {
       frexp(m2, &e);                          // m2 = f 2^e, f in [0.5, 1)
@@ -346,12 +384,38 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
   double v[2][16];
+  if (k_contiguous(g, it.side)) {
+    // each wave stages its 16 x 64 step through LDS: coalesced loads (lanes
+    // (lane / 16, lane % 16) take k = 16 c + lane % 16 of lines lane / 16 +
+    // 4 r, four whole 128-byte lines per load), then fragment-order reads.
+    // The tile belongs to one wave, whose LDS accesses complete in order.
+    __shared__ double stage[4][16][64];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+      double w[4][4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      v[h][q] = load_elem(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          w[r][c] = load_elem(M, g, it.side, it.lb * 16 + kg + 4 * r, st * I8_KSTEP + 16 * c + li, lines, K);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) stage[wave][kg + 4 * r][stage_col(kg + 4 * r, 16 * c + li)] = w[r][c];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[h][q] = stage[wave][li][stage_col(li, kg * 16 + q)];
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        v[h][q] = load_elem(M, g, it.side, line, st * I8_KSTEP + kg * 16 + q, lines, K);
+    }
   }
   if (s.nkc == 1) {  // the line's exponent from this chunk (no exponent pass ran)
     __shared__ double red[4][4][16];
