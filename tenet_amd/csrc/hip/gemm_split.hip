@@ -198,6 +198,25 @@ __device__ __forceinline__ PrepItem prep_item(const PrepArgs& x) {
   return it;
 }
 
+__device__ __forceinline__ int line_count(const GJob& g, int side) { return side == 0 ? g.m : g.n; }
+
+// Whether consecutive k of one line are adjacent in memory (see elem_index).
+// Such sources are read with lanes (lane / 16, lane % 16) on 16 consecutive k
+// of lines lane / 16 + 4 r -- four whole 128-byte lines per load -- and staged
+// through LDS into fragment order, instead of gathering 8 bytes from 64 lines
+// per load.
+__device__ __forceinline__ bool k_contiguous(const GJob& g, int side) {
+  return side == 0 ? (g.ops & 1) != 0 : (g.ops & 2) == 0;
+}
+
+// Column of element (row, col) of a 16 x 64 INT8-step staging tile: the XOR
+// keeps both the coalesced writes (4 rows x 16 columns per instruction) and
+// the fragment order reads (16 rows x 1 column per 16 lanes) at two LDS passes.
+__device__ __forceinline__ int stage_col(int row, int col) { return col ^ (row & 15) ^ ((row & 1) << 4); }
+
+// The same for a 16 x 32 BF16-step tile, read 8 k per lane.
+__device__ __forceinline__ int stage_col32(int row, int col) { return col ^ (((row & 1) << 4) | ((row >> 1) & 7)); }
+
 // Wave w handles the chunk's k steps w and w + 4; lane l holds line l % 16
 // and k = 8 (l / 16) .. + 7 of a step -- the fragment order, so each lane
 // writes 16 bytes per plane.  For sources contiguous along lines, the 16 lanes
@@ -218,17 +237,39 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   const int li = lane & 15, kg = lane >> 4;
   const int line = it.lb * 16 + li;
   double v[2][8];
-  double mx = 0.0;
+  if (k_contiguous(g, it.side)) {
+    __shared__ double stage[4][16][32];  // one 16 x 32 step per wave (its LDS accesses complete in order)
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
+      double w[4][2];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kk = st * KSTEP + kg * 8 + q;
-      v[h][q] = load_elem(M, g, it.side, line, kk, lines, K);
-      mx = fmax(mx, fabs(v[h][q]));
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          w[r][c] = load_elem(M, g, it.side, it.lb * 16 + kg + 4 * r, st * KSTEP + 16 * c + li, lines, K);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) stage[wave][kg + 4 * r][stage_col32(kg + 4 * r, 16 * c + li)] = w[r][c];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[h][q] = stage[wave][li][stage_col32(li, kg * 8 + q)];
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[h][q] = load_elem(M, g, it.side, line, st * KSTEP + kg * 8 + q, lines, K);
     }
   }
+  double mx = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) mx = fmax(mx, fabs(v[h][q]));
   red[wave][kg][li] = mx;
   __syncthreads();
   if (t < 16) {
@@ -289,17 +330,7 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
 //      digit up, so every digit fits INT8 (d_0 <= 127) and
 //        A = 2^(e - 7) sum_p d_p 2^(-8 p)   (truncated after S digits).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int line_count(const GJob& g, int side) { return side == 0 ? g.m : g.n; }
 
-// Whether consecutive k of one line are adjacent in memory (see elem_index).
-__device__ __forceinline__ bool k_contiguous(const GJob& g, int side) {
-  return side == 0 ? (g.ops & 1) != 0 : (g.ops & 2) == 0;
-}
-
-// Column of element (row, col) of a 16 x 64 staging tile: the XOR keeps both
-// the coalesced writes (4 rows x 16 columns per instruction) and the fragment
-// order reads (16 rows x 1 column per 16 lanes) at two LDS passes.
-__device__ __forceinline__ int stage_col(int row, int col) { return col ^ (row & 15) ^ ((row & 1) << 4); }
 
 __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
   __shared__ double red[4][1][16];
