@@ -261,7 +261,7 @@ int tnt_event_destroy(tnt_ctx* ctx, tnt_event* ev);
 
 /* ---- kernel timing ---------------------------------------------------------- */
 /* When on, every recoupling replay (kind 0), every block-GEMM table run (kind
- * 1 + precision: all launches of one call, split-BF16 pre-passes included) and
+ * 1 + precision: all launches of one call, split pre-passes included) and
  * every FP64/FP32 GEMM kernel launch (kind 8 + 8 * precision + tile shape, shapes
  * as in tnt_set_tile_config) is bracketed by HIP events on the context stream;
  * stats report launches, total milliseconds and total algorithmic work (bytes
