@@ -1,4 +1,8 @@
-// Split-BF16 (Ozaki-style) block GEMM for gfx950.
+// Split (Ozaki-style) block GEMMs for gfx950: FP64 products from BF16 or INT8
+// digit planes.  The BF16 scheme is described here; the INT8 scheme (whole-K
+// line exponents, INT8 digits with carries, exact INT32 level sums on
+// v_mfma_i32_16x16x64_i8) at split_i8_chunk_exp / split_i8_gemm_kernel below.
+// Both share the tables, the workspace and the GEMM stage pipeline.
 //
 // FP64 C = op(A) op(B) from BF16 matrix-core products, K cut into chunks of
 // 256 (one chunk = one exact FP32 window of the GEMM):
