@@ -572,6 +572,33 @@ def test_split_i8_digits_on_the_carry_boundary_are_exact(rt, carry_side, lop, ro
     assert rel(got, A @ B) < 1e-14, rel(got, A @ B)
 
 
+@pytest.mark.parametrize("prec", [2, 3])
+def test_split_line_exponents_absorb_extreme_scales(rt, prec):
+    """Rows of A and columns of B scaled by 10^u, u uniform in [-100, 100],
+    one all-zero row and column: the line exponents absorb the scales, so
+    every C_ij is as accurate relative to ||a_i|| ||b_j|| as without them."""
+    rng = np.random.default_rng(8 + prec)
+    m, k, n = 90, 700, 70
+    ra, cb = 10.0 ** rng.uniform(-100, 100, m), 10.0 ** rng.uniform(-100, 100, n)
+    A = rng.standard_normal((m, k)) * ra[:, None]
+    B = rng.standard_normal((k, n)) * cb[None, :]
+    A[7] = 0.0
+    B[:, 5] = 0.0
+    bl, br = _buf(rt, A.reshape(-1, order="F")), _buf(rt, B.reshape(-1, order="F"))
+    bd = _buf(rt, np.zeros(m * n))
+    jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
+    try:
+        rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 1.0, 0.0, prec))
+        got = _get(rt, bd, m * n).reshape(n, m).T
+    finally:
+        _free(rt, bl, br, bd)
+    assert np.all(np.isfinite(got))
+    assert not got[7].any() and not got[:, 5].any()
+    scale = np.outer(np.linalg.norm(A, axis=1), np.linalg.norm(B, axis=0))
+    scale[scale == 0] = 1.0
+    assert np.max(np.abs(got - A @ B) / scale) < 1e-11
+
+
 def test_split_i8_long_k_spans_two_exact_windows(rt):
     """K = 16684 > 16384: the INT32 level sums are flushed after the first
     window and the second window adds into D (alpha, beta != 0)."""
