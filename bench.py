@@ -69,8 +69,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, D, budget_s=20.0):
-    """Time the oracle (numpy FP64, host cores) on the same workload."""
+def cpu_baseline(workload, D, budget_s=20.0, wl=None):
+    """Time the oracle (numpy FP64, host cores) on the same workload.  With the
+    device workload `wl` given, the oracle's operands are its downloaded
+    payloads, so the parity check compares results of identical inputs."""
     import numpy as np
 
     from oracle.fusion import HomSpace as OH, Leg as OL
@@ -90,9 +92,11 @@ def cpu_baseline(workload, D, budget_s=20.0):
         def leg(g):
             return OL(list(zip(g.ids, g.degs)), g.is_dual)
         oV, oP, oM = leg(V), leg(P), leg(M)
-        L = OTensor.randn(OH(orule, [oV, oM], [oV]), 3)
-        A = OTensor.randn(OH(orule, [oV], [oP, oV]), 4)
-        W = OTensor.randn(OH(orule, [oM, oP], [oP, oM]), 5)
+        sL, sA, sW = OH(orule, [oV, oM], [oV]), OH(orule, [oV], [oP, oV]), OH(orule, [oM, oP], [oP, oM])
+        if wl is not None:
+            L, A, W = OTensor(sL, wl.L.to_numpy()), OTensor(sA, wl.A.to_numpy()), OTensor(sW, wl.W.to_numpy())
+        else:
+            L, A, W = OTensor.randn(sL, 3), OTensor.randn(sA, 4), OTensor.randn(sW, 5)
         Ad = A.adjoint()
 
         def step():
@@ -302,13 +306,13 @@ def run_gemm_vs_vendor(args):
     for jb, grp in zip(jobs, groups):
         best = None
         for cfg in (-1, 0, 1, 2, 3, 5, 6):
-            h.tnt_set_tile_config(cfg)
+            h.tnt_set_tile_config(rt._ctx, cfg)
             t = timed(lambda: rt._check(h.tnt_gemm_grouped(rt._ctx, jb, len(grp), 1.0, 0.0, 0)))
             if best is None or t < best[0]:
                 best = (t, cfg)
         t_ours += best[0]
         cfgs.append(GEMM_KERNELS[best[1]].format("double") if best[1] >= 0 else "size rule")
-    h.tnt_set_tile_config(-1)
+    h.tnt_set_tile_config(rt._ctx, -1)
 
     def vendor():
         for o in ops:
@@ -446,7 +450,7 @@ def main():
             hh = N.hip()
             trials = []
             for c in (-1, 1, 6):
-                hh.tnt_set_tile_config(c)
+                hh.tnt_set_tile_config(rt._ctx, c)
                 wl.graph = None
                 wl.capture()
                 for _ in range(3):
@@ -464,7 +468,7 @@ def main():
                     rt._check(hh.tnt_graph_destroy(rt._ctx, g))
             batched_cfg = best[1]
             wl.graph = best[2]
-            hh.tnt_set_tile_config(batched_cfg)  # kept for the event-timed eager region
+            hh.tnt_set_tile_config(rt._ctx, batched_cfg)  # kept for the event-timed eager region
         else:
             wl.capture()
         log(f"[bench] batched: {wl.n_jobs} sector GEMMs, {args.precision} rel. Frobenius error vs FP64 {batched_err:.3e}")
@@ -549,7 +553,8 @@ def main():
     parity = None
     if args.cpu_baseline == "on":
         import numpy as np
-        t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D)
+        t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D,
+                                                       wl=wl if args.workload in ("u1_env", "su2_env") else None)
         if args.workload == "batched":
             nsample = len(out)
             flops_sample = sum(c.flops for *_, c in wl.pairs[:nsample])

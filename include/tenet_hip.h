@@ -68,12 +68,20 @@ enum tnt_status {
   TNT_E_OOM = -7
 };
 
-/* Payload element types.  Buffers are FP64 (like the reference's device
- * subset, tenet-dense/src/cuda_adapter.rs:108-140); TNT_F32 is reserved. */
-enum tnt_dtype { TNT_F64 = 0, TNT_F32 = 1 };
+/* Payload element types: DenseRead / DenseWrite::{F64, F32, C32, C64}
+ * (tenet-dense/src/executor.rs:228-285).  FP64 payloads are accepted by every
+ * entry point (the reference's device subset is FP64,
+ * tenet-dense/src/cuda_adapter.rs:108-140).  FP32 payloads are accepted by
+ * tnt_buf_alloc / _wrap / _upload / _download / _zero and by tnt_gemm_batch
+ * (precision TNT_PREC_F64 or TNT_PREC_F32; the result is rounded to FP32 once);
+ * every other entry point returns TNT_E_UNSUPPORTED for them.  Complex
+ * payloads (TNT_C32, TNT_C64) are rejected with TNT_E_UNSUPPORTED by every
+ * entry point, tnt_buf_alloc and tnt_buf_wrap included. */
+enum tnt_dtype { TNT_F64 = 0, TNT_F32 = 1, TNT_C32 = 2, TNT_C64 = 3 };
 
-/* MatrixOp (tenet-dense/src/executor.rs:24-31); Adjoint == transpose for real data. */
-enum tnt_op { TNT_OP_IDENTITY = 0, TNT_OP_ADJOINT = 1 };
+/* MatrixOp {Identity, Transpose, Adjoint} (tenet-dense/src/executor.rs:24-31),
+ * same order.  On real payloads Transpose and Adjoint are the same operation. */
+enum tnt_op { TNT_OP_IDENTITY = 0, TNT_OP_TRANSPOSE = 1, TNT_OP_ADJOINT = 2 };
 
 /* Payload arithmetic of the block GEMM. */
 enum tnt_precision {
@@ -102,8 +110,9 @@ typedef struct {
 } tnt_gemm_job;
 
 /* One grouped-GEMM job over explicit device pointers (several contractions in
- * one launch).  Column-major with explicit leading dimensions; op applies to
- * the stored matrices. */
+ * one launch).  Column-major with explicit leading dimensions; op (a tnt_op)
+ * applies to the stored matrices.  rows, contracted, cols and the leading
+ * dimensions must lie in [0, 2^31 - 1]. */
 typedef struct {
   double* dst;
   const double* lhs;
@@ -204,7 +213,13 @@ int tnt_buf_info(const tnt_buf* buf, void** device_ptr, size_t* len, int* dtype,
 
 /* ---- batched block GEMM ---------------------------------------------------- */
 /* runs: optional strided-batch run partition (executor.rs:473-488); it is a
- * scheduling hint only and may be NULL. */
+ * scheduling hint only and may be NULL.  lhs_op / rhs_op are tnt_op values
+ * (MatrixOp); any other value is TNT_E_INVALID.  dst, lhs and rhs must share
+ * one payload type: three FP64 payloads (every precision) or three FP32
+ * payloads (TNT_PREC_F64: FP64 arithmetic, result rounded to FP32 once;
+ * TNT_PREC_F32: FP32 MFMA); mixed types are TNT_E_INVALID, split precisions
+ * on FP32 payloads and complex payloads TNT_E_UNSUPPORTED.  Every check runs
+ * before anything is queued. */
 int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
                    const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
                    int lhs_op, int rhs_op, double alpha, double beta, int precision);
@@ -273,24 +288,48 @@ int tnt_ctx_kernel_stats(tnt_ctx* ctx, int kind, size_t* launches, double* total
 int tnt_ctx_reset_stats(tnt_ctx* ctx);
 
 /* ---- tile-shape autotuner --------------------------------------------------- */
-/* Force one GEMM tile configuration for tables built after this call
- * (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128, 4: narrow-output vector kernel,
- * 5: 128x128 on 8 waves, 6: 32x32 per wave with operands in registers (tiny
- * jobs; also takes narrow jobs of <= 256 rows); -1: per-job choice).  Narrow
- * outputs otherwise keep the vector kernel.  Used by the autotuner sweep; no
- * reference counterpart. */
-int tnt_set_tile_config(int cfg);
+/* Force one GEMM tile configuration for the tables this context builds after
+ * the call (0: 128x128, 1: 64x64, 2: 128x64, 3: 64x128, 4: narrow-output
+ * vector kernel, 5: 128x128 on 8 waves, 6: 32x32 per wave with operands in
+ * registers (tiny jobs; also takes narrow jobs of <= 256 rows); -1: per-job
+ * choice).  Narrow outputs otherwise keep the vector kernel.  Used by the
+ * autotuner sweep; no reference counterpart.  Per context, like every knob
+ * below: contexts leased to different threads do not share them. */
+int tnt_set_tile_config(tnt_ctx* ctx, int cfg);
 /* Install the tuned per-shape table: entries (m_max, n_max, cfg) scanned in
  * order, first match wins; n = 0 clears it. */
-int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n);
+int tnt_set_tile_table(tnt_ctx* ctx, const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n);
 
 /* Number of digit planes per operand of TNT_PREC_SPLIT_BF16 and
  * TNT_PREC_SPLIT_I8 (2..6, default 5: relative error ~1e-12 (BF16) and
  * ~1e-11 (INT8) on random data; 4 trades accuracy for speed).  No reference
  * counterpart. */
-int tnt_set_split_slices(int slices);
+int tnt_set_split_slices(tnt_ctx* ctx, int slices);
+
+/* ---- test hooks --------------------------------------------------------------- */
+/* Failure injection for tests of the error paths (no reference counterpart;
+ * the reference's error handling is mirrored, runtime.rs:561-620,
+ * fusion_replay.rs:610-631).  TNT_FAULT_ALLOC: the context's pool allocation
+ * after `after` more successful ones fails with TNT_E_OOM (no HIP call is
+ * made); TNT_FAULT_LAUNCH: the status check after the `after`-th next batch of
+ * kernel launches reports TNT_E_HIP (the kernels themselves are queued
+ * normally); TNT_FAULT_NONE clears both.  after = -1 disarms one kind. */
+enum tnt_fault { TNT_FAULT_NONE = 0, TNT_FAULT_ALLOC = 1, TNT_FAULT_LAUNCH = 2 };
+int tnt_ctx_inject_fault(tnt_ctx* ctx, int kind, int after);
+/* Entries of the context's table cache (content-addressed device copies of
+ * per-call job / region tables) kept before least recently used entries not
+ * pinned by a live graph are evicted (default 512; tests use small caps). */
+int tnt_ctx_set_table_cache_cap(tnt_ctx* ctx, size_t entries);
 
 /* ---- hipGraph batching ----------------------------------------------------- */
+/* Everything queued on the context between begin and end is recorded, not
+ * run.  Every table a recorded call needs must exist before capture (run the
+ * call once first); nothing is allocated or uploaded while capturing.  While
+ * any graph of a context is alive, the memory it recorded is kept: the
+ * context's cached tables used by the graph are pinned, and workspaces that
+ * grow or tables that are rebuilt retire their old memory until the last
+ * graph is destroyed.  Buffers, plans and artifacts the caller owns must
+ * outlive the graphs that use them. */
 int tnt_graph_begin(tnt_ctx* ctx);
 int tnt_graph_end(tnt_ctx* ctx, tnt_graph** out);
 int tnt_graph_launch(tnt_ctx* ctx, const tnt_graph* graph);

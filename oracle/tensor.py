@@ -12,41 +12,116 @@ from .symmetry import make_rule
 MASK64 = (1 << 64) - 1
 
 
-def splitmix_unit(seed, n):
-    """n draws of the reference seeded fill (tenet/src/typed.rs:2086-2092), vectorised.
-
-    state_k = seed + k * 0x9E3779B97F4A7C15; value = mix(state_k);
-    unit = (value >> 11) / 2^52 - 1   in [-1, 1).
-    """
+def splitmix_raw(seed, n):
+    """The first n 64-bit outputs of splitmix64 started at `seed` (the stream of
+    tenet/src/typed.rs:2086-2092): state_k = seed + k * 0x9E3779B97F4A7C15,
+    output = mix(state_k), k = 1..n."""
     k = np.arange(1, n + 1, dtype=np.uint64)
     with np.errstate(over="ignore"):
         z = np.uint64(seed & MASK64) + k * np.uint64(0x9E3779B97F4A7C15)
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def splitmix_unit(seed, n):
+    """n draws of the reference seeded fill (tenet/src/typed.rs:2086-2092), vectorised:
+    unit = (output >> 11) / 2^52 - 1   in [-1, 1).
+    """
+    z = splitmix_raw(seed, n)
     return (z >> np.uint64(11)).astype(np.float64) / float(1 << 52) - 1.0
+
+
+# Constants of the normal fill's own log / sin / cos (mathematics only):
+# ln 2 split so that e * LN2_HI is exact for |e| <= 1074 (low 21 bits of the
+# double nearest ln 2 cleared), the atanh series coefficients 1/(2k+1), and the
+# Taylor coefficients (-1)^k / (2k+1)! and (-1)^k / (2k)! of sin and cos.
+LN2_HI = 0.6931471803691238        # 0x3FE62E42FEE00000
+LN2_LO = 1.9082149292705877e-10    # ln 2 - LN2_HI
+SQRT_HALF = 0.7071067811865476
+PI_HALF = 1.5707963267948966
+ATANH_C = [1.0 / (2 * k + 1) for k in range(1, 12)]   # z^1 .. z^11 of atanh(s)/s - 1
+
+
+def _fact(n):
+    f = 1
+    for i in range(2, n + 1):
+        f *= i
+    return f
+
+
+SIN_C = [(-1) ** k / _fact(2 * k + 1) for k in range(1, 10)]   # z^1 .. z^9 of sin(t)/t - 1
+COS_C = [(-1) ** k / _fact(2 * k) for k in range(1, 10)]       # z^1 .. z^9 of cos(t) - 1
+
+
+def _horner(z, coeffs):
+    """sum_k coeffs[k] z^(k+1) evaluated as z*(c0 + z*(c1 + ...)), one rounding per op."""
+    p = np.full_like(z, coeffs[-1])
+    for c in reversed(coeffs[:-1]):
+        p = p * z + c
+    return p * z
+
+
+def log_unit(u):
+    """ln(u) for u in (0, 1] from +, -, *, / only (no fused multiply-add).
+
+    u = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1),
+    |s| < 0.172, series to s^23 (truncation < 2^-60 relative); then
+    ln u = e LN2_HI + (ln m + e LN2_LO).  Every operation is a correctly rounded
+    IEEE-754 double operation, so the device fill evaluating the same sequence
+    (with contraction off) gives the same bits.  Accuracy: within 2 ulp of ln.
+    """
+    m, e = np.frexp(u)
+    low = m < SQRT_HALF
+    m = np.where(low, m * 2.0, m)
+    e = np.where(low, e - 1, e).astype(np.float64)
+    f = m - 1.0
+    s = f / (m + 1.0)
+    s2 = s * 2.0
+    lnm = s2 + s2 * _horner(s * s, ATANH_C)
+    return e * LN2_HI + (lnm + e * LN2_LO)
+
+
+def sincos_2pi(t):
+    """(sin 2 pi t, cos 2 pi t) for t in (0, 1] from +, -, *, rint only.
+
+    q = rint(4t), x = 4t - q in [-1/2, 1/2] (exact), a = x pi/2 in [-pi/4, pi/4];
+    Taylor series of sin and cos to a^19 / a^18 (truncation < 2^-60); the
+    quadrant q mod 4 rotates (sin a, cos a) by exact sign/swap.
+    """
+    t4 = t * 4.0
+    q = np.rint(t4)
+    x = t4 - q
+    a = x * PI_HALF
+    z = a * a
+    sa = a + a * _horner(z, SIN_C)
+    ca = 1.0 + _horner(z, COS_C)
+    qm = np.mod(q, 4.0)
+    sin = np.where(qm == 0, sa, np.where(qm == 1, ca, np.where(qm == 2, -sa, -ca)))
+    cos = np.where(qm == 0, ca, np.where(qm == 1, -sa, np.where(qm == 2, -ca, sa)))
+    return sin, cos
 
 
 def splitmix_normal(seed, n):
     """normal(0,1) by Box-Muller on consecutive draw pairs of the same stream.
 
-    Elements 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0, 1]:
-    r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).  (The reference has no
-    normal fill; this definition serves the synthetic workloads.)
+    Elements 2p and 2p+1 use draws (2p+1, 2p+2) of the seeded splitmix64
+    stream (tenet/src/typed.rs:2086-2092) mapped to u = ((z >> 11) + 1) 2^-53
+    in (0, 1]:  r = sqrt(-2 ln u1),  (r cos 2 pi u2, r sin 2 pi u2).  ln, sin and
+    cos are this module's log_unit / sincos_2pi, so that the device fill
+    (tnt_buf_fill_random with normal != 0) is bit-identical.  The reference has
+    no normal fill; this definition serves the synthetic workloads.
     """
     m = 2 * ((n + 1) // 2)
-    k = np.arange(1, m + 1, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed & MASK64) + k * np.uint64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
+    z = splitmix_raw(seed, m)
     u = ((z >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0 ** -53
     u1, u2 = u[0::2], u[1::2]
-    r = np.sqrt(-2.0 * np.log(u1))
+    r = np.sqrt(log_unit(u1) * -2.0)
+    sn, cs = sincos_2pi(u2)
     out = np.empty(m)
-    out[0::2] = r * np.cos(2.0 * np.pi * u2)
-    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    out[0::2] = r * cs
+    out[1::2] = r * sn
     return out[:n]
 
 

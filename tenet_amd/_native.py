@@ -18,7 +18,9 @@ ERRORS = {-1: "INVALID", -2: "UNSUPPORTED", -3: "SHAPE", -4: "OOB", -5: "DEVICE_
           -6: "HIP", -7: "OOM"}
 
 PREC_F64, PREC_F32, PREC_SPLIT_BF16, PREC_SPLIT_I8 = 0, 1, 2, 3
-OP_IDENTITY, OP_ADJOINT = 0, 1
+OP_IDENTITY, OP_TRANSPOSE, OP_ADJOINT = 0, 1, 2   # MatrixOp order (tenet-dense/src/executor.rs:24-31)
+F64, F32, C32, C64 = 0, 1, 2, 3                     # DenseRead / DenseWrite payload types
+FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH = 0, 1, 2
 RULES = {"U1": 0, "SU2": 1, "Z2": 2, "U1xZ2": 3}
 
 
@@ -113,10 +115,12 @@ HIP_API = {
     "tnt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
     "tnt_ctx_kernel_stats": (C.c_int, [VP, C.c_int, SZP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnt_ctx_reset_stats": (C.c_int, [VP]),
-    "tnt_set_tile_config": (C.c_int, [C.c_int]),
-    "tnt_set_split_slices": (C.c_int, [C.c_int]),
-    "tnt_set_tile_table": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+    "tnt_set_tile_config": (C.c_int, [VP, C.c_int]),
+    "tnt_set_split_slices": (C.c_int, [VP, C.c_int]),
+    "tnt_set_tile_table": (C.c_int, [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int32), SZ]),
+    "tnt_ctx_inject_fault": (C.c_int, [VP, C.c_int, C.c_int]),
+    "tnt_ctx_set_table_cache_cap": (C.c_int, [VP, SZ]),
     "tnt_graph_begin": (C.c_int, [VP]),
     "tnt_graph_end": (C.c_int, [VP, PP]),
     "tnt_graph_launch": (C.c_int, [VP, VP]),

@@ -19,9 +19,14 @@ HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/snapshot.cpp", "csrc/host/host
 HIP_SRC = ["csrc/hip/backend.hip", "csrc/hip/xform.hip", "csrc/hip/gemm.hip",
            "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip", "csrc/common/narrow_fusion.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
-             # no packed FP32 instructions anywhere in device code
+             # no packed FP32 instructions anywhere in device code.  hipcc has
+             # no device-only form of a cc1 target feature (-Xarch_device does
+             # not forward -Xclang), so the host pass sees it too and warns once
+             # per function that x86 does not know it; those lines are dropped
+             # from the compiler output below so that real warnings stay visible.
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
+_HOST_PASS_NOISE = ("'-packed-fp32-ops' is not a recognized feature for this target",)
 
 
 def _run(cmd):
@@ -42,18 +47,27 @@ def build_hip(jobs=8):
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
+
+    def finish(p):
+        _, err = p.communicate()
+        lines = [ln for ln in err.splitlines() if not any(n in ln for n in _HOST_PASS_NOISE)]
+        lines = [ln for ln in lines if not (ln.strip().endswith("warnings generated.") and not any(
+            "warning:" in x for x in lines))]
+        if lines:
+            print("\n".join(lines), file=sys.stderr, flush=True)
+        if p.returncode != 0:
+            raise RuntimeError("hipcc failed")
+
     for src in HIP_SRC:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
         cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
         print(" ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd, cwd=HERE))
+        procs.append(subprocess.Popen(cmd, cwd=HERE, stderr=subprocess.PIPE, text=True))
         if len(procs) >= jobs:
-            if procs.pop(0).wait() != 0:
-                raise RuntimeError("hipcc failed")
+            finish(procs.pop(0))
     for p in procs:
-        if p.wait() != 0:
-            raise RuntimeError("hipcc failed")
+        finish(p)
     out = os.path.join(LIB, "libtenet_hip.so")
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out] + objs)
     return out

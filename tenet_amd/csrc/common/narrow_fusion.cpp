@@ -136,7 +136,7 @@ bool plan_narrow_fusion(const tnt_contract_desc* d, NarrowFusion& f) {
   if (d->out && (!single_only(d->out) || d->n_uncovered))
     return fail(f, "output transform has Multi blocks, or uncovered core regions");
   if (d->rhs_len > kMaxRhs) return fail(f, "rhs too large to map on the host");
-  const bool rhs_adj = d->rhs_op == TNT_OP_ADJOINT;
+  const bool rhs_adj = d->rhs_op == TNT_OP_TRANSPOSE || d->rhs_op == TNT_OP_ADJOINT;  // real payloads
 
   std::vector<Job> jobs;
   for (size_t i = 0; i < d->n_jobs; ++i) {

@@ -29,7 +29,26 @@ using namespace tnt_dev;
 struct TableCacheEntry {
   void* dev = nullptr;
   size_t bytes = 0;
+  uint64_t last_call = 0;  // the entry-point call that last returned it
+  int pins = 0;            // live graphs that recorded its address
+  std::list<std::string>::iterator lru;  // position in tnt_ctx::table_lru (front = most recent)
 };
+
+// Tuned per-shape tile rule (tnt_set_tile_table).
+struct TileRule {
+  int64_t m_max, n_max;
+  int32_t cfg;
+};
+
+// The context's GEMM tiling knobs, read when a table is built.
+struct TileKnobs {
+  int override_cfg = -1;         // tnt_set_tile_config (autotuner sweeps)
+  std::vector<TileRule> table;   // tnt_set_tile_table
+};
+
+// Entries of the table cache beyond which unpinned entries not used by the
+// current call are evicted, least recently used first.
+constexpr size_t kTableCacheCap = 512;
 
 struct tnt_ctx {
   int device = 0;
@@ -42,6 +61,18 @@ struct tnt_ctx {
   // content-addressed device copies of small per-call tables
   std::unordered_map<std::string, TableCacheEntry> tables;
   std::list<std::string> table_lru;
+  size_t table_cap = kTableCacheCap;
+  uint64_t call = 0;  // incremented by every entry point that uses the cache
+  // graphs alive; while any is, memory they may have recorded is retired
+  // instead of freed (released when the last graph is destroyed)
+  int live_graphs = 0;
+  std::vector<void*> retired;
+  std::vector<std::string> capture_keys;  // cache entries returned during the open capture
+  TileKnobs knobs;
+  int split_slices = 5;
+  // test hook (tnt_ctx_inject_fault): countdowns to an injected failure, -1 = off
+  int fail_alloc_after = -1;
+  int fail_launch_after = -1;
   // pinned host scalar for reductions
   double* host_scalar = nullptr;
   double* dev_scalar = nullptr;
@@ -188,6 +219,7 @@ static bool fusion_enabled() {
 struct tnt_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<std::string> pinned;  // cache entries this graph recorded
 };
 
 #define FAIL(ctx, code, msg)        \
@@ -210,9 +242,20 @@ static int set_device(tnt_ctx* ctx) {
   return TNT_OK;
 }
 
+// Status of the launches just queued: the runtime's launch error, or an
+// injected one (tnt_ctx_inject_fault).
+static int launch_status(tnt_ctx* ctx) {
+  if (ctx->fail_launch_after >= 0 && ctx->fail_launch_after-- == 0)
+    FAIL(ctx, TNT_E_HIP, "injected launch failure (tnt_ctx_inject_fault)");
+  HIPCHK(ctx, hipGetLastError());
+  return TNT_OK;
+}
+
 // Pooled device allocation: reuse a free block of at least `bytes` and at
 // most 2x that; never frees while a graph is being captured.
 static int pool_alloc(tnt_ctx* ctx, size_t bytes, void** out) {
+  if (ctx->fail_alloc_after >= 0 && ctx->fail_alloc_after-- == 0)
+    FAIL(ctx, TNT_E_OOM, "injected allocation failure (tnt_ctx_inject_fault)");
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~size_t(255);
   auto it = ctx->free_blocks.lower_bound(bytes);
@@ -254,6 +297,14 @@ static void pool_free(tnt_ctx* ctx, void* p) {
   ctx->live.erase(it);
 }
 
+// Release of context-owned memory that a live graph may have recorded: kept
+// until the last graph of the context is destroyed.
+static void retire(tnt_ctx* ctx, void* p) {
+  if (!p) return;
+  if (ctx->live_graphs > 0) ctx->retired.push_back(p);
+  else pool_free(ctx, p);
+}
+
 template <class T>
 static int upload_vec(tnt_ctx* ctx, const std::vector<T>& v, T** dev, std::vector<void*>& allocs) {
   void* p = nullptr;
@@ -269,13 +320,37 @@ static int upload_vec(tnt_ctx* ctx, const std::vector<T>& v, T** dev, std::vecto
   return TNT_OK;
 }
 
+// Starts an entry-point call that looks tables up in the cache: entries it
+// returns are not evicted before the call has queued its kernels.
+static void begin_call(tnt_ctx* ctx) { ++ctx->call; }
+
+// Evicts least recently used entries while the cache is over its cap,
+// skipping entries pinned by a live graph or returned during the current
+// call.  Freed blocks go back to the stream-ordered pool: kernels queued
+// earlier on the context stream still read them before any later reuse.
+static void trim_tables(tnt_ctx* ctx) {
+  auto it = ctx->table_lru.end();
+  while (ctx->tables.size() > ctx->table_cap && it != ctx->table_lru.begin()) {
+    --it;
+    auto e = ctx->tables.find(*it);
+    if (e == ctx->tables.end() || e->second.pins > 0 || e->second.last_call == ctx->call) continue;
+    pool_free(ctx, e->second.dev);
+    ctx->tables.erase(e);
+    it = ctx->table_lru.erase(it);
+  }
+}
+
 // Content-addressed device copy of a small table (job lists, region lists).
 static int cached_table(tnt_ctx* ctx, const void* data, size_t bytes, const char* tag, void** dev) {
   std::string key(tag);
   key.append(static_cast<const char*>(data), bytes);
   auto it = ctx->tables.find(key);
   if (it != ctx->tables.end()) {
-    *dev = it->second.dev;
+    TableCacheEntry& e = it->second;
+    e.last_call = ctx->call;
+    ctx->table_lru.splice(ctx->table_lru.begin(), ctx->table_lru, e.lru);
+    if (ctx->capturing) ctx->capture_keys.push_back(key);
+    *dev = e.dev;
     return TNT_OK;
   }
   if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "new job table requested during graph capture; run the call once before capturing");
@@ -284,18 +359,14 @@ static int cached_table(tnt_ctx* ctx, const void* data, size_t bytes, const char
   if (rc) return rc;
   HIPCHK(ctx, hipMemcpyAsync(p, data, bytes, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->tables[key] = {p, bytes};
-  ctx->table_lru.push_back(key);
-  if (ctx->table_lru.size() > 512) {  // bounded cache; the stream is drained before reuse
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    const std::string& old = ctx->table_lru.front();
-    auto o = ctx->tables.find(old);
-    if (o != ctx->tables.end()) {
-      pool_free(ctx, o->second.dev);
-      ctx->tables.erase(o);
-    }
-    ctx->table_lru.pop_front();
-  }
+  ctx->table_lru.push_front(key);
+  TableCacheEntry e;
+  e.dev = p;
+  e.bytes = bytes;
+  e.last_call = ctx->call;
+  e.lru = ctx->table_lru.begin();
+  ctx->tables.emplace(std::move(key), e);
+  trim_tables(ctx);
   *dev = p;
   return TNT_OK;
 }
@@ -614,16 +685,8 @@ static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double*
 // ---------------------------------------------------------------------------
 // GEMM tables + tile-shape autotuner
 // ---------------------------------------------------------------------------
-// Tile shape per job: measured preference table (filled by tnt_autotune via
-// the environment-independent heuristic below when no measurement exists).
-struct TileRule {
-  int64_t m_max, n_max;
-  int32_t cfg;
-};
-static std::vector<TileRule> tile_table;  // tuned per-shape rules (tnt_set_tile_table)
-
-static int tile_override = -1;  // set by tnt_set_tile_config (autotuner sweeps)
-
+// Tile shape per job: the context's measured preference table (installed by
+// the autotuner), else the size rule below.
 // Table-level default: the largest tile shape that still gives every CU at
 // least ~2 (128x128) or 1 (64x128 / 128x64) workgroups and covers the jobs
 // with little padding (>= 80 % of its area used), counted over the jobs that
@@ -652,9 +715,9 @@ static int default_family(const std::vector<GJob>& jobs) {
   return TILE_64x64;
 }
 
-static int job_cfg(const GJob& j, int family) {
+static int job_cfg(const GJob& j, int family, const TileKnobs& kn) {
   if (j.n <= 16) return TILE_SKINNY;
-  for (const auto& r : tile_table)
+  for (const auto& r : kn.table)
     if (j.m <= r.m_max && j.n <= r.n_max) return r.cfg;
   int cfg = family;
   // keep narrow dimensions off oversized tiles
@@ -665,7 +728,7 @@ static int job_cfg(const GJob& j, int family) {
   return cfg;
 }
 
-static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int forced = -1) {
+static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const TileKnobs& kn, int forced = -1) {
   t.host_jobs = jobs;
   std::vector<std::vector<GTile>> per(TILE_CFG_COUNT);
   // larger jobs first for load balance
@@ -675,17 +738,20 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, int fo
     double ca = double(jobs[a].m) * jobs[a].n * jobs[a].k, cb = double(jobs[b].m) * jobs[b].n * jobs[b].k;
     return ca > cb;
   });
-  const int ovr = forced >= 0 ? forced : tile_override;
+  const int ovr = forced >= 0 ? forced : kn.override_cfg;
   const int family = default_family(jobs);
   for (size_t oi : order) {
     const GJob& j = jobs[oi];
     if (j.m <= 0 || j.n <= 0) continue;
-    int cfg = job_cfg(j, family);
+    int cfg = job_cfg(j, family, kn);
     // a forced shape leaves narrow outputs on the vector kernel, except that
     // forced one-wave tiles also take narrow jobs of few rows (one launch for
     // a batch of tiny GEMMs instead of two)
     if (ovr >= 0 && (cfg != TILE_SKINNY || (ovr == TILE_SMALL && j.m <= 256))) cfg = ovr;
     if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
+    // nothing contracted: D = beta D, written by the workgroup-tile epilogue
+    // (that kernel loads no operand element when k == 0)
+    if (j.k == 0) cfg = TILE_64x64;
     int bm = tile_rows(cfg), bn = tile_cols(cfg);
     int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
     for (int a = 0; a < tm; ++a)
@@ -728,9 +794,21 @@ static double table_flops(const GemmTable& t) {
   return f;
 }
 
+// storage: payload type of d, a and b (TNT_F64, or TNT_F32 for FP64 / FP32
+// arithmetic only; the caller has checked the combination).
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
-                          double alpha, double beta, int precision) {
+                          double alpha, double beta, int precision, int storage = TNT_F64) {
   TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
+  if (storage == TNT_F32) {
+    for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+      if (!t.n_tiles_cfg[c]) continue;
+      GemmArgs g{t.jobs, t.tiles + t.tile_start_cfg[c], d, a, b, alpha, beta};
+      TimedScope tk(ctx, timing_kind_kernel(precision, c), ctx->timing ? t.flops_cfg[c] : 0.0);
+      if (precision == TNT_PREC_F32) launch_gemm_f32_on_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
+      else launch_gemm_f64_on_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
+    }
+    return launch_status(ctx);
+  }
   if (precision == TNT_PREC_SPLIT_BF16 || precision == TNT_PREC_SPLIT_I8) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
       if (!t.n_narrow_cfg[c]) continue;
@@ -741,11 +819,10 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
     }
     if (!ctx->split) ctx->split.reset(split_workspace_create());
     const int scheme = precision == TNT_PREC_SPLIT_I8 ? SPLIT_SCHEME_I8 : SPLIT_SCHEME_BF16;
-    int rc = split_gemm_run(ctx->split.get(), scheme, t.split_jobs, t.jobs, t.host_tiles, t.tiles, d, a, b,
-                            alpha, beta, ctx->stream, ctx->capturing, ctx->err);
+    int rc = split_gemm_run(ctx->split.get(), scheme, ctx->split_slices, t.split_jobs, t.jobs, d, a, b, alpha, beta,
+                            ctx->stream, ctx->capturing, ctx->live_graphs > 0, ctx->err);
     if (rc) return rc;
-    HIPCHK(ctx, hipGetLastError());
-    return TNT_OK;
+    return launch_status(ctx);
   }
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
     if (!t.n_tiles_cfg[c]) continue;
@@ -754,8 +831,7 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
     if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
     else launch_gemm_f64(g, c, t.n_tiles_cfg[c], ctx->stream);
   }
-  HIPCHK(ctx, hipGetLastError());
-  return TNT_OK;
+  return launch_status(ctx);
 }
 
 static bool job_in_bounds(const GJob& j, size_t dlen, size_t alen, size_t blen) {
@@ -771,6 +847,45 @@ static bool job_in_bounds(const GJob& j, size_t dlen, size_t alen, size_t blen) 
   if (j.d_off < 0 || j.a_off < 0 || j.b_off < 0) return false;
   return span(j.d_off, j.m, j.n, j.ldd) <= (int64_t)dlen && span(j.a_off, ar, ac, j.lda) <= (int64_t)alen &&
          span(j.b_off, br, bc, j.ldb) <= (int64_t)blen;
+}
+
+// MatrixOp values (Identity, Transpose, Adjoint); on real payloads both
+// non-identity ops transpose the stored matrix.
+static bool valid_op(int op) { return op == TNT_OP_IDENTITY || op == TNT_OP_TRANSPOSE || op == TNT_OP_ADJOINT; }
+static bool op_transposes(int op) { return op == TNT_OP_TRANSPOSE || op == TNT_OP_ADJOINT; }
+
+static bool valid_precision(int p) { return p >= TNT_PREC_F64 && p <= TNT_PREC_SPLIT_I8; }
+
+// Payload type check of tnt_buf_alloc / tnt_buf_wrap.
+static int check_dtype(tnt_ctx* ctx, int dtype) {
+  if (dtype == TNT_F64 || dtype == TNT_F32) return TNT_OK;
+  if (dtype == TNT_C32 || dtype == TNT_C64) FAIL(ctx, TNT_E_UNSUPPORTED, "complex payloads (C32, C64) are not supported");
+  FAIL(ctx, TNT_E_INVALID, "unknown payload type");
+}
+
+static size_t dtype_bytes(int dtype) { return dtype == TNT_F32 ? 4 : 8; }
+
+// The job of a DenseGemmBatchJob against whole flat payloads (ld = row count
+// of the stored operand).  Dimensions must fit the kernels' 32-bit indices.
+static int make_seam_job(tnt_ctx* ctx, const tnt_gemm_job& j, int lhs_op, int rhs_op, GJob* out) {
+  if (j.rows > INT32_MAX || j.cols > INT32_MAX || j.contracted > INT32_MAX)
+    FAIL(ctx, TNT_E_UNSUPPORTED, "job dimension exceeds 2^31 - 1");
+  if (j.dst_offset > (size_t)INT64_MAX || j.lhs_offset > (size_t)INT64_MAX || j.rhs_offset > (size_t)INT64_MAX)
+    FAIL(ctx, TNT_E_OOB, "job offset exceeds 2^63 - 1");
+  GJob g{};
+  g.d_off = (int64_t)j.dst_offset;
+  g.a_off = (int64_t)j.lhs_offset;
+  g.b_off = (int64_t)j.rhs_offset;
+  g.m = (int32_t)j.rows;
+  g.k = (int32_t)j.contracted;
+  g.n = (int32_t)j.cols;
+  const bool la = op_transposes(lhs_op), rb = op_transposes(rhs_op);
+  g.lda = la ? std::max(g.k, 1) : std::max(g.m, 1);
+  g.ldb = rb ? std::max(g.n, 1) : std::max(g.k, 1);
+  g.ldd = std::max(g.m, 1);
+  g.ops = (la ? 1 : 0) | (rb ? 2 : 0);
+  *out = g;
+  return TNT_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -801,6 +916,8 @@ int tnt_ctx_destroy(tnt_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->split.reset();
+  for (auto p : ctx->retired) pool_free(ctx, p);
+  ctx->retired.clear();
   for (auto& kv : ctx->tables) pool_free(ctx, kv.second.dev);
   for (auto& kv : ctx->live) (void)hipFree(kv.first);
   for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
@@ -838,15 +955,15 @@ int tnt_ctx_pool_bytes(const tnt_ctx* ctx, size_t* bytes) {
 
 int tnt_buf_alloc(tnt_ctx* ctx, size_t len, int dtype, tnt_buf** out) {
   if (!ctx || !out) return TNT_E_INVALID;
-  if (dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, "payload buffers are FP64");
+  if (int rc = check_dtype(ctx, dtype)) return rc;
+  if (len > (SIZE_MAX >> 4)) FAIL(ctx, TNT_E_OOM, "payload length too large");
   if (int rc = set_device(ctx)) return rc;
   auto* b = new tnt_buf;
   b->len = len;
   b->dtype = dtype;
   b->device = ctx->device;
   b->owned = true;
-  size_t es = dtype == TNT_F64 ? 8 : 4;
-  int rc = pool_alloc(ctx, len * es, &b->ptr);
+  int rc = pool_alloc(ctx, len * dtype_bytes(dtype), &b->ptr);
   if (rc) {
     delete b;
     return rc;
@@ -857,7 +974,7 @@ int tnt_buf_alloc(tnt_ctx* ctx, size_t len, int dtype, tnt_buf** out) {
 
 int tnt_buf_wrap(tnt_ctx* ctx, void* ptr, size_t len, int dtype, tnt_buf** out) {
   if (!ctx || !out || (!ptr && len)) return TNT_E_INVALID;
-  if (dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, "payload buffers are FP64");
+  if (int rc = check_dtype(ctx, dtype)) return rc;
   hipPointerAttribute_t attr;
   if (ptr && hipPointerGetAttributes(&attr, ptr) == hipSuccess) {
     if (attr.type == hipMemoryTypeDevice && attr.device != ctx->device)
@@ -888,15 +1005,24 @@ int tnt_buf_free(tnt_ctx* ctx, tnt_buf* buf) {
 static int check_buf(tnt_ctx* ctx, const tnt_buf* b, const char* what) {
   if (!b) FAIL(ctx, TNT_E_INVALID, std::string("null buffer: ") + what);
   if (b->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, std::string(what) + " lives on another device");
-  if (b->dtype != TNT_F64) FAIL(ctx, TNT_E_UNSUPPORTED, std::string(what) + " must be an FP64 payload");
+  if (b->dtype != TNT_F64)
+    FAIL(ctx, TNT_E_UNSUPPORTED, std::string(what) + " must be an FP64 payload (FP32 payloads: tnt_gemm_batch only)");
+  return TNT_OK;
+}
+
+// A buffer of any supported payload type on the context's device.
+static int check_any_buf(tnt_ctx* ctx, const tnt_buf* b, const char* what) {
+  if (!b) FAIL(ctx, TNT_E_INVALID, std::string("null buffer: ") + what);
+  if (b->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, std::string(what) + " lives on another device");
   return TNT_OK;
 }
 
 int tnt_buf_upload(tnt_ctx* ctx, tnt_buf* buf, const void* host, size_t len) {
   if (!ctx || !buf || (!host && len)) return TNT_E_INVALID;
   if (len > buf->len) FAIL(ctx, TNT_E_SHAPE, "upload longer than buffer");
+  if (int rc = check_any_buf(ctx, buf, "buf")) return rc;
   if (int rc = set_device(ctx)) return rc;
-  size_t es = buf->dtype == TNT_F64 ? 8 : 4;
+  size_t es = dtype_bytes(buf->dtype);
   HIPCHK(ctx, hipMemcpyAsync(buf->ptr, host, len * es, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return TNT_OK;
@@ -905,8 +1031,9 @@ int tnt_buf_upload(tnt_ctx* ctx, tnt_buf* buf, const void* host, size_t len) {
 int tnt_buf_download(tnt_ctx* ctx, const tnt_buf* buf, void* host, size_t len) {
   if (!ctx || !buf || (!host && len)) return TNT_E_INVALID;
   if (len > buf->len) FAIL(ctx, TNT_E_SHAPE, "download longer than buffer");
+  if (int rc = check_any_buf(ctx, buf, "buf")) return rc;
   if (int rc = set_device(ctx)) return rc;
-  size_t es = buf->dtype == TNT_F64 ? 8 : 4;
+  size_t es = dtype_bytes(buf->dtype);
   HIPCHK(ctx, hipMemcpyAsync(host, buf->ptr, len * es, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return TNT_OK;
@@ -914,8 +1041,10 @@ int tnt_buf_download(tnt_ctx* ctx, const tnt_buf* buf, void* host, size_t len) {
 
 int tnt_buf_zero(tnt_ctx* ctx, tnt_buf* buf) {
   if (!ctx || !buf) return TNT_E_INVALID;
+  if (int rc = check_any_buf(ctx, buf, "buf")) return rc;
   if (int rc = set_device(ctx)) return rc;
-  launch_fill(static_cast<double*>(buf->ptr), (int64_t)buf->len, 0.0, ctx->stream);
+  if (buf->dtype == TNT_F32) launch_zero_f32(static_cast<float*>(buf->ptr), (int64_t)buf->len, ctx->stream);
+  else launch_fill(static_cast<double*>(buf->ptr), (int64_t)buf->len, 0.0, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
   return TNT_OK;
 }
@@ -939,41 +1068,42 @@ int tnt_buf_info(const tnt_buf* buf, void** ptr, size_t* len, int* dtype, int* d
   return TNT_OK;
 }
 
+// Validation shared by tnt_gemm_batch and its allocation-free variant: every
+// check before anything is queued; fills `gj`.
+static int gemm_batch_jobs(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                           const tnt_gemm_job* jobs, size_t n_jobs, int lhs_op, int rhs_op, int precision,
+                           std::vector<GJob>& gj) {
+  if (int rc = check_any_buf(ctx, dst, "dst")) return rc;
+  if (int rc = check_any_buf(ctx, lhs, "lhs")) return rc;
+  if (int rc = check_any_buf(ctx, rhs, "rhs")) return rc;
+  if (dst->dtype != lhs->dtype || dst->dtype != rhs->dtype)
+    FAIL(ctx, TNT_E_INVALID, "dst, lhs and rhs payload types differ");
+  if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
+  if (!valid_op(lhs_op) || !valid_op(rhs_op)) FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
+  if (!valid_precision(precision)) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (dst->dtype == TNT_F32 && precision != TNT_PREC_F64 && precision != TNT_PREC_F32)
+    FAIL(ctx, TNT_E_UNSUPPORTED, "split precisions need FP64 payloads");
+  gj.resize(n_jobs);
+  for (size_t i = 0; i < n_jobs; ++i) {
+    if (int rc = make_seam_job(ctx, jobs[i], lhs_op, rhs_op, &gj[i])) return rc;
+    if (!job_in_bounds(gj[i], dst->len, lhs->len, rhs->len))
+      FAIL(ctx, TNT_E_OOB, "GEMM job addresses elements outside its buffers");
+  }
+  return TNT_OK;
+}
+
 int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
                    const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
                    int lhs_op, int rhs_op, double alpha, double beta, int precision) {
   (void)runs;
   (void)n_runs;
   if (!ctx) return TNT_E_INVALID;
-  if (int rc = check_buf(ctx, dst, "dst")) return rc;
-  if (int rc = check_buf(ctx, lhs, "lhs")) return rc;
-  if (int rc = check_buf(ctx, rhs, "rhs")) return rc;
-  if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
-  if ((lhs_op != TNT_OP_IDENTITY && lhs_op != TNT_OP_ADJOINT) || (rhs_op != TNT_OP_IDENTITY && rhs_op != TNT_OP_ADJOINT))
-    FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
-  std::vector<GJob> gj(n_jobs);
-  for (size_t i = 0; i < n_jobs; ++i) {
-    const tnt_gemm_job& j = jobs[i];
-    if (j.rows > INT32_MAX || j.cols > INT32_MAX || j.contracted > INT32_MAX)
-      FAIL(ctx, TNT_E_UNSUPPORTED, "job dimension exceeds 2^31");
-    GJob g{};
-    g.d_off = (int64_t)j.dst_offset;
-    g.a_off = (int64_t)j.lhs_offset;
-    g.b_off = (int64_t)j.rhs_offset;
-    g.m = (int32_t)j.rows;
-    g.k = (int32_t)j.contracted;
-    g.n = (int32_t)j.cols;
-    g.lda = lhs_op == TNT_OP_ADJOINT ? std::max(g.k, 1) : std::max(g.m, 1);
-    g.ldb = rhs_op == TNT_OP_ADJOINT ? std::max(g.n, 1) : std::max(g.k, 1);
-    g.ldd = std::max(g.m, 1);
-    g.ops = (lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (rhs_op == TNT_OP_ADJOINT ? 2 : 0);
-    if (!job_in_bounds(g, dst->len, lhs->len, rhs->len)) FAIL(ctx, TNT_E_OOB, "GEMM job addresses elements outside its buffers");
-    gj[i] = g;
-  }
+  std::vector<GJob> gj;
+  if (int rc = gemm_batch_jobs(ctx, dst, lhs, rhs, jobs, n_jobs, lhs_op, rhs_op, precision, gj)) return rc;
   if (int rc = set_device(ctx)) return rc;
+  begin_call(ctx);
   GemmTable t;
-  build_gemm_table(t, gj);
+  build_gemm_table(t, gj, ctx->knobs);
   void* dj = nullptr;
   void* dt = nullptr;
   int rc = cached_table(ctx, t.host_jobs.data(), sizeof(GJob) * std::max<size_t>(t.host_jobs.size(), 1), "J", &dj);
@@ -981,19 +1111,24 @@ int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf
   if (rc) return rc;
   t.jobs = static_cast<GJob*>(dj);
   t.tiles = static_cast<GTile*>(dt);
-  return run_gemm_table(ctx, t, (uintptr_t)dst->ptr, (uintptr_t)lhs->ptr, (uintptr_t)rhs->ptr, alpha, beta, precision);
+  return run_gemm_table(ctx, t, (uintptr_t)dst->ptr, (uintptr_t)lhs->ptr, (uintptr_t)rhs->ptr, alpha, beta,
+                        precision, dst->dtype);
 }
 
 int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, double alpha,
                      double beta, int precision) {
   if (!ctx) return TNT_E_INVALID;
   if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (!valid_precision(precision)) FAIL(ctx, TNT_E_INVALID, "unknown precision");
   std::vector<GJob> gj(n_jobs);
   for (size_t i = 0; i < n_jobs; ++i) {
     const tnt_gemm_ptr_job& j = jobs[i];
     if (!j.dst || !j.lhs || !j.rhs || j.rows < 0 || j.cols < 0 || j.contracted < 0)
       FAIL(ctx, TNT_E_INVALID, "invalid grouped job");
+    if (!valid_op(j.lhs_op) || !valid_op(j.rhs_op)) FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
+    if (j.rows > INT32_MAX || j.cols > INT32_MAX || j.contracted > INT32_MAX || j.ldd > INT32_MAX ||
+        j.ldl > INT32_MAX || j.ldr > INT32_MAX)
+      FAIL(ctx, TNT_E_UNSUPPORTED, "grouped job dimension or leading dimension exceeds 2^31 - 1");
     if (((uintptr_t)j.dst | (uintptr_t)j.lhs | (uintptr_t)j.rhs) & 7)
       FAIL(ctx, TNT_E_INVALID, "grouped job pointers must be 8-byte aligned");
     GJob g{};
@@ -1006,15 +1141,16 @@ int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, 
     g.lda = (int32_t)j.ldl;
     g.ldb = (int32_t)j.ldr;
     g.ldd = (int32_t)j.ldd;
-    g.ops = (j.lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (j.rhs_op == TNT_OP_ADJOINT ? 2 : 0);
+    g.ops = (op_transposes(j.lhs_op) ? 1 : 0) | (op_transposes(j.rhs_op) ? 2 : 0);
     int64_t ar = (g.ops & 1) ? g.k : g.m, br = (g.ops & 2) ? g.n : g.k;
     if (g.lda < std::max<int64_t>(1, ar) || g.ldb < std::max<int64_t>(1, br) || g.ldd < std::max<int32_t>(1, g.m))
       FAIL(ctx, TNT_E_SHAPE, "leading dimension smaller than the row count");
     gj[i] = g;
   }
   if (int rc = set_device(ctx)) return rc;
+  begin_call(ctx);
   GemmTable t;
-  build_gemm_table(t, gj);
+  build_gemm_table(t, gj, ctx->knobs);
   void* dj = nullptr;
   void* dt = nullptr;
   int rc = cached_table(ctx, t.host_jobs.data(), sizeof(GJob) * std::max<size_t>(t.host_jobs.size(), 1), "PJ", &dj);
@@ -1050,12 +1186,21 @@ int tnt_tree_transform(tnt_ctx* ctx, const tnt_plan* plan, tnt_buf* dst, const t
   if (dst->ptr == src->ptr && plan->dst_len > 0) FAIL(ctx, TNT_E_INVALID, "in-place tree transform is not supported");
   if (int rc = set_device(ctx)) return rc;
   run_plan(ctx, plan, static_cast<double*>(dst->ptr), static_cast<const double*>(src->ptr), alpha, beta, overwrite);
-  HIPCHK(ctx, hipGetLastError());
-  return TNT_OK;
+  return launch_status(ctx);
 }
 
 int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact** out) {
   if (!ctx || !d || !out) return TNT_E_INVALID;
+  if (!valid_op(d->lhs_op) || !valid_op(d->rhs_op)) FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
+  if (d->n_jobs && !d->jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
+  if (d->n_uncovered && !d->uncovered) FAIL(ctx, TNT_E_INVALID, "null uncovered-region list");
+  if (d->lhs_len < 0 || d->rhs_len < 0 || d->core_len < 0) FAIL(ctx, TNT_E_SHAPE, "negative operand length");
+  std::vector<GJob> gj(d->n_jobs);
+  for (size_t i = 0; i < d->n_jobs; ++i) {
+    if (int rc = make_seam_job(ctx, d->jobs[i], d->lhs_op, d->rhs_op, &gj[i])) return rc;
+    if (!job_in_bounds(gj[i], (size_t)d->core_len, (size_t)d->lhs_len, (size_t)d->rhs_len))
+      FAIL(ctx, TNT_E_OOB, "core GEMM job addresses elements outside its operands");
+  }
   if (int rc = set_device(ctx)) return rc;
   auto* a = new tnt_artifact;
   a->device = ctx->device;
@@ -1080,31 +1225,12 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     if (int rc = build_plan(ctx, d->out, &a->out)) return fail(rc);
     if (a->out->src_len != d->core_len) return fail((ctx->err = "output transform length mismatch", TNT_E_SHAPE));
   }
-  std::vector<GJob> gj(d->n_jobs);
-  for (size_t i = 0; i < d->n_jobs; ++i) {
-    const tnt_gemm_job& j = d->jobs[i];
-    GJob g{};
-    g.d_off = (int64_t)j.dst_offset;
-    g.a_off = (int64_t)j.lhs_offset;
-    g.b_off = (int64_t)j.rhs_offset;
-    g.m = (int32_t)j.rows;
-    g.k = (int32_t)j.contracted;
-    g.n = (int32_t)j.cols;
-    g.lda = d->lhs_op == TNT_OP_ADJOINT ? std::max(g.k, 1) : std::max(g.m, 1);
-    g.ldb = d->rhs_op == TNT_OP_ADJOINT ? std::max(g.n, 1) : std::max(g.k, 1);
-    g.ldd = std::max(g.m, 1);
-    g.ops = (d->lhs_op == TNT_OP_ADJOINT ? 1 : 0) | (d->rhs_op == TNT_OP_ADJOINT ? 2 : 0);
-    if (!job_in_bounds(g, d->core_len, d->lhs_len, d->rhs_len)) {
-      ctx->err = "core GEMM job addresses elements outside its operands";
-      return fail(TNT_E_OOB);
-    }
-    gj[i] = g;
-  }
-  build_gemm_table(a->gemm, gj);
+  build_gemm_table(a->gemm, gj, ctx->knobs);
   if (int rc = upload_gemm_table(ctx, a->gemm, a->allocs)) return fail(rc);
   std::vector<int64_t> uo, ul;
   for (size_t i = 0; i < d->n_uncovered; ++i) {
-    if (d->uncovered[i].offset < 0 || d->uncovered[i].offset + d->uncovered[i].len > d->core_len) {
+    if (d->uncovered[i].offset < 0 || d->uncovered[i].len < 0 ||
+        d->uncovered[i].offset + d->uncovered[i].len > d->core_len) {
       ctx->err = "uncovered region out of range";
       return fail(TNT_E_OOB);
     }
@@ -1157,17 +1283,17 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<GJob> jobs = a->gemm.host_jobs;
   GemmTable t;
-  build_gemm_table(t, jobs, cfg);
+  build_gemm_table(t, jobs, ctx->knobs, cfg);
   std::vector<void*> allocs;
   if (int rc = upload_gemm_table(ctx, t, allocs)) {
     for (auto p : allocs) pool_free(ctx, p);
     return rc;
   }
-  // release the previous tables (they are the first two gemm allocations of the artifact)
+  // release the previous tables (a live graph may have recorded them)
   auto drop = [&](void* p) {
     auto it = std::find(a->allocs.begin(), a->allocs.end(), p);
     if (it != a->allocs.end()) {
-      pool_free(ctx, p);
+      retire(ctx, p);
       a->allocs.erase(it);
     }
   };
@@ -1203,7 +1329,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   if (int rc = check_buf(ctx, lhs, "lhs")) return rc;
   if (int rc = check_buf(ctx, rhs, "rhs")) return rc;
   if (a->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, "artifact uploaded to another device");
-  if (precision < TNT_PREC_F64 || precision > TNT_PREC_SPLIT_I8) FAIL(ctx, TNT_E_INVALID, "unknown precision");
+  if (!valid_precision(precision)) FAIL(ctx, TNT_E_INVALID, "unknown precision");
   int64_t need_l = a->lhs ? a->lhs->src_len : a->lhs_len;
   int64_t need_r = a->rhs ? a->rhs->src_len : a->rhs_len;
   int64_t need_d = a->out ? a->out->dst_len : a->core_len;
@@ -1216,8 +1342,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   if (a->fused) {
     launch_coeff_gather(a->fused->d.coeffs, R, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
     run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
-    HIPCHK(ctx, hipGetLastError());
-    return TNT_OK;
+    return launch_status(ctx);
   }
   if (a->lhs) {
     run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
@@ -1238,8 +1363,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision);
     if (rc) return rc;
   }
-  HIPCHK(ctx, hipGetLastError());
-  return TNT_OK;
+  return launch_status(ctx);
 }
 
 int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
@@ -1268,6 +1392,7 @@ int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt
     *out = 0.0;
     return TNT_OK;
   }
+  begin_call(ctx);
   // one packed table: begins | ends | partials, regions and weights separately
   void *dcb, *dce, *dcr, *dw;
   int rc = cached_table(ctx, cb.data(), 8 * cb.size(), "DB", &dcb);
@@ -1319,6 +1444,7 @@ int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, con
     launch_fill(dst, 1, 0.0, ctx->stream);
     return TNT_OK;
   }
+  begin_call(ctx);
   void *dcb, *dce, *dcr, *dw, *parts;
   int rc = cached_table(ctx, cb.data(), 8 * cb.size(), "DB", &dcb);
   if (!rc) rc = cached_table(ctx, ce.data(), 8 * ce.size(), "DE", &dce);
@@ -1329,7 +1455,7 @@ int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, con
   // stream, so stream order keeps consecutive calls from overlapping in it
   if (ctx->dot_parts_len < cb.size()) {
     if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "dot scratch must grow during graph capture; run the call once before capturing");
-    if (ctx->dot_parts) pool_free(ctx, ctx->dot_parts);
+    if (ctx->dot_parts) retire(ctx, ctx->dot_parts);
     ctx->dot_parts = nullptr;
     ctx->dot_parts_len = 0;
     if ((rc = pool_alloc(ctx, 8 * cb.size(), &ctx->dot_parts))) return rc;
@@ -1401,6 +1527,7 @@ int tnt_graph_begin(tnt_ctx* ctx) {
   if (int rc = set_device(ctx)) return rc;
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   ctx->capturing = true;
+  ctx->capture_keys.clear();
   return TNT_OK;
 }
 
@@ -1419,9 +1546,22 @@ int tnt_graph_end(tnt_ctx* ctx, tnt_graph** out) {
   if (e != hipSuccess) {
     (void)hipGraphDestroy(g->graph);
     delete g;
+    ctx->capture_keys.clear();
     ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
     return TNT_E_HIP;
   }
+  // pin the cache entries the graph recorded; keep retired memory while it lives
+  std::sort(ctx->capture_keys.begin(), ctx->capture_keys.end());
+  ctx->capture_keys.erase(std::unique(ctx->capture_keys.begin(), ctx->capture_keys.end()), ctx->capture_keys.end());
+  for (const auto& k : ctx->capture_keys) {
+    auto it = ctx->tables.find(k);
+    if (it != ctx->tables.end()) {
+      ++it->second.pins;
+      g->pinned.push_back(k);
+    }
+  }
+  ctx->capture_keys.clear();
+  ++ctx->live_graphs;
   *out = g;
   return TNT_OK;
 }
@@ -1438,7 +1578,17 @@ int tnt_graph_destroy(tnt_ctx* ctx, tnt_graph* g) {
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipGraphExecDestroy(g->exec);
   (void)hipGraphDestroy(g->graph);
+  for (const auto& k : g->pinned) {
+    auto it = ctx->tables.find(k);
+    if (it != ctx->tables.end() && it->second.pins > 0) --it->second.pins;
+  }
   delete g;
+  if (ctx->live_graphs > 0 && --ctx->live_graphs == 0) {
+    // nothing can replay the retired memory any more
+    for (auto p : ctx->retired) pool_free(ctx, p);
+    ctx->retired.clear();
+    if (ctx->split) split_workspace_release(ctx->split.get());
+  }
   return TNT_OK;
 }
 
@@ -1479,23 +1629,49 @@ int tnt_ctx_reset_stats(tnt_ctx* ctx) {
   return TNT_OK;
 }
 
-// Tile-shape override used by the autotuner sweep (-1 restores the table).
-int tnt_set_tile_config(int cfg) {
-  if (cfg < -1 || cfg >= TILE_CFG_COUNT) return TNT_E_INVALID;
-  tile_override = cfg;
+int tnt_ctx_inject_fault(tnt_ctx* ctx, int kind, int after) {
+  if (!ctx) return TNT_E_INVALID;
+  if (after < -1) FAIL(ctx, TNT_E_INVALID, "fault countdown must be >= -1");
+  switch (kind) {
+    case TNT_FAULT_NONE: ctx->fail_alloc_after = ctx->fail_launch_after = -1; return TNT_OK;
+    case TNT_FAULT_ALLOC: ctx->fail_alloc_after = after; return TNT_OK;
+    case TNT_FAULT_LAUNCH: ctx->fail_launch_after = after; return TNT_OK;
+    default: FAIL(ctx, TNT_E_INVALID, "unknown fault kind");
+  }
+}
+
+int tnt_ctx_set_table_cache_cap(tnt_ctx* ctx, size_t entries) {
+  if (!ctx) return TNT_E_INVALID;
+  if (entries == 0) FAIL(ctx, TNT_E_INVALID, "table cache cap must be >= 1");
+  ctx->table_cap = entries;
+  trim_tables(ctx);
   return TNT_OK;
 }
 
-int tnt_set_split_slices(int slices) { return split_set_slices(slices); }
+// Tile-shape override used by the autotuner sweep (-1 restores the table).
+int tnt_set_tile_config(tnt_ctx* ctx, int cfg) {
+  if (!ctx) return TNT_E_INVALID;
+  if (cfg < -1 || cfg >= TILE_CFG_COUNT) FAIL(ctx, TNT_E_INVALID, "unknown tile configuration");
+  ctx->knobs.override_cfg = cfg;
+  return TNT_OK;
+}
 
-int tnt_set_tile_table(const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n) {
-  if (n && (!m_max || !n_max || !cfg)) return TNT_E_INVALID;
+int tnt_set_split_slices(tnt_ctx* ctx, int slices) {
+  if (!ctx) return TNT_E_INVALID;
+  if (slices < 2 || slices > 6) FAIL(ctx, TNT_E_INVALID, "split slices must be 2..6");
+  ctx->split_slices = slices;
+  return TNT_OK;
+}
+
+int tnt_set_tile_table(tnt_ctx* ctx, const int64_t* m_max, const int64_t* n_max, const int32_t* cfg, size_t n) {
+  if (!ctx) return TNT_E_INVALID;
+  if (n && (!m_max || !n_max || !cfg)) FAIL(ctx, TNT_E_INVALID, "null tile table");
   std::vector<TileRule> t;
   for (size_t i = 0; i < n; ++i) {
-    if (cfg[i] < 0 || cfg[i] >= TILE_CFG_COUNT) return TNT_E_INVALID;
+    if (cfg[i] < 0 || cfg[i] >= TILE_CFG_COUNT) FAIL(ctx, TNT_E_INVALID, "unknown tile configuration");
     t.push_back({m_max[i], n_max[i], cfg[i]});
   }
-  tile_table.swap(t);
+  ctx->knobs.table.swap(t);
   return TNT_OK;
 }
 

@@ -8,6 +8,9 @@
 //   FP64: v_mfma_f64_16x16x4f64, FP64 accumulate.
 //   FP32: operands rounded to FP32 while staging into LDS,
 //         v_mfma_f32_16x16x4f32, FP32 accumulate, FP64 epilogue.
+// Payload storage S is FP64 or FP32 (DenseRead::F32 jobs of the reference's
+// executor seam); loads convert to the arithmetic type, the epilogue computes
+// alpha acc + beta D in FP64 and rounds to S once.
 //
 // Workgroup = 4 wave64s in a 2x2 grid (8 in a 2x4 grid for TILE_128x128_W8);
 // each wave owns a (BM/GM)x(BN/GN) sub-tile made of 16x16 MFMA tiles.  K advances in steps of
@@ -57,7 +60,7 @@ constexpr int BK = 16;
 // Register budget: 4-wave tiles at most 256 VGPRs+AGPRs per lane, so two
 // waves share each SIMD (one wave per SIMD leaves LDS and global latency
 // exposed); 8-wave tiles at most 128, four waves per SIMD.
-template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false>
+template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false, typename S = double>
 __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 ? 4 : 2))) void gemm_kernel(
     GemmArgs g) {
   constexpr int NTH = 64 * GM * GN;
@@ -74,9 +77,9 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   const int m = job.m, n = job.n, k = job.k;
   const int m0 = tile.tm * BM, n0 = tile.tn * BN;
   const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
-  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
-  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
-  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
+  const gptr<const S> A = gmem<const S>(g.abase) + job.a_off;
+  const gptr<const S> B = gmem<const S>(g.bbase) + job.b_off;
+  const gptr<S> D = gmem<S>(g.dbase) + job.d_off;
   const int64_t lda = job.lda, ldb = job.ldb, ldd = job.ldd;
 
   const int tid = threadIdx.x;
@@ -100,10 +103,10 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   const int b_o = b_adj ? tid / BN : tid / BK;
   const int b_step = b_adj ? NTH / BN : NTH / BK;
   // element (row i, k kk) of op(A): identity A[i + kk*lda], adjoint A[kk + i*lda]
-  gptr<const double> pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
+  gptr<const S> pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
   const int64_t a_rstride = (int64_t)a_step * lda;  // between consecutive loads r
   const int64_t a_kadv = a_adj ? BK : (int64_t)BK * lda;
-  gptr<const double> pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
+  gptr<const S> pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
   const int64_t b_rstride = (int64_t)b_step * ldb;
   const int64_t b_kadv = b_adj ? (int64_t)BK * ldb : BK;
   // LDS destinations: As[buf][kk][i], Bs[buf][kk][j]
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   // B identity: (kk = b_c, j = b_o + b_step*r);  adjoint: (kk = b_o + b_step*r, j = b_c)
   const bool m_full = m0 + BM <= m, n_full = n0 + BN <= n;
 
-  double ra[A_LOADS], rb[B_LOADS];
+  S ra[A_LOADS], rb[B_LOADS];
   auto load = [&](int k0) {
     const bool k_full = k0 + BK <= k;
     if (m_full && k_full) {
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
         const int i = a_adj ? a_o + a_step * r : a_c;
         const int kk = a_adj ? a_c : a_o + a_step * r;
         const bool ok = (m0 + i < m) && (k0 + kk < k);
-        ra[r] = ok ? pa[r * a_rstride] : 0.0;
+        ra[r] = ok ? pa[r * a_rstride] : S(0);
       }
     }
     if (n_full && k_full) {
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
         const int j = b_adj ? b_c : b_o + b_step * r;
         const int kk = b_adj ? b_o + b_step * r : b_c;
         const bool ok = (n0 + j < n) && (k0 + kk < k);
-        rb[r] = ok ? pb[r * b_rstride] : 0.0;
+        rb[r] = ok ? pb[r * b_rstride] : S(0);
       }
     }
     pa += a_kadv;
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
         const int col = min(n0 + wn + j * 16 + (lane & 15), n - 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          old[j][r] = D[min(m0 + wm + i * 16 + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * ldd];
+          old[j][r] = (double)D[min(m0 + wm + i * 16 + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * ldd];
       }
     }
 #pragma unroll
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
         const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
         if (row < m) {
           const double v = alpha * (double)acc[i][j][r];
-          if constexpr (BETA) D[row + (int64_t)col * ldd] = v + beta * old[j][r];
-          else D[row + (int64_t)col * ldd] = v;
+          if constexpr (BETA) D[row + (int64_t)col * ldd] = (S)(v + beta * old[j][r]);
+          else D[row + (int64_t)col * ldd] = (S)v;
         }
       }
     }
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
 // Narrow-output GEMM (n <= 16): memory-bound, so no matrix cores -- one
 // thread per output row, op(B) staged in LDS, FP64 FMA.  Reads op(A) once
 // (coalesced along rows for identity A) and writes D once.
-template <typename T>
+template <typename T, typename S = double>
 __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
   constexpr int NMAX = 16, KC = 256;
   __shared__ double Bsm[KC][NMAX];
@@ -229,9 +232,9 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
   const GJob job = g.jobs[tile.job];
   const int m = job.m, n = job.n, k = job.k;
   const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
-  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
-  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
-  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
+  const gptr<const S> A = gmem<const S>(g.abase) + job.a_off;
+  const gptr<const S> B = gmem<const S>(g.bbase) + job.b_off;
+  const gptr<S> D = gmem<S>(g.dbase) + job.d_off;
   const int i = tile.tm * 256 + threadIdx.x;
   double acc[NMAX];
 #pragma unroll
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
     for (int e = threadIdx.x; e < kc * NMAX; e += 256) {
       const int kk = e / NMAX, j = e % NMAX;
       double v = 0.0;
-      if (j < n) v = b_adj ? B[j + (int64_t)(k0 + kk) * job.ldb] : B[(k0 + kk) + (int64_t)j * job.ldb];
+      if (j < n) v = (double)(b_adj ? B[j + (int64_t)(k0 + kk) * job.ldb] : B[(k0 + kk) + (int64_t)j * job.ldb]);
       Bsm[kk][j] = (double)(T)v;
     }
     __syncthreads();
@@ -261,13 +264,13 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
     double old[NMAX] = {};
     if (beta != 0.0) {
 #pragma unroll
-      for (int j = 0; j < NMAX; ++j) old[j] = D[i + (int64_t)min(j, n - 1) * job.ldd];
+      for (int j = 0; j < NMAX; ++j) old[j] = (double)D[i + (int64_t)min(j, n - 1) * job.ldd];
     }
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
       if (j < n) {
         const double v = alpha * acc[j];
-        D[i + (int64_t)j * job.ldd] = (beta != 0.0) ? v + beta * old[j] : v;
+        D[i + (int64_t)j * job.ldd] = (S)((beta != 0.0) ? v + beta * old[j] : v);
       }
     }
   }
@@ -281,8 +284,9 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs g) {
 // Inside a block of 8 k the lanes of group q = lane / 16 supply k = k0 + q + 4p
 // to the p-th MFMA.  Loads are unconditional (indices clamped into the job's
 // operands); out-of-range k is zeroed by select, so the loop body is
-// straight-line code and each stage keeps its own registers.
-template <typename T, bool BETA = false>
+// straight-line code and each stage keeps its own registers.  A job with
+// nothing contracted (k == 0) loads nothing: its operands may be empty.
+template <typename T, bool BETA = false, typename S = double>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_tiles) {
   constexpr int MT = 2, NT = 2;
   const int lane = threadIdx.x & 63, l16 = lane & 15, q4 = lane >> 4;
@@ -295,9 +299,9 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
   const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
   const int64_t sAi = a_adj ? job.lda : 1, sAk = a_adj ? 1 : job.lda;
   const int64_t sBj = b_adj ? 1 : job.ldb, sBk = b_adj ? job.ldb : 1;
-  const gptr<const double> A = gmem<const double>(g.abase) + job.a_off;
-  const gptr<const double> B = gmem<const double>(g.bbase) + job.b_off;
-  gptr<const double> pa[MT], pb[NT];
+  const gptr<const S> A = gmem<const S>(g.abase) + job.a_off;
+  const gptr<const S> B = gmem<const S>(g.bbase) + job.b_off;
+  gptr<const S> pa[MT], pb[NT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) pa[i] = A + (int64_t)min(m0 + 16 * i + l16, m - 1) * sAi;
 #pragma unroll
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
   for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-  double ra[2][2][MT], rb[2][2][NT];  // [stage][p][fragment]
+  S ra[2][2][MT], rb[2][2][NT];  // [stage][p][fragment]
   const int nb = (k + 7) / 8;
   auto load = [&](auto sc, int blk) {
     constexpr int s = decltype(sc)::value;
@@ -328,9 +332,9 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
       const bool ok = blk * 8 + q4 + 4 * p < k;
       T af[MT], bf[NT];
 #pragma unroll
-      for (int i = 0; i < MT; ++i) af[i] = (T)(ok ? ra[s][p][i] : 0.0);
+      for (int i = 0; i < MT; ++i) af[i] = (T)(ok ? ra[s][p][i] : S(0));
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bf[j] = (T)(ok ? rb[s][p][j] : 0.0);
+      for (int j = 0; j < NT; ++j) bf[j] = (T)(ok ? rb[s][p][j] : S(0));
 #pragma unroll
       for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
   // two register stages: block t + 1 is in flight while block t computes; the
   // scheduling barriers keep the compiler from hoisting the next stage's loads
   // above the current compute (which would make it wait for them)
-  load(S0{}, 0);
+  if (k > 0) load(S0{}, 0);
   for (int t = 0; t < nb; t += 2) {
     load(S1{}, t + 1);
     __builtin_amdgcn_sched_barrier(0);
@@ -356,7 +360,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
   // Epilogue.  The BETA instance (launched for beta != 0) reads the whole
   // tile of D first, indices clamped into the job, so the wave waits once
   // before the stores instead of once per element.
-  const gptr<double> D = gmem<double>(g.dbase) + job.d_off;
+  const gptr<S> D = gmem<S>(g.dbase) + job.d_off;
   const double alpha = g.alpha, beta = g.beta;
   double old[BETA ? MT : 1][NT][4];
   if constexpr (BETA) {
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
         const int col = min(n0 + 16 * j + l16, n - 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          old[i][j][r] = D[min(m0 + 16 * i + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * job.ldd];
+          old[i][j][r] = (double)D[min(m0 + 16 * i + MfmaOp<T>::row(lane, r), m - 1) + (int64_t)col * job.ldd];
       }
   }
 #pragma unroll
@@ -381,60 +385,68 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
         const int row = m0 + 16 * i + MfmaOp<T>::row(lane, r);
         if (row < m) {
           const double v = alpha * (double)acc[i][j][r];
-          if constexpr (BETA) D[row + (int64_t)col * job.ldd] = v + beta * old[i][j][r];
-          else D[row + (int64_t)col * job.ldd] = v;
+          if constexpr (BETA) D[row + (int64_t)col * job.ldd] = (S)(v + beta * old[i][j][r]);
+          else D[row + (int64_t)col * job.ldd] = (S)v;
         }
       }
     }
 }
 
-template <typename T, bool BETA>
+template <typename T, bool BETA, typename S>
 static void launch_tiles(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
   switch (cfg) {
     case TILE_128x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_128x64:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 64, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 64, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_64x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 128, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 128, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_128x128_W8:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4, BETA>), dim3(n_tiles), dim3(512), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4, BETA, S>), dim3(n_tiles), dim3(512), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 2, 2, BETA>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
   }
 }
 
-template <typename T>
+template <typename T, typename S>
 static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
   if (n_tiles == 0) return;
   switch (cfg) {
     case TILE_SKINNY:
-      hipLaunchKernelGGL((gemm_skinny_kernel<T>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, S>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_SMALL:  // four independent one-wave tiles per workgroup
       if (a.beta != 0.0)
-        hipLaunchKernelGGL((gemm_small_kernel<T, true>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
+        hipLaunchKernelGGL((gemm_small_kernel<T, true, S>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
       else
-        hipLaunchKernelGGL((gemm_small_kernel<T, false>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
+        hipLaunchKernelGGL((gemm_small_kernel<T, false, S>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
       break;
     default:
-      if (a.beta != 0.0) launch_tiles<T, true>(a, cfg, n_tiles, s);
-      else launch_tiles<T, false>(a, cfg, n_tiles, s);
+      if (a.beta != 0.0) launch_tiles<T, true, S>(a, cfg, n_tiles, s);
+      else launch_tiles<T, false, S>(a, cfg, n_tiles, s);
       break;
   }
 }
 
 void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<double>(a, cfg, n_tiles, s);
+  launch_cfg<double, double>(a, cfg, n_tiles, s);
 }
 
 void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<float>(a, cfg, n_tiles, s);
+  launch_cfg<float, double>(a, cfg, n_tiles, s);
+}
+
+void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<double, float>(a, cfg, n_tiles, s);
+}
+
+void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<float, float>(a, cfg, n_tiles, s);
 }
 
 }  // namespace tnt_dev

@@ -108,21 +108,42 @@ struct SplitTables {
   uint32_t tile_start[SPLIT_SHAPES] = {}, tile_count[SPLIT_SHAPES] = {};
   uint32_t items_a = 0, items_b = 0, exp_items_a = 0, exp_items_b = 0;
   int64_t frags_a = 0, frags_b = 0, a_exps = 0, b_exps = 0, a_rows = 0, b_rows = 0;
+  uint64_t last_use = 0;  // SplitWorkspace::uses at the table's last run
 };
+
+// Job-list tables kept per workspace before least recently used ones are
+// dropped (only while no graph of the context is alive).
+constexpr size_t kSplitTableCap = 256;
 
 struct SplitWorkspace {
   void* planes = nullptr;  // A planes | B planes (bf16) | A exps | B exps (int32)
   size_t bytes = 0;
   std::unordered_map<std::string, SplitTables> tables;
-  std::vector<void*> table_allocs;
+  uint64_t uses = 0;
+  // device memory a live graph may have recorded: outgrown plane buffers and
+  // dropped tables, freed by split_workspace_release once no graph is alive
+  std::vector<void*> retired;
+  static void free_table(SplitTables& t) {
+    (void)hipFree(t.jobs);
+    (void)hipFree(t.tiles);
+    (void)hipFree(t.prefix);
+  }
   ~SplitWorkspace() {
     if (planes) (void)hipFree(planes);
-    for (auto p : table_allocs) (void)hipFree(p);
+    for (auto& kv : tables) free_table(kv.second);
+    for (auto p : retired) (void)hipFree(p);
   }
 };
 
 SplitWorkspace* split_workspace_create() { return new SplitWorkspace(); }
 void split_workspace_destroy(SplitWorkspace* ws) { delete ws; }
+
+void split_workspace_release(SplitWorkspace* ws) {
+  if (!ws || ws->retired.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto p : ws->retired) (void)hipFree(p);
+  ws->retired.clear();
+}
 
 // ---------------------------------------------------------------------------
 // Pre-pass: one launch over items (operand side, job, block of 16 lines,
@@ -561,17 +582,13 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b) {
 }
 
 // 16 bytes per lane from global memory straight into LDS at lds + 16 * lane:
-// global address = base (uniform, SGPR pair) + voff (per lane).  Written as
-// asm so that the compiler's wait insertion does not treat every later LDS
-// read as depending on every copy in flight (it cannot tell the two stage
-// buffers apart across this many copies and would wait for all of them,
-// prefetch included); the kernel waits with explicit vmcnt instead.  The
-// copy has no register result, so nothing is left in flight in a VGPR.
+// global address = base (uniform) + voff (per lane).  The copy has no
+// register result, so nothing is left in flight in a VGPR; the kernel waits
+// for the copies with explicit vmcnt before the barrier that publishes them.
 __device__ __forceinline__ void copy_to_lds(uint64_t base, uint32_t voff, uint32_t lds) {
-  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
-               :
-               : "v"(voff), "s"(base), "s"(lds)
-               : "memory", "m0");
+  typedef __attribute__((address_space(1))) void gvoid;
+  typedef __attribute__((address_space(3))) void lvoid;
+  __builtin_amdgcn_global_load_lds((gvoid*)(base + voff), (lvoid*)(uintptr_t)lds, 16, 0, 0);
 }
 
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
@@ -863,14 +880,6 @@ __global__ __launch_bounds__(64 * GM * GN) void split_i8_gemm_kernel(SplitGemmAr
   }
 }
 
-static int split_slices_setting = 5;
-
-int split_set_slices(int slices) {
-  if (slices < 2 || slices > 6) return TNT_E_INVALID;
-  split_slices_setting = slices;
-  return TNT_OK;
-}
-
 template <int S>
 static void launch_split_s(const SplitTables& t, const SplitGemmArgs& a0, hipStream_t s) {
   SplitGemmArgs a = a0;
@@ -937,11 +946,13 @@ static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
   return SPLIT_64x64;
 }
 
-int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
-                   const std::vector<GTile>& /*host_tiles*/, const GTile* /*dev_tiles*/, uintptr_t d,
-                   uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s, bool capturing,
-                   std::string& err) {
-  const int S = split_slices_setting;
+int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
+                   uintptr_t d, uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s, bool capturing,
+                   bool graphs_alive, std::string& err) {
+  if (S < 2 || S > 6) {
+    err = "split slices outside 2..6";
+    return TNT_E_INVALID;
+  }
   if (host_jobs.empty()) return TNT_OK;
   const bool i8 = scheme == SPLIT_SCHEME_I8;
   const int kstep = i8 ? I8_KSTEP : KSTEP, kchunk = i8 ? I8_KCHUNK : KCHUNK;
@@ -953,6 +964,24 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
     if (capturing) {
       err = "split GEMM tables are built on first use; run the call once before capturing";
       return TNT_E_INVALID;
+    }
+    if (ws->tables.size() >= kSplitTableCap) {
+      // drop the least recently used half; memory a live graph may hold is retired
+      std::vector<std::pair<uint64_t, std::string>> byuse;
+      for (auto& kv : ws->tables) byuse.emplace_back(kv.second.last_use, kv.first);
+      std::sort(byuse.begin(), byuse.end());
+      SPLIT_HIP(hipStreamSynchronize(s));
+      for (size_t i = 0; i < byuse.size() / 2; ++i) {
+        auto e = ws->tables.find(byuse[i].second);
+        if (graphs_alive) {
+          ws->retired.push_back(e->second.jobs);
+          ws->retired.push_back(e->second.tiles);
+          ws->retired.push_back(e->second.prefix);
+        } else {
+          SplitWorkspace::free_table(e->second);
+        }
+        ws->tables.erase(e);
+      }
     }
     SplitTables t;
     std::vector<SplitJob> sj(host_jobs.size());
@@ -1024,22 +1053,27 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
     pre.insert(pre.end(), pre_b.begin(), pre_b.end());
     pre.insert(pre.end(), xpre_a.begin(), xpre_a.end());
     pre.insert(pre.end(), xpre_b.begin(), xpre_b.end());
-    void *pj, *pt, *pp;
-    SPLIT_HIP(hipMalloc(&pj, sizeof(SplitJob) * sj.size()));
-    ws->table_allocs.push_back(pj);
-    SPLIT_HIP(hipMalloc(&pt, sizeof(GTile) * std::max<size_t>(tiles.size(), 1)));
-    ws->table_allocs.push_back(pt);
-    SPLIT_HIP(hipMalloc(&pp, sizeof(int64_t) * pre.size()));
-    ws->table_allocs.push_back(pp);
-    SPLIT_HIP(hipMemcpyAsync(pj, sj.data(), sizeof(SplitJob) * sj.size(), hipMemcpyHostToDevice, s));
-    if (!tiles.empty()) SPLIT_HIP(hipMemcpyAsync(pt, tiles.data(), sizeof(GTile) * tiles.size(), hipMemcpyHostToDevice, s));
-    SPLIT_HIP(hipMemcpyAsync(pp, pre.data(), sizeof(int64_t) * pre.size(), hipMemcpyHostToDevice, s));
-    SPLIT_HIP(hipStreamSynchronize(s));
+    void *pj = nullptr, *pt = nullptr, *pp = nullptr;
+    hipError_t e1 = hipMalloc(&pj, sizeof(SplitJob) * sj.size());
+    hipError_t e2 = e1 == hipSuccess ? hipMalloc(&pt, sizeof(GTile) * std::max<size_t>(tiles.size(), 1)) : e1;
+    hipError_t e3 = e2 == hipSuccess ? hipMalloc(&pp, sizeof(int64_t) * pre.size()) : e2;
+    if (e3 != hipSuccess) {
+      (void)hipFree(pj);
+      (void)hipFree(pt);
+      (void)hipGetLastError();
+      err = std::string("hipMalloc (split GEMM tables): ") + hipGetErrorString(e3);
+      return e3 == hipErrorOutOfMemory ? TNT_E_OOM : TNT_E_HIP;
+    }
     t.jobs = static_cast<SplitJob*>(pj);
     t.tiles = static_cast<GTile*>(pt);
     t.prefix = static_cast<int64_t*>(pp);
     it = ws->tables.emplace(key, t).first;
+    SPLIT_HIP(hipMemcpyAsync(pj, sj.data(), sizeof(SplitJob) * sj.size(), hipMemcpyHostToDevice, s));
+    if (!tiles.empty()) SPLIT_HIP(hipMemcpyAsync(pt, tiles.data(), sizeof(GTile) * tiles.size(), hipMemcpyHostToDevice, s));
+    SPLIT_HIP(hipMemcpyAsync(pp, pre.data(), sizeof(int64_t) * pre.size(), hipMemcpyHostToDevice, s));
+    SPLIT_HIP(hipStreamSynchronize(s));
   }
+  it->second.last_use = ++ws->uses;
   const SplitTables& t = it->second;
   // workspace: A fragments | B fragments (1 KiB each) | A exps | B exps (int32,
   // line x chunk) | A line exps | B line exps (int32, INT8 scheme)
@@ -1051,7 +1085,10 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host
       return TNT_E_INVALID;
     }
     SPLIT_HIP(hipStreamSynchronize(s));
-    if (ws->planes) SPLIT_HIP(hipFree(ws->planes));
+    if (ws->planes) {
+      if (graphs_alive) ws->retired.push_back(ws->planes);  // a graph may have recorded it
+      else SPLIT_HIP(hipFree(ws->planes));
+    }
     ws->planes = nullptr;
     ws->bytes = 0;
     SPLIT_HIP(hipMalloc(&ws->planes, bytes));

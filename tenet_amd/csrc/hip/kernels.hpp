@@ -169,6 +169,9 @@ inline int tile_cols(int cfg) {
 // share `cfg`; the caller splits the table by configuration.
 void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
 void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+// The same kernels over FP32 payloads (FP64 or FP32 arithmetic).
+void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
 
 // Split-BF16 (Ozaki) GEMM: operands are pre-split into `slices` BF16 planes
 // with per-row (lhs) / per-column (rhs) power-of-two scales.
@@ -195,6 +198,7 @@ void launch_weighted_dot_partials(const double* a, const double* b, const int64_
 void launch_sum_partials(const double* partials, uint32_t n, double* out, hipStream_t s);
 void launch_axpby(double* y, const double* x, int64_t n, double alpha, double beta, hipStream_t s);
 void launch_fill(double* y, int64_t n, double value, hipStream_t s);
+void launch_zero_f32(float* y, int64_t n, hipStream_t s);
 void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,

@@ -69,6 +69,11 @@ __global__ __launch_bounds__(256) void fill_kernel(double* __restrict__ y, int64
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) y[i] = v;
 }
 
+__global__ __launch_bounds__(256) void zero_f32_kernel(float* __restrict__ y, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) y[i] = 0.0f;
+}
+
 __global__ __launch_bounds__(256) void scale_regions_kernel(double* __restrict__ y,
                                                              const int64_t* __restrict__ off,
                                                              const int64_t* __restrict__ len,
@@ -93,6 +98,11 @@ void launch_axpby(double* y, const double* x, int64_t n, double alpha, double be
 void launch_fill(double* y, int64_t n, double value, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n, value);
+}
+
+void launch_zero_f32(float* y, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n);
 }
 
 void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,
@@ -120,17 +130,83 @@ __global__ __launch_bounds__(256) void fill_unit_kernel(double* __restrict__ y, 
 }
 
 // normal(0,1) by Box-Muller on consecutive pairs of the same stream: element
-// 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0,1].
+// 2p and 2p+1 use draws (2p+1, 2p+2) mapped to u in (0,1].  ln, sin and cos
+// are evaluated by the fixed sequences below from +, -, *, / and sqrt only
+// (every one a correctly rounded IEEE-754 double operation, contraction into
+// fma disabled), so that the fill is a pure function of the seed, the same bits
+// on any device and on the host: ln u = e ln2 + 2 atanh((m-1)/(m+1)) with
+// u = m 2^e, m in [sqrt(1/2), sqrt(2)), series to s^23; sin / cos of
+// a = (4t - rint(4t)) pi/2 in [-pi/4, pi/4] by Taylor series to a^19 / a^18,
+// rotated by the quadrant rint(4t) mod 4.  Coefficients: 1/(2k+1), and
+// (-1)^k / (2k+1)!, (-1)^k / (2k)!, each the double nearest the exact value.
+__device__ constexpr double kAtanhC[11] = {0x1.5555555555555p-2, 0x1.999999999999ap-3, 0x1.2492492492492p-3,
+                                           0x1.c71c71c71c71cp-4, 0x1.745d1745d1746p-4, 0x1.3b13b13b13b14p-4,
+                                           0x1.1111111111111p-4, 0x1.e1e1e1e1e1e1ep-5, 0x1.af286bca1af28p-5,
+                                           0x1.8618618618618p-5, 0x1.642c8590b2164p-5};
+__device__ constexpr double kSinC[9] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+                                        0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+                                        -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49, -0x1.2f49b46814157p-57};
+__device__ constexpr double kCosC[9] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
+                                        0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
+                                        -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45, -0x1.6827863b97d97p-53};
+constexpr double kLn2Hi = 0x1.62e42fee00000p-1;  // low 21 bits clear: e * kLn2Hi is exact
+constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;  // ln 2 - kLn2Hi
+constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
+constexpr double kPiHalf = 0x1.921fb54442d18p+0;
+
+// sum_k c[k] z^(k+1) as z * (c0 + z * (c1 + ...)), one rounding per operation
+template <int N>
+__device__ __forceinline__ double horner_z(double z, const double (&c)[N]) {
+#pragma clang fp contract(off)
+  double p = c[N - 1];
+#pragma unroll
+  for (int i = N - 2; i >= 0; --i) p = p * z + c[i];
+  return p * z;
+}
+
+__device__ __forceinline__ double log_unit(double u) {
+#pragma clang fp contract(off)
+  int e = 0;
+  double m = frexp(u, &e);
+  if (m < kSqrtHalf) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (m + 1.0);
+  const double s2 = s * 2.0;
+  const double lnm = s2 + s2 * horner_z(s * s, kAtanhC);
+  const double ed = (double)e;
+  return ed * kLn2Hi + (lnm + ed * kLn2Lo);
+}
+
+__device__ __forceinline__ void sincos_2pi(double t, double* sn, double* cs) {
+#pragma clang fp contract(off)
+  const double t4 = t * 4.0;
+  const double q = rint(t4);
+  const double a = (t4 - q) * kPiHalf;
+  const double z = a * a;
+  const double sa = a + a * horner_z(z, kSinC);
+  const double ca = 1.0 + horner_z(z, kCosC);
+  switch ((int)q & 3) {
+    case 0: *sn = sa; *cs = ca; break;
+    case 1: *sn = ca; *cs = -sa; break;
+    case 2: *sn = -sa; *cs = -ca; break;
+    default: *sn = -ca; *cs = sa; break;
+  }
+}
+
 __global__ __launch_bounds__(256) void fill_normal_kernel(double* __restrict__ y, int64_t n,
                                                            uint64_t seed) {
+#pragma clang fp contract(off)
   const int64_t stride = (int64_t)gridDim.x * 256;
   const int64_t pairs = (n + 1) / 2;
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < pairs; p += stride) {
-    double u1 = ((double)(splitmix_at(seed, 2 * p) >> 11) + 1.0) * 0x1.0p-53;
-    double u2 = ((double)(splitmix_at(seed, 2 * p + 1) >> 11) + 1.0) * 0x1.0p-53;
-    double r = sqrt(-2.0 * log(u1));
+    const double u1 = ((double)(splitmix_at(seed, 2 * p) >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = ((double)(splitmix_at(seed, 2 * p + 1) >> 11) + 1.0) * 0x1.0p-53;
+    const double r = sqrt(log_unit(u1) * -2.0);
     double s, c;
-    sincospi(2.0 * u2, &s, &c);
+    sincos_2pi(u2, &s, &c);
     y[2 * p] = r * c;
     if (2 * p + 1 < n) y[2 * p + 1] = r * s;
   }

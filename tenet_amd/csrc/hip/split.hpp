@@ -13,15 +13,17 @@ struct SplitWorkspace;
 enum SplitScheme { SPLIT_SCHEME_BF16 = 0, SPLIT_SCHEME_I8 = 1 };
 
 // Runs dst = alpha * A B + beta * dst for every job of the table with the
-// split scheme given (BF16 or INT8 digit planes).  Returns a TNT status; `err`
-// receives the message.
-int split_gemm_run(SplitWorkspace* ws, int scheme, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
-                   const std::vector<GTile>& host_tiles, const GTile* dev_tiles, uintptr_t d,
-                   uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s,
-                   bool capturing, std::string& err);
-int split_set_slices(int slices);
+// split scheme given (BF16 or INT8 digit planes, `slices` planes per operand,
+// 2..6).  While `graphs_alive`, memory the workspace outgrows or drops is
+// retired instead of freed (a graph may have recorded it).  Returns a TNT
+// status; `err` receives the message.
+int split_gemm_run(SplitWorkspace* ws, int scheme, int slices, const std::vector<GJob>& host_jobs,
+                   const GJob* dev_jobs, uintptr_t d, uintptr_t a, uintptr_t b, double alpha, double beta,
+                   hipStream_t s, bool capturing, bool graphs_alive, std::string& err);
 SplitWorkspace* split_workspace_create();
 void split_workspace_destroy(SplitWorkspace* ws);
+// Frees the retired memory (call once no graph of the context is alive).
+void split_workspace_release(SplitWorkspace* ws);
 
 struct SplitWorkspaceDeleter {
   void operator()(SplitWorkspace* w) const { split_workspace_destroy(w); }

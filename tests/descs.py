@@ -87,12 +87,12 @@ def contract_desc_replay(cdesc, lhs, rhs, core_len):
     for i in range(d.n_jobs):
         j = d.jobs[i]
         m, k, n = j.rows, j.contracted, j.cols
-        if d.lhs_op == 1:
+        if d.lhs_op != N.OP_IDENTITY:
             # stored X is k x m column-major; op(X) = X^T is its (m, k) row-major view
             A = L[j.lhs_offset:j.lhs_offset + m * k].reshape(m, k)
         else:
             A = L[j.lhs_offset:j.lhs_offset + m * k].reshape(k, m).T
-        if d.rhs_op == 1:
+        if d.rhs_op != N.OP_IDENTITY:
             B = R[j.rhs_offset:j.rhs_offset + k * n].reshape(k, n)
         else:
             B = R[j.rhs_offset:j.rhs_offset + k * n].reshape(n, k).T

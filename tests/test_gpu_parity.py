@@ -120,12 +120,12 @@ def test_gemm_tile_configs_match_numpy(rt, cfg, lop, rop, alpha, beta):
     L, R, D = _buf(rt, np.concatenate(lhs)), _buf(rt, np.concatenate(rhs)), _buf(rt, d0)
     h = N.hip()
     try:
-        h.tnt_set_tile_config(cfg)
+        h.tnt_set_tile_config(rt._ctx, cfg)
         arr = (N.GemmJob * len(jobs))(*jobs)
         rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, alpha, beta, 0))
         got = _get(rt, D, do)
     finally:
-        h.tnt_set_tile_config(-1)
+        h.tnt_set_tile_config(rt._ctx, -1)
         _free(rt, L, R, D)
     w = alpha * np.concatenate(want) + beta * d0
     assert rel(got, w) < 1e-14
@@ -155,13 +155,13 @@ def test_gemm_small_tiles_bit_identical_to_workgroup_tiles(rt, prec, lop, rop):
     try:
         for cfg in (1, 6):
             D = _buf(rt, D0)
-            h.tnt_set_tile_config(cfg)
+            h.tnt_set_tile_config(rt._ctx, cfg)
             arr = (N.GemmJob * len(jobs))(*jobs)
             rt._check(h.tnt_gemm_batch(rt._ctx, D, L, R, arr, len(jobs), None, 0, lop, rop, 0.75, -1.5, prec))
             out[cfg] = _get(rt, D, do)
             _free(rt, D)
     finally:
-        h.tnt_set_tile_config(-1)
+        h.tnt_set_tile_config(rt._ctx, -1)
         _free(rt, L, R)
     assert np.array_equal(out[1], out[6])
 
@@ -514,12 +514,12 @@ def test_split_error_vs_slices(rt, prec, slices, tol):
     bl, br = _buf(rt, A.reshape(-1, order="F")), _buf(rt, B.reshape(-1, order="F"))
     bd = _buf(rt, np.zeros(m * n))
     jobs = (N.GemmJob * 1)(N.GemmJob(0, 0, 0, m, k, n))
-    rt._check(N.hip().tnt_set_split_slices(slices))
+    rt._check(N.hip().tnt_set_split_slices(rt._ctx, slices))
     try:
         rt._check(N.hip().tnt_gemm_batch(rt._ctx, bd, bl, br, jobs, 1, None, 0, 0, 0, 1.0, 0.0, prec))
         got = _get(rt, bd, m * n).reshape(n, m).T
     finally:
-        N.hip().tnt_set_split_slices(5)
+        N.hip().tnt_set_split_slices(rt._ctx, 5)
         _free(rt, bl, br, bd)
     err = rel(got, A @ B)
     print(f"split prec={prec} slices={slices}: rel err {err:.3e}")
