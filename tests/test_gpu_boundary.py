@@ -219,7 +219,7 @@ def test_dimensions_beyond_int32_rejected(rt):
 @pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_zero_contracted_jobs_scale_destination(rt, cfg, prec):
     shapes = [(33, 0, 17), (5, 0, 40), (64, 3, 64)]
-    rng = np.random.default_rng(cfg + 10 * prec)
+    rng = np.random.default_rng(cfg + 1 + 10 * prec)
     arr, nl, nr, nd = jobs_of(shapes)
     lhs, rhs, d0 = rng.normal(size=nl), rng.normal(size=nr), rng.normal(size=nd)
     bl, br, bd = put(rt, lhs), put(rt, rhs), put(rt, d0)
