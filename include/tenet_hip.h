@@ -263,6 +263,19 @@ int tnt_weighted_dot(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt
  * Krylov loop, KrylovVector::dot_real, tenet-krylov/src/vector.rs:13). */
 int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
                            const double* weights, size_t n_regions, tnt_buf* out, size_t out_index);
+/* tnt_weighted_dot_async with no device allocation at all: no counterpart in
+ * the reference.  For callers that cannot allocate (code under hipGraph
+ * capture, tight serving and Krylov loops, harnesses that forbid hipMalloc).
+ * The chunk tables and partial sums live in a fixed 1 MiB device array per
+ * GPU (8 bytes per region + 28 per chunk of 65536 elements: up to about
+ * 37000 chunks, 2.4 G elements); larger region lists return
+ * TNT_E_UNSUPPORTED.  The tables are staged from host memory the context
+ * holds, so nothing is allocated per call.  The result equals
+ * tnt_weighted_dot_async's bit for bit.  Every call on one GPU shares the
+ * array: calls on one context are ordered by its stream, calls that overlap
+ * from two contexts of one GPU race. */
+int tnt_weighted_dot_async_noalloc(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                                   const double* weights, size_t n_regions, tnt_buf* out, size_t out_index);
 /* y = alpha * x + beta * y over whole payloads. */
 int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta);
 

@@ -108,6 +108,7 @@ HIP_API = {
                                    C.POINTER(C.c_double)]),
     "tnt_axpby": (C.c_int, [VP, VP, VP, C.c_double, C.c_double]),
     "tnt_weighted_dot_async": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ, VP, SZ]),
+    "tnt_weighted_dot_async_noalloc": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ, VP, SZ]),
     "tnt_event_create": (C.c_int, [VP, PP]),
     "tnt_event_record": (C.c_int, [VP, VP]),
     "tnt_event_query": (C.c_int, [VP, VP]),

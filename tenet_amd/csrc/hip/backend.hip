@@ -11,6 +11,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -76,6 +77,12 @@ struct tnt_ctx {
   // pinned host scalar for reductions
   double* host_scalar = nullptr;
   double* dev_scalar = nullptr;
+  // allocation-free dot (tnt_weighted_dot_async_noalloc): this device's copy
+  // of the file-scope `scratch` array, and a host staging area of the same
+  // size for its tables, both set up when the context is created
+  void* scratch_dev = nullptr;
+  std::unique_ptr<unsigned char[]> scratch_host;
+  std::string scratch_err;
   std::unique_ptr<SplitWorkspace, SplitWorkspaceDeleter> split;
   // optional per-kernel-class event timing (tnt_ctx_set_timing)
   bool timing = false;
@@ -849,6 +856,46 @@ static bool job_in_bounds(const GJob& j, size_t dlen, size_t alen, size_t blen) 
          span(j.b_off, br, bc, j.ldb) <= (int64_t)blen;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free dot (tnt_weighted_dot_async_noalloc)
+// ---------------------------------------------------------------------------
+// Chunk tables, weights and per-chunk partials of one call:
+//   [weights: n_regions f64][begins: n_chunks i64][ends: n_chunks i64]
+//   [region ids: n_chunks i32, padded to 8 bytes][partials: n_chunks f64]
+// Each GPU holds its own copy of the array until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: every
+// host use goes through the address hipGetSymbolAddress returned for the
+// device (tnt_ctx::scratch_dev).
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static void* g_scratch_dev[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Resolves this device's address of `scratch` once per device (the device is
+// current: called from tnt_ctx_create).  A failure is kept in scratch_err and
+// reported by the allocation-free entry point only.
+static void scratch_setup(tnt_ctx* ctx) {
+  ctx->scratch_host.reset(new unsigned char[kNoAllocScratchBytes]);
+  if (ctx->device < 0 || ctx->device >= kMaxScratchDevices) {
+    ctx->scratch_err = "device ordinal beyond the allocation-free scratch table";
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      ctx->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return;
+    }
+    g_scratch_dev[ctx->device] = ptr;
+  }
+  ctx->scratch_dev = g_scratch_dev[ctx->device];
+}
+
 // MatrixOp values (Identity, Transpose, Adjoint); on real payloads both
 // non-identity ops transpose the stored matrix.
 static bool valid_op(int op) { return op == TNT_OP_IDENTITY || op == TNT_OP_TRANSPOSE || op == TNT_OP_ADJOINT; }
@@ -907,6 +954,7 @@ int tnt_ctx_create(int device, tnt_ctx** out) {
     delete c;
     return TNT_E_HIP;
   }
+  scratch_setup(c);
   *out = c;
   return TNT_OK;
 }
@@ -1469,6 +1517,69 @@ int tnt_weighted_dot_async(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, con
   launch_sum_partials(static_cast<double*>(parts), (uint32_t)cb.size(), dst, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
   return TNT_OK;
+}
+
+// tnt_weighted_dot_async without any device allocation: chunk tables and
+// partials live in this device's `scratch` array (kNoAllocScratchBytes), the
+// tables are staged through the context's host buffer.  Same chunking and
+// kernels as tnt_weighted_dot_async, so the result is bit-identical.  Every
+// partial is written by its chunk's workgroup before it is summed, so
+// nothing in the array needs clearing.  Calls on one GPU share the array:
+// calls queued on one context are ordered by its stream, calls from two
+// contexts (streams) of one GPU that overlap race.
+int tnt_weighted_dot_async_noalloc(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf* b, const tnt_region* regions,
+                                   const double* weights, size_t n_regions, tnt_buf* out, size_t out_index) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, a, "a")) return rc;
+  if (int rc = check_buf(ctx, b, "b")) return rc;
+  if (int rc = check_buf(ctx, out, "out")) return rc;
+  if (out_index >= out->len) FAIL(ctx, TNT_E_OOB, "result index outside the output buffer");
+  if (n_regions && (!regions || !weights)) FAIL(ctx, TNT_E_INVALID, "null region list");
+  if (!ctx->scratch_dev) FAIL(ctx, TNT_E_HIP, "allocation-free scratch unavailable: " + ctx->scratch_err);
+  const int64_t chunk = 1 << 16;
+  size_t n_chunks = 0;
+  for (size_t r = 0; r < n_regions; ++r) {
+    const tnt_region& g = regions[r];
+    if (g.offset < 0 || g.len < 0 || (size_t)(g.offset + g.len) > a->len || (size_t)(g.offset + g.len) > b->len)
+      FAIL(ctx, TNT_E_OOB, "region outside the payloads");
+    n_chunks += (size_t)((g.len + chunk - 1) / chunk);
+  }
+  const size_t off_cb = 8 * n_regions, off_ce = off_cb + 8 * n_chunks, off_cr = off_ce + 8 * n_chunks;
+  const size_t off_parts = off_cr + ((4 * n_chunks + 7) & ~size_t(7));
+  const size_t need = off_parts + 8 * n_chunks;
+  if (need > kNoAllocScratchBytes || n_chunks > UINT32_MAX)
+    FAIL(ctx, TNT_E_UNSUPPORTED, "region list needs " + std::to_string(need) +
+                                     " bytes of scratch; the allocation-free dot holds at most " +
+                                     std::to_string(kNoAllocScratchBytes) + " (8 per region + 28 per 65536-element chunk)");
+  if (int rc = set_device(ctx)) return rc;
+  double* dst = static_cast<double*>(out->ptr) + out_index;
+  if (n_chunks == 0) {
+    launch_fill(dst, 1, 0.0, ctx->stream);
+    return launch_status(ctx);
+  }
+  unsigned char* hs = ctx->scratch_host.get();
+  std::memcpy(hs, weights, 8 * n_regions);
+  int64_t* cb = reinterpret_cast<int64_t*>(hs + off_cb);
+  int64_t* ce = reinterpret_cast<int64_t*>(hs + off_ce);
+  int32_t* cr = reinterpret_cast<int32_t*>(hs + off_cr);
+  size_t c = 0;
+  for (size_t r = 0; r < n_regions; ++r) {
+    const tnt_region& g = regions[r];
+    for (int64_t s0 = 0; s0 < g.len; s0 += chunk, ++c) {
+      cb[c] = g.offset + s0;
+      ce[c] = g.offset + std::min(g.len, s0 + chunk);
+      cr[c] = (int32_t)r;
+    }
+  }
+  unsigned char* ds = static_cast<unsigned char*>(ctx->scratch_dev);
+  // the tables only; stream order keeps the previous call's kernels ahead of this copy
+  HIPCHK(ctx, hipMemcpyAsync(ds, hs, off_parts, hipMemcpyHostToDevice, ctx->stream));
+  launch_weighted_dot_partials(static_cast<const double*>(a->ptr), static_cast<const double*>(b->ptr),
+                               reinterpret_cast<const int64_t*>(ds + off_cb), reinterpret_cast<const int64_t*>(ds + off_ce),
+                               reinterpret_cast<const int32_t*>(ds + off_cr), reinterpret_cast<const double*>(ds),
+                               reinterpret_cast<double*>(ds + off_parts), (uint32_t)n_chunks, ctx->stream);
+  launch_sum_partials(reinterpret_cast<const double*>(ds + off_parts), (uint32_t)n_chunks, dst, ctx->stream);
+  return launch_status(ctx);
 }
 
 struct tnt_event {
