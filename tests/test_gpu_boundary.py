@@ -374,3 +374,24 @@ def test_injected_launch_failure_quarantines_leased_runtime():
         assert np.isfinite(A.contract([1, 3], A, [2, 0], [0, 2, 1, 3], 2).to_numpy()).all()
     assert pool.idle == 1
     pool.close()
+
+
+@pytest.mark.parametrize("cfg", [0, 3, 5])
+@pytest.mark.parametrize("beta", [0.0, 0.75])
+def test_workgroup_tiles_long_k_and_ragged_edges(rt, cfg, beta):
+    """Long K (many slabs, odd tail), several tiles per job and ragged edges:
+    the workgroup tiles against numpy."""
+    shapes = [(300, 1001, 260), (129, 16, 1), (1, 17, 129), (256, 257, 128)]
+    rng = np.random.default_rng(cfg * 7 + int(4 * beta))
+    arr, nl, nr, nd = jobs_of(shapes)
+    lhs, rhs, d0 = rng.normal(size=nl), rng.normal(size=nr), rng.normal(size=nd)
+    bl, br, bd = put(rt, lhs), put(rt, rhs), put(rt, d0)
+    try:
+        rt._check(h().tnt_set_tile_config(rt._ctx, cfg))
+        rt._check(h().tnt_gemm_batch(rt._ctx, bd, bl, br, arr, len(shapes), None, 0, 0, 0, 1.25, beta, 0))
+        got = get(rt, bd, nd)
+    finally:
+        h().tnt_set_tile_config(rt._ctx, -1)
+        free(rt, bl, br, bd)
+    want = dense_ref(shapes, lhs, rhs, d0, 1.25, beta, False, False)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-14
