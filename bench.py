@@ -184,7 +184,8 @@ def run_serve(args):
     t = time.perf_counter()
     mpo = "heisenberg" if args.rule == "U1" else "random"
     srv = HeffServer(args.rule, n_streams=args.streams, max_inflight=args.inflight,
-                     precision=args.precision, mpo=mpo, graphs=args.serve_graphs == "on")
+                     precision=args.precision, mpo=mpo, graphs=args.serve_graphs == "on",
+                     max_batch=args.max_batch)
     srv.warm()
     cold = time.perf_counter() - t
     log(f"[bench] serve: {len(srv.buckets)} bond-dimension buckets {srv.buckets}, environments resident, "
@@ -236,6 +237,7 @@ def run_serve(args):
         "data": "synthetic (seeded normal(0,1) environments and states on device; "
                 + ("spin-1/2 Heisenberg MPO)" if mpo == "heisenberg" else "seeded normal(0,1) MPO tensors)"),
         "config": {"workload": f"serve_two_site_heff_{args.rule.lower()}", "mpo": mpo, "arrivals": f"poisson {args.rate:.0f}/s",
+                   "max_batch": args.max_batch,
                    "bond_dims": "lognormal median 512 sigma 0.7 clipped 32..4096, snapped to buckets",
                    "buckets": srv.buckets, "streams": args.streams, "slots": args.inflight,
                    "mean_D": float(ds.mean())},
@@ -243,12 +245,13 @@ def run_serve(args):
         "offered_rate": stats["offered_rate"],
         "tflops": stats["tflops"],
         "host_issue_ms_per_request": stats["host_issue_ms_per_request"],
+        "batches": stats["batches"],
         "issue": "hipGraph replay of the four contractions per request" if args.serve_graphs == "on"
                  else "prebuilt launch lists",
         "capacity": None if sat is None else {
             "matvecs_per_s": sat["matvecs_per_s"], "offered_rate": sat["offered_rate"], "tflops": sat["tflops"],
             "latency_ms_incl_queueing": {"p50": sat["p50_ms"], "p99": sat["p99_ms"]},
-            "host_issue_ms_per_request": sat["host_issue_ms_per_request"]},
+            "host_issue_ms_per_request": sat["host_issue_ms_per_request"], "batches": sat["batches"]},
         "cpu_baseline": cpu,
         "cold_compile_s": cold,
     }
@@ -362,6 +365,7 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: mean arrival rate (req/s)")
     ap.add_argument("--streams", type=int, default=4, help="serve: backend contexts (HIP streams)")
     ap.add_argument("--inflight", type=int, default=16, help="serve: request slots")
+    ap.add_argument("--max-batch", type=int, default=8, help="serve: largest batch of same-bucket requests")
     ap.add_argument("--serve-graphs", default="on", choices=["on", "off"],
                     help="serve: issue each request's contractions as one hipGraph replay")
     ap.add_argument("--saturate", type=float, default=64000.0,

@@ -279,6 +279,35 @@ int tnt_weighted_dot_async_noalloc(tnt_ctx* ctx, const tnt_buf* a, const tnt_buf
 /* y = alpha * x + beta * y over whole payloads. */
 int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double beta);
 
+/* ---- device scalars of a Krylov loop ---------------------------------------- */
+/* The scalar recurrences of CG / Lanczos kept on the device, so that a loop
+ * iteration queues its matvec, dots (tnt_weighted_dot_async into a scalar
+ * buffer), scalar steps and vector updates without waiting, and reads its
+ * scalars back once (the loop of tenet-krylov/src/cg.rs:64-190 over
+ * KrylovVector::dot_real / axpy / scale, vector.rs:2-20, with the
+ * synchronisation moved out of every dot).  No other reference counterpart. */
+/* y = a x + b y, a = alpha_factor * s[alpha_index] (alpha_factor when the
+ * index is negative), b likewise; beta_factor == 0 writes a x without reading
+ * y, alpha_factor == 0 writes b y without reading x; nothing is written when
+ * guard_index >= 0 and s[guard_index] == 0. */
+int tnt_axpby_dev(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, const tnt_buf* scalars, int64_t alpha_index,
+                  double alpha_factor, int64_t beta_index, double beta_factor, int64_t guard_index);
+enum tnt_scalar_code {
+  TNT_SCALAR_SET = 0,      /* s[out] = imm */
+  TNT_SCALAR_DIV = 1,      /* s[out] = s[a] / s[b] */
+  TNT_SCALAR_COPY = 2,     /* s[out] = s[a] */
+  TNT_SCALAR_RSQRT = 3,    /* s[out] = 1 / sqrt(s[a]) */
+  TNT_SCALAR_NEG = 4,      /* s[out] = -s[a] */
+  TNT_SCALAR_CG_GUARD = 5  /* s[out] = 1 if s[a] (curvature) and s[b] (step) are finite and, when
+                              imm != 0, s[a] > 0; else 0 */
+};
+typedef struct {
+  int32_t op, out, a, b;
+  double imm;
+} tnt_scalar_op;
+/* Runs up to 8 scalar steps in order on one device thread (stream-ordered). */
+int tnt_scalars(tnt_ctx* ctx, tnt_buf* scalars, const tnt_scalar_op* ops, size_t n_ops);
+
 /* ---- completion events (serving loop) -------------------------------------- */
 /* Record on the context stream; query returns 1 when all work queued before the
  * record has completed, 0 while it runs.  No reference counterpart. */

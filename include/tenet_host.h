@@ -116,6 +116,19 @@ double tnth_contract_flops(const tnth_contract* c);
  * Multi groups whose coefficients are U[t] = scale[t] * rhs[idx[t]]
  * (idx[t] < 0: U[t] = 0).  Returns TNT_E_UNSUPPORTED, with the failed
  * condition in tnth_last_error(), when the contraction is not eligible. */
+/* Stacked (batched) contraction: `copies` independent instances of the
+ * contraction d in one descriptor, for one tnt_contract call that runs them
+ * all with the launches of one (the serving loop's cross-request batching;
+ * no reference counterpart -- the reference contracts one tensor pair per
+ * call, tenet-tensors/src/contract/dynamic.rs:1241-1530).  shared: 0 = every
+ * copy reads the same lhs payload (given once, its transform run once), the
+ * rhs and the destination are stacked (copy c at c * length); 1 = the rhs is
+ * shared; 2 = nothing is shared.  The result owns its tables. */
+typedef struct tnth_batch tnth_batch;
+int tnth_contract_batch(const tnt_contract_desc* d, size_t copies, int shared, tnth_batch** out);
+const tnt_contract_desc* tnth_batch_desc(const tnth_batch* b);
+void tnth_batch_destroy(tnth_batch* b);
+
 typedef struct tnth_fusion tnth_fusion;
 int tnth_narrow_fusion(const tnt_contract_desc* d, tnth_fusion** out);
 void tnth_fusion_destroy(tnth_fusion* f);

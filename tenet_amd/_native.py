@@ -60,6 +60,13 @@ class TransformDesc(C.Structure):
                 ("n_inactive", C.c_size_t), ("inactive", C.POINTER(C.c_uint32))]
 
 
+class ScalarOp(C.Structure):
+    _fields_ = [("op", C.c_int32), ("out", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("imm", C.c_double)]
+
+
+SCALAR_SET, SCALAR_DIV, SCALAR_COPY, SCALAR_RSQRT, SCALAR_NEG, SCALAR_CG_GUARD = 0, 1, 2, 3, 4, 5
+
+
 class Region(C.Structure):
     _fields_ = [("offset", C.c_int64), ("len", C.c_int64)]
 
@@ -109,6 +116,8 @@ HIP_API = {
     "tnt_axpby": (C.c_int, [VP, VP, VP, C.c_double, C.c_double]),
     "tnt_weighted_dot_async": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ, VP, SZ]),
     "tnt_weighted_dot_async_noalloc": (C.c_int, [VP, VP, VP, C.POINTER(Region), C.POINTER(C.c_double), SZ, VP, SZ]),
+    "tnt_axpby_dev": (C.c_int, [VP, VP, VP, VP, C.c_int64, C.c_double, C.c_int64, C.c_double, C.c_int64]),
+    "tnt_scalars": (C.c_int, [VP, VP, C.POINTER(ScalarOp), SZ]),
     "tnt_event_create": (C.c_int, [VP, PP]),
     "tnt_event_record": (C.c_int, [VP, VP]),
     "tnt_event_query": (C.c_int, [VP, VP]),
@@ -167,6 +176,9 @@ HOST_API = {
     "tnth_contract_dst_space": (VP, [VP]),
     "tnth_contract_flops": (C.c_double, [VP]),
     "tnth_narrow_fusion": (C.c_int, [C.POINTER(ContractDesc), PP]),
+    "tnth_contract_batch": (C.c_int, [C.POINTER(ContractDesc), SZ, C.c_int, PP]),
+    "tnth_batch_desc": (C.POINTER(ContractDesc), [VP]),
+    "tnth_batch_destroy": (None, [VP]),
     "tnth_fusion_destroy": (None, [VP]),
     "tnth_fusion_desc": (C.POINTER(TransformDesc), [VP]),
     "tnth_fusion_gather": (SZ, [VP, C.POINTER(I64P), C.POINTER(C.POINTER(C.c_double))]),

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/snapshot.cpp", "csrc/host/host_capi.cpp",
-            "csrc/common/narrow_fusion.cpp"]
+            "csrc/common/narrow_fusion.cpp", "csrc/common/batch_desc.cpp"]
 HIP_SRC = ["csrc/hip/backend.hip", "csrc/hip/xform.hip", "csrc/hip/gemm.hip",
            "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip", "csrc/common/narrow_fusion.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -1632,6 +1632,43 @@ int tnt_axpby(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, double alpha, double b
   return TNT_OK;
 }
 
+int tnt_axpby_dev(tnt_ctx* ctx, tnt_buf* y, const tnt_buf* x, const tnt_buf* scalars, int64_t alpha_index,
+                  double alpha_factor, int64_t beta_index, double beta_factor, int64_t guard_index) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, y, "y")) return rc;
+  if (int rc = check_buf(ctx, x, "x")) return rc;
+  if (int rc = check_buf(ctx, scalars, "scalars")) return rc;
+  if (x->len != y->len) FAIL(ctx, TNT_E_SHAPE, "axpby payload lengths differ");
+  const int64_t ns = (int64_t)scalars->len;
+  if (alpha_index >= ns || beta_index >= ns || guard_index >= ns) FAIL(ctx, TNT_E_OOB, "scalar index outside the scalar buffer");
+  if (int rc = set_device(ctx)) return rc;
+  launch_axpby_dev(static_cast<double*>(y->ptr), static_cast<const double*>(x->ptr), (int64_t)y->len,
+                   static_cast<const double*>(scalars->ptr), alpha_index, alpha_factor, beta_index, beta_factor,
+                   guard_index, ctx->stream);
+  return launch_status(ctx);
+}
+
+int tnt_scalars(tnt_ctx* ctx, tnt_buf* scalars, const tnt_scalar_op* ops, size_t n_ops) {
+  if (!ctx) return TNT_E_INVALID;
+  if (int rc = check_buf(ctx, scalars, "scalars")) return rc;
+  if (n_ops > (size_t)kMaxScalarOps) FAIL(ctx, TNT_E_UNSUPPORTED, "at most 8 scalar steps per call");
+  if (n_ops && !ops) FAIL(ctx, TNT_E_INVALID, "null scalar program");
+  ScalarProgram prog{};
+  prog.n = (int32_t)n_ops;
+  const int64_t ns = (int64_t)scalars->len;
+  for (size_t i = 0; i < n_ops; ++i) {
+    const tnt_scalar_op& o = ops[i];
+    if (o.op < TNT_SCALAR_SET || o.op > TNT_SCALAR_CG_GUARD) FAIL(ctx, TNT_E_INVALID, "unknown scalar step");
+    if (o.out < 0 || o.out >= ns || o.a < 0 || o.a >= ns || o.b < 0 || o.b >= ns)
+      FAIL(ctx, TNT_E_OOB, "scalar index outside the scalar buffer");
+    prog.ops[i] = ScalarOp{o.op, o.out, o.a, o.b, o.imm};
+  }
+  if (n_ops == 0) return TNT_OK;
+  if (int rc = set_device(ctx)) return rc;
+  launch_scalars(static_cast<double*>(scalars->ptr), prog, ctx->stream);
+  return launch_status(ctx);
+}
+
 int tnt_graph_begin(tnt_ctx* ctx) {
   if (!ctx) return TNT_E_INVALID;
   if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "graph capture already active");

@@ -199,6 +199,22 @@ void launch_sum_partials(const double* partials, uint32_t n, double* out, hipStr
 void launch_axpby(double* y, const double* x, int64_t n, double alpha, double beta, hipStream_t s);
 void launch_fill(double* y, int64_t n, double value, hipStream_t s);
 void launch_zero_f32(float* y, int64_t n, hipStream_t s);
+void launch_axpby_dev(double* y, const double* x, int64_t n, const double* sc, int64_t ai, double af, int64_t bi,
+                      double bf, int64_t gi, hipStream_t s);
+// A short program of scalar steps on a device scalar buffer (Krylov loops).
+enum ScalarOpCode : int32_t {
+  SCALAR_SET = 0, SCALAR_DIV = 1, SCALAR_COPY = 2, SCALAR_RSQRT = 3, SCALAR_NEG = 4, SCALAR_CG_GUARD = 5
+};
+struct ScalarOp {
+  int32_t op, out, a, b;
+  double imm;
+};
+constexpr int kMaxScalarOps = 8;
+struct ScalarProgram {
+  ScalarOp ops[kMaxScalarOps];
+  int32_t n;
+};
+void launch_scalars(double* sc, const ScalarProgram& prog, hipStream_t s);
 void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,

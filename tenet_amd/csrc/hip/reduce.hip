@@ -100,6 +100,59 @@ void launch_fill(double* y, int64_t n, double value, hipStream_t s) {
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n, value);
 }
 
+// y = a x + b y with a = af * s[ai] (af when ai < 0) and b = bf * s[bi] (bf
+// when bi < 0); bf == 0 writes a x without reading y, af == 0 writes b y
+// without reading x.  Skipped entirely when gi >= 0 and s[gi] == 0 (a step a
+// Krylov loop rejected on the device).  Both products are rounded before the
+// sum (no fma), so an update equals the host loop's axpy (b = 1) and its
+// scale-then-axpy (a = 1) bit for bit.
+__global__ __launch_bounds__(256) void axpby_dev_kernel(double* y, const double* x, int64_t n,
+                                                        const double* __restrict__ sc, int64_t ai, double af,
+                                                        int64_t bi, double bf, int64_t gi) {
+#pragma clang fp contract(off)
+  if (gi >= 0 && sc[gi] == 0.0) return;
+  const double a = ai >= 0 ? af * sc[ai] : af, b = bi >= 0 ? bf * sc[bi] : bf;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    if (bf == 0.0) y[i] = a * x[i];
+    else if (af == 0.0) y[i] = b * y[i];
+    else y[i] = a * x[i] + b * y[i];
+  }
+}
+
+void launch_axpby_dev(double* y, const double* x, int64_t n, const double* sc, int64_t ai, double af, int64_t bi,
+                      double bf, int64_t gi, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(axpby_dev_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, x, n, sc, ai, af, bi, bf, gi);
+}
+
+// The scalar steps of a device Krylov loop, one thread, in order.
+__global__ void scalars_kernel(double* __restrict__ sc, ScalarProgram prog) {
+  for (int i = 0; i < prog.n; ++i) {
+    const ScalarOp& o = prog.ops[i];
+    const double a = sc[o.a], b = sc[o.b];
+    double r = 0.0;
+    switch (o.op) {
+      case SCALAR_DIV: r = a / b; break;
+      case SCALAR_COPY: r = a; break;
+      case SCALAR_RSQRT: r = 1.0 / sqrt(a); break;
+      case SCALAR_NEG: r = -a; break;
+      case SCALAR_CG_GUARD: {
+        // a = curvature p.Ap, b = step rho / curvature, imm != 0: check curvature
+        const bool ok = isfinite(a) && (o.imm == 0.0 || a > 0.0) && isfinite(b);
+        r = ok ? 1.0 : 0.0;
+        break;
+      }
+      default: r = o.imm; break;  // SCALAR_SET
+    }
+    sc[o.out] = r;
+  }
+}
+
+void launch_scalars(double* sc, const ScalarProgram& prog, hipStream_t s) {
+  hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(1), 0, s, sc, prog);
+}
+
 void launch_zero_f32(float* y, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(zero_f32_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, n);

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../../include/tenet_host.h"
+#include "../common/batch_desc.hpp"
 #include "../common/narrow_fusion.hpp"
 #include "structure.hpp"
 
@@ -223,6 +224,28 @@ void tnth_contract_destroy(tnth_contract* c) { delete c; }
 const tnt_contract_desc* tnth_contract_desc(const tnth_contract* c) { return &c->c->desc; }
 const tnth_space* tnth_contract_dst_space(const tnth_contract* c) { return &c->dst; }
 double tnth_contract_flops(const tnth_contract* c) { return c->c->flops; }
+
+struct tnth_batch {
+  tnt_batch::BatchedContract b;
+};
+
+int tnth_contract_batch(const tnt_contract_desc* d, size_t copies, int shared, tnth_batch** out) {
+  if (!d || !out) {
+    g_err = "null argument";
+    return TNT_E_INVALID;
+  }
+  auto* h = new tnth_batch;
+  if (!tnt_batch::batch_contract(*d, copies, shared, h->b, g_err)) {
+    delete h;
+    *out = nullptr;
+    return TNT_E_INVALID;
+  }
+  *out = h;
+  return TNT_OK;
+}
+
+const tnt_contract_desc* tnth_batch_desc(const tnth_batch* b) { return &b->b.desc; }
+void tnth_batch_destroy(tnth_batch* b) { delete b; }
 
 struct tnth_fusion {
   tnt_fuse::NarrowFusion f;

@@ -18,6 +18,8 @@ Index conventions (legs of V = bond, P = physical, M = MPO bond):
     X3 = X2 . W2  (w1, s2 contracted)    (a', w2 ; s1', s2', c)
     H theta = X3 . R (w2, c contracted)  (a' ; s1', s2', c')
 """
+import math
+
 import numpy as np
 
 from .tensor import TensorMap
@@ -111,34 +113,54 @@ class TwoSiteHeff:
                  env[l]._buf, env[r]._buf) for l, r, ax, bx, perm, nout, res in self.CHAIN]
 
 
-def lanczos(op, v0, iters=30, tol=1e-10):
+def lanczos(op, v0, iters=30, tol=1e-10, stats=None):
     """Lowest eigenvalue of a real symmetric operator by Lanczos with full
-    reorthogonalisation; vectors and FP64 inner products stay on the device.
+    reorthogonalisation (two modified Gram-Schmidt passes); vectors, FP64
+    inner products and the normalisations stay on the device.
 
-    op: callable TensorMap -> TensorMap (same space); v0: start vector.
-    Returns (eigenvalue, ritz residual norm, iterations).
+    Each iteration queues the matvec, alpha = q.w, the 2 (k+1) projections
+    (each dot into a device scalar, the subtraction reading it there), |w|^2
+    and 1/|w|, and reads alpha and |w|^2 back once for the host's tridiagonal
+    eigensolve; the normalisation of the next basis vector is queued after
+    that decision.  op: callable TensorMap -> TensorMap (same space); v0:
+    start vector.  Returns (eigenvalue, ritz residual norm, iterations);
+    ``stats`` (a dict, optional) receives the host synchronisations taken.
     """
-    q = v0.copy()
-    q.scale_(1.0 / q.norm())
-    Q = [q]
-    alphas, betas = [], []
-    theta_min, res = None, None
-    for k in range(iters):
-        w = op(Q[-1]).copy()
-        a = Q[-1].inner(w)
-        alphas.append(a)
-        # full reorthogonalisation (classical Gram-Schmidt twice)
-        for _ in range(2):
-            for qi in Q:
-                w.axpy_(qi, -qi.inner(w))
-        b = w.norm()
-        T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
-        evals, evecs = np.linalg.eigh(T)
-        theta_min = float(evals[0])
-        res = abs(b * evecs[-1, 0])
-        if res < tol or b < 1e-300:
-            return theta_min, res, k + 1
-        betas.append(b)
-        w.scale_(1.0 / b)
-        Q.append(w)
-    return theta_min, res, iters
+    from .krylov import DeviceScalars
+    rt = v0.rt
+    S = DeviceScalars(rt, n=2 * iters + 8)
+    A0, B0, C0, C1, INV = 0, iters, 2 * iters, 2 * iters + 1, 2 * iters + 2
+    try:
+        q = v0.copy()
+        S.dot(q, q, C0)
+        S.program((3, INV, C0, C0, 0.0))            # 1 / |v0|
+        S.axpby(q, q, ai=INV, bf=0.0)              # q = v0 / |v0|
+        Q = [q]
+        alphas, betas = [], []
+        theta_min, res = None, None
+        for k in range(iters):
+            w = op(Q[-1]).copy()
+            S.dot(Q[-1], w, A0 + k)
+            for _ in range(2):
+                for qi in Q:
+                    S.dot(qi, w, C1)
+                    S.axpby(w, qi, ai=C1, af=-1.0)  # w -= (q_i . w) q_i
+            S.dot(w, w, B0 + k)
+            S.program((3, INV, B0 + k, B0 + k, 0.0))
+            s = S.read()
+            alphas.append(float(s[A0 + k]))
+            b = math.sqrt(max(float(s[B0 + k]), 0.0))
+            T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
+            evals, evecs = np.linalg.eigh(T)
+            theta_min = float(evals[0])
+            res = abs(b * evecs[-1, 0])
+            if res < tol or b < 1e-300:
+                return theta_min, res, k + 1
+            betas.append(b)
+            S.axpby(w, w, ai=INV, bf=0.0)          # w / |w|
+            Q.append(w)
+        return theta_min, res, iters
+    finally:
+        if stats is not None:
+            stats["host_syncs"] = S.syncs
+        S.close()

@@ -10,23 +10,29 @@ Scheduling (host thread):
   * arrivals are admitted when a slot is free (at most `max_inflight` in
     flight) and the HBM footprint of the in-flight requests plus the new
     one fits the budget -- admission by HBM footprint;
-  * admitted requests are issued round-robin over `n_streams` backend
-    contexts (one HIP stream each, default priority), so requests that arrive
-    while others run overlap on the device;
+  * admitted requests of one bond-dimension bucket that are waiting together
+    are batched: up to `max_batch` of them run as one stacked contraction
+    chain (tnth_contract_batch: every copy's sector GEMMs in the same grouped
+    launches, every copy's recoupling in the same passes, the environments
+    read once), so a batch costs the launches of one request;
+  * batches (and single requests) are issued round-robin over `n_streams`
+    backend contexts (one HIP stream each, default priority), so work that
+    arrives while other work runs overlaps on the device;
   * completion is polled with stream events, freeing slot and footprint,
     after which waiting arrivals are admitted.
-Device buffers: a request holds one buffer set (theta, the three H_eff
-intermediates, the result) of its (stream, bucket) pair from the moment it is
-admitted until its completion event fires; sets are reused by later requests
-on that pair, and the footprint admission counts the bytes of the sets in
-flight.  The four tnt_contract calls of a buffer set are captured once, when
+Device buffers: a batch holds one buffer set (theta, the three H_eff
+intermediates and the result, each stacked over the batch's copies) of its
+(stream, bucket, batch size) from the moment it is admitted until its
+completion event fires; sets are reused by later batches of that key, and
+the footprint admission counts the bytes of the sets in flight.  The four tnt_contract calls of a buffer set are captured once, when
 the server warms up, as a hipGraph on its stream, so a request issues one fill
 (its seed), one graph launch, one stream-ordered dot (its result slot) and
 one event record (`graphs=False`, and sets built later while serving, replay
 the prebuilt argument lists of the four calls instead).  Every request
 computes on its own buffers with deterministic kernels, so its result is
-bit-for-bit the same alone or in any mix, and with or without graphs (tests
-in tests/test_gpu_heff.py).
+bit-for-bit the same alone or in any mix, batched or not, and with or without
+graphs (the stacked chain runs each copy's elements through the same kernels
+in the same order; tests in tests/test_gpu_heff.py).
 Latency = completion seen by the scheduler - arrival.
 """
 import ctypes as C
@@ -61,21 +67,52 @@ def default_buckets(dmin=32, dmax=4096, n=15):
 
 class _Req:
     __slots__ = ("rid", "arrival", "D", "seed", "ctx_index", "event", "bufs", "y_copy",
-                 "footprint", "t_issue", "t_done", "slot")
+                 "footprint", "t_issue", "t_done", "slot", "batch")
+
+
+class _Batch:
+    __slots__ = ("reqs", "ctx_index", "D", "event", "bufs", "footprint")
+
+
+class _Raw:
+    """A device payload buffer without a tensor space (stacked copies)."""
+
+    def __init__(self, rt, n, base=None, offset=0):
+        self.rt, self.n, self._base = rt, n, base
+        b = C.c_void_p()
+        if base is None:
+            rt._check(N.hip().tnt_buf_alloc(rt._ctx, max(1, n), N.F64, C.byref(b)))
+        else:
+            p = C.c_void_p()
+            rt._check(N.hip().tnt_buf_info(base._buf, C.byref(p), None, None, None))
+            rt._check(N.hip().tnt_buf_wrap(rt._ctx, C.c_void_p(p.value + 8 * offset), max(1, n), N.F64, C.byref(b)))
+        self._buf = b
+
+    def __del__(self):
+        try:
+            N.hip().tnt_buf_free(self.rt._ctx, self._buf)
+        except Exception:
+            pass
 
 
 class _BufSet:
-    """One request's device buffers on one (stream, bucket) pair and the
-    prebuilt tnt_contract argument tuples of the H_eff chain over them."""
-    __slots__ = ("theta", "scratch", "y", "calls", "bytes", "graph")
+    """One batch's device buffers on one (stream, bucket, batch size) and the
+    prebuilt tnt_contract argument tuples of the H_eff chain over them;
+    thetas / ys hold copy c's views."""
+    __slots__ = ("B", "theta", "scratch", "y", "thetas", "ys", "regions", "calls", "bytes", "graph", "keep")
 
 
 class HeffServer:
     def __init__(self, rule="U1", buckets=None, n_streams=4, max_inflight=16,
                  hbm_budget_bytes=48 << 30, precision="split_i8", env_seed=7, mpo="random",
-                 graphs=True):
+                 graphs=True, max_batch=8, batch_max_d=1024):
+        """max_batch: largest batch (a power of two; 1 disables batching);
+        batch_max_d: largest bucket that batches (larger requests fill the
+        device alone)."""
         self.rule = rule
         self.graphs = graphs
+        self.batch_sizes = [b for b in (1, 2, 4, 8, 16) if b <= max(1, max_batch)]
+        self.batch_max_d = batch_max_d
         self.precision = precision
         self.max_inflight = max_inflight
         self.budget = hbm_budget_bytes
@@ -91,9 +128,9 @@ class HeffServer:
         self.results = None
         self._prec = {"f64": N.PREC_F64, "f32": N.PREC_F32, "split_bf16": N.PREC_SPLIT_BF16,
                       "split_i8": N.PREC_SPLIT_I8}[precision]
-        self._free = {}                       # (stream, D) -> idle buffer sets
+        self._free = {}                       # (stream, D, batch size) -> idle buffer sets
         self._events = [[] for _ in self.rts]  # idle events per stream
-        self._dot = {}                        # D -> (regions, weights, count) of theta's sectors
+        self._dot = {}                        # D -> ((offset, len) per sector of theta, weights, count)
 
     # -- per-bucket structures (cold: compiled per context once) -------------
     def warm(self):
@@ -104,32 +141,73 @@ class HeffServer:
         addresses it was captured with."""
         for D in self.buckets:
             for k, rt in enumerate(self.rts):
-                bs = self._new_set(D, k)
-                self.flops[D] = self._view(D, k)._flops
-                self.fp[D] = bs.bytes
-                self._free.setdefault((k, D), []).append(bs)
+                for B in self._sizes(D):
+                    bs = self._new_set(D, k, B)
+                    if B == 1:
+                        self.flops[D] = self._view(D, k)._flops
+                        self.fp[D] = bs.bytes
+                    self._free.setdefault((k, D, B), []).append(bs)
                 rt.synchronize()
         if self.graphs:
-            for (k, _), sets in self._free.items():
+            for (k, _, _), sets in self._free.items():
                 for bs in sets:
                     if bs.graph is None:
                         bs.graph = self._capture(k, bs.calls)
                 self.rts[k].synchronize()
 
-    def _new_set(self, D, k):
+    def _sizes(self, D):
+        return self.batch_sizes if D <= self.batch_max_d else [1]
+
+    def _new_set(self, D, k, B=1):
         v = self._view(D, k)
+        rt = self.rts[k]
         bs = _BufSet()
-        bs.theta = TensorMap.zeros(self.rts[k], v.theta_space)
-        bs.scratch = {}
-        bs.y = v.apply(bs.theta, bs.scratch)
-        bs.calls = v.launch_list(bs.theta, bs.scratch)
-        bs.bytes = 8 * (bs.theta.space.len + sum(t.space.len for t in bs.scratch.values()))
+        bs.B = B
         bs.graph = None  # captured by warm(); sets built later, while serving, issue eagerly
         if D not in self._dot:
             secs = v.theta_space.sectors
-            regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * c) for _, o, r, c, _ in secs])
             ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
-            self._dot[D] = (regs, ws, len(secs))
+            self._dot[D] = ([(o, r * c) for _, o, r, c, _ in secs], ws, len(secs))
+        offs, ws, ns = self._dot[D]
+        n_theta = v.theta_space.len
+        # the dots run on copy c's views (theta and H theta of one request)
+        regs = (N.Region * max(1, ns))(*[N.Region(o, ln) for o, ln in offs])
+        bs.regions = [(regs, ws, ns)] * B
+        if B == 1:
+            bs.theta = TensorMap.zeros(rt, v.theta_space)
+            bs.scratch = {}
+            bs.y = v.apply(bs.theta, bs.scratch)
+            bs.calls = v.launch_list(bs.theta, bs.scratch)
+            bs.thetas, bs.ys = [bs.theta._buf], [bs.y._buf]
+            bs.bytes = 8 * (bs.theta.space.len + sum(t.space.len for t in bs.scratch.values()))
+            return bs
+        # stacked chain: environments shared, theta and every intermediate stacked
+        env = {"L": v.L, "W1": v.W1, "W2": v.W2, "R": v.R}
+        spaces = {"theta": v.theta_space}
+        bufs = {"theta": _Raw(rt, B * n_theta)}
+        bs.calls = []
+        for lhs, rhs, ax, bx, perm, nout, res in TwoSiteHeff.CHAIN:
+            ls = env[lhs].space if lhs in env else spaces[lhs]
+            rsp = env[rhs].space if rhs in env else spaces[rhs]
+            shared = 0 if lhs in env else 1
+            c = rt.contraction_batch(ls, rsp, ax, bx, perm, nout, B, shared)
+            spaces[res] = c.dst_space
+            bufs[res] = _Raw(rt, B * c.dst_space.len)
+            lb = env[lhs]._buf if lhs in env else bufs[lhs]._buf
+            rb = env[rhs]._buf if rhs in env else bufs[rhs]._buf
+            bs.calls.append((c.art, bufs[res]._buf, lb, rb))
+        bs.theta, bs.y = bufs["theta"], bufs["y"]
+        bs.scratch = {k_: b_ for k_, b_ in bufs.items() if k_ not in ("theta", "y")}
+        n_y = spaces["y"].len
+        bs.keep = [_Raw(rt, n_theta, bs.theta, c * n_theta) for c in range(B)] + \
+                  [_Raw(rt, n_y, bs.y, c * n_y) for c in range(B)]
+        bs.thetas = [x._buf for x in bs.keep[:B]]
+        bs.ys = [x._buf for x in bs.keep[B:]]
+        bs.bytes = 8 * sum(b_.n for b_ in bufs.values())
+        h = N.hip()
+        rt._check(h.tnt_buf_zero(rt._ctx, bs.theta._buf))
+        for art, d, a, b in bs.calls:  # run once: every table and workspace exists before capture
+            rt._check(h.tnt_contract(rt._ctx, art, d, a, b, 1.0, 0.0, self._prec))
         return bs
 
     def _capture(self, k, calls):
@@ -151,32 +229,36 @@ class HeffServer:
         rt._check(rc_end)
         return g
 
-    def _issue(self, r, out_buf, keep):
-        """Queue one request on its stream: fill theta, the four contractions,
-        the dot into the request's result slot, then a completion event."""
+    def _issue(self, bt, out_buf, keep):
+        """Queue one batch on its stream: fill each copy's theta, the four
+        (stacked) contractions, each request's dot into its result slot, then
+        one completion event."""
         h = N.hip()
-        k, D = r.ctx_index, r.D
+        k, D, B = bt.ctx_index, bt.D, len(bt.reqs)
         rt = self.rts[k]
-        free = self._free.setdefault((k, D), [])
-        bs = free.pop() if free else self._new_set(D, k)
+        free = self._free.setdefault((k, D, B), [])
+        bs = free.pop() if free else self._new_set(D, k, B)
         ctx = rt._ctx
-        rt._check(h.tnt_buf_fill_random(ctx, bs.theta._buf, r.seed & ((1 << 64) - 1), 1))
+        for c, r in enumerate(bt.reqs):
+            rt._check(h.tnt_buf_fill_random(ctx, bs.thetas[c], r.seed & ((1 << 64) - 1), 1))
         if bs.graph is not None:
             rt._check(h.tnt_graph_launch(ctx, bs.graph))
         else:
             for art, d, a, b in bs.calls:
                 rt._check(h.tnt_contract(ctx, art, d, a, b, 1.0, 0.0, self._prec))
-        regs, ws, ns = self._dot[D]
-        rt._check(h.tnt_weighted_dot_async(ctx, bs.theta._buf, bs.y._buf, regs, ws, ns, out_buf._buf, r.rid))
-        if keep:  # a copy of the result vector, read at completion (tests)
-            r.y_copy = TensorMap.zeros(rt, bs.y.space)
-            rt._check(h.tnt_axpby(ctx, r.y_copy._buf, bs.y._buf, 1.0, 0.0))
+        for c, r in enumerate(bt.reqs):
+            regs, ws, ns = bs.regions[c]
+            rt._check(h.tnt_weighted_dot_async(ctx, bs.thetas[c], bs.ys[c], regs, ws, ns, out_buf._buf, r.rid))
+            if keep:  # a copy of the result vector, read at completion (tests)
+                r.y_copy = TensorMap.zeros(rt, self._view(D, k).theta_space)
+                src = bs.y._buf if B == 1 else bs.ys[c]
+                rt._check(h.tnt_axpby(ctx, r.y_copy._buf, src, 1.0, 0.0))
         ev = self._events[k].pop() if self._events[k] else None
         if ev is None:
             ev = C.c_void_p()
             rt._check(h.tnt_event_create(ctx, C.byref(ev)))
         rt._check(h.tnt_event_record(ctx, ev))
-        r.bufs, r.event = bs, ev
+        bt.bufs, bt.event = bs, ev
 
     def _view(self, D, k):
         key = (D, k)
@@ -193,61 +275,94 @@ class HeffServer:
         out_buf = [TensorMap.zeros(rt, self._vector_space(n)) for rt in self.rts]
         slots = deque(range(self.max_inflight))
         pending = deque(sorted(enumerate(trace), key=lambda x: x[1][0]))
-        inflight = []
+        waiting = {}    # bucket -> arrived, not yet admitted requests (arrival order)
+        inflight = []   # batches
         used = 0
         done = []
         next_ctx = 0
         results = {}
         issue_s = 0.0
         max_inflight_seen = 0
+        batch_hist = {}
+        max_used = 0
         for rt in self.rts:
             rt.synchronize()
         t0 = time.perf_counter()
-        while pending or inflight:
+        while pending or waiting or inflight:
             now = time.perf_counter() - t0
-            # admit arrivals (slot + HBM footprint)
-            while pending and pending[0][1][0] <= now and slots:
-                i, (arr, D, seed) = pending[0]
-                fp = self.fp[D]
-                if used + fp > self.budget and inflight:
-                    break
-                pending.popleft()
-                r = _Req()
-                r.rid, r.arrival, r.D, r.seed = i, arr, D, seed
-                r.footprint = fp
-                r.slot = slots.popleft()
-                r.ctx_index = next_ctx
-                next_ctx = (next_ctx + 1) % len(self.rts)
-                r.t_issue = time.perf_counter()
-                self._issue(r, out_buf[r.ctx_index], keep_results)
-                issue_s += time.perf_counter() - r.t_issue
-                used += fp
-                inflight.append(r)
-            max_inflight_seen = max(max_inflight_seen, len(inflight))
-            # retire finished requests
+            # admit the arrivals waiting now (slot + HBM footprint), batching
+            # requests of one bucket: a batch is the largest batch size not
+            # above the bucket's waiting requests and the free slots
+            while pending and pending[0][1][0] <= now:
+                item = pending.popleft()
+                waiting.setdefault(item[1][1], deque()).append(item)
+            blocked = False
+            for D in sorted(waiting, key=lambda d: waiting[d][0][1][0]):  # oldest bucket first
+                items = waiting[D]
+                while items and slots and not blocked:
+                    # batch only while every stream is busy: an idle stream
+                    # serves a lone request sooner than a batch would
+                    limit = min(len(items), len(slots)) if len(inflight) >= len(self.rts) else 1
+                    # the largest batch whose buffers fit the HBM budget; one
+                    # request is admitted over budget only when nothing runs
+                    fits = [b for b in self._sizes(D) if b <= limit and used + b * self.fp[D] <= self.budget]
+                    if not fits and not inflight:
+                        fits = [1]
+                    if not fits:
+                        blocked = True  # wait for memory to free up
+                        break
+                    B = max(fits)
+                    fp = self.fp[D] * B
+                    bt = _Batch()
+                    bt.reqs, bt.D, bt.footprint = [], D, fp
+                    for _ in range(B):
+                        i, (arr, _, seed) = items.popleft()
+                        r = _Req()
+                        r.rid, r.arrival, r.D, r.seed = i, arr, D, seed
+                        r.footprint = self.fp[D]
+                        r.slot = slots.popleft()
+                        r.batch = bt
+                        bt.reqs.append(r)
+                    bt.ctx_index = next_ctx
+                    next_ctx = (next_ctx + 1) % len(self.rts)
+                    t_i = time.perf_counter()
+                    for r in bt.reqs:
+                        r.ctx_index, r.t_issue = bt.ctx_index, t_i
+                    self._issue(bt, out_buf[bt.ctx_index], keep_results)
+                    issue_s += time.perf_counter() - t_i
+                    used += fp
+                    max_used = max(max_used, used)
+                    inflight.append(bt)
+                    batch_hist[B] = batch_hist.get(B, 0) + 1
+                if not items:
+                    del waiting[D]
+            max_inflight_seen = max(max_inflight_seen, sum(len(b.reqs) for b in inflight))
+            # retire finished batches
             still = []
-            for r in inflight:
-                rt = self.rts[r.ctx_index]
-                q = h.tnt_event_query(rt._ctx, r.event)
+            for bt in inflight:
+                rt = self.rts[bt.ctx_index]
+                q = h.tnt_event_query(rt._ctx, bt.event)
                 if q < 0:
                     rt._check(q)
                 if q == 1:
-                    r.t_done = time.perf_counter() - t0
-                    if keep_results:
-                        results[(r.rid, "y")] = r.y_copy.to_numpy()
-                        r.y_copy = None
-                    self._events[r.ctx_index].append(r.event)
-                    self._free[(r.ctx_index, r.D)].append(r.bufs)
-                    used -= r.footprint
-                    slots.append(r.slot)
-                    done.append(r)
-                    r.bufs = r.event = None
+                    t_done = time.perf_counter() - t0
+                    for r in bt.reqs:
+                        r.t_done = t_done
+                        if keep_results:
+                            results[(r.rid, "y")] = r.y_copy.to_numpy()
+                            r.y_copy = None
+                        slots.append(r.slot)
+                        done.append(r)
+                    self._events[bt.ctx_index].append(bt.event)
+                    self._free[(bt.ctx_index, bt.D, len(bt.reqs))].append(bt.bufs)
+                    used -= bt.footprint
+                    bt.bufs = bt.event = None
                 else:
-                    still.append(r)
+                    still.append(bt)
             inflight = still
             # idle until the next arrival: sleep through long gaps, spin the last
             # 0.2 ms (a sleep can overshoot by tens of microseconds)
-            if not inflight and pending and pending[0][1][0] - now > 4e-4:
+            if not inflight and not waiting and pending and pending[0][1][0] - now > 4e-4:
                 time.sleep(pending[0][1][0] - now - 2e-4)
         wall = time.perf_counter() - t0
         for k, rt in enumerate(self.rts):
@@ -269,6 +384,8 @@ class HeffServer:
             "offered_rate": n / max(trace[-1][0], 1e-9),
             "host_issue_ms_per_request": issue_s / max(1, n) * 1e3,
             "max_inflight": max_inflight_seen,
+            "batches": dict(sorted(batch_hist.items())),
+            "max_footprint_bytes": max_used,
         }
 
     # -- helpers -----------------------------------------------------------------
@@ -284,7 +401,7 @@ class HeffServer:
             for ev in evs:
                 h.tnt_event_destroy(self.rts[k]._ctx, ev)
         self._events = [[] for _ in self.rts]
-        for (k, _), sets in self._free.items():
+        for (k, _, _), sets in self._free.items():
             for bs in sets:
                 if bs.graph is not None:
                     h.tnt_graph_destroy(self.rts[k]._ctx, bs.graph)

@@ -301,6 +301,26 @@ class _HostContraction:
             pass
 
 
+class _HostBatch:
+    """`copies` stacked instances of a host contraction (tnth_contract_batch);
+    shared: 0 = the lhs is shared by every copy, 1 = the rhs, 2 = neither."""
+
+    def __init__(self, host, copies, shared):
+        out = C.c_void_p()
+        N.check_host(N.host().tnth_contract_batch(host.desc, copies, shared, C.byref(out)))
+        self._h = out
+        self.host = host
+        self.copies, self.shared = copies, shared
+        self.desc = N.host().tnth_batch_desc(out)
+        self.flops = host.flops * copies
+
+    def __del__(self):
+        try:
+            N.host().tnth_batch_destroy(self._h)
+        except Exception:
+            pass
+
+
 _HOST_CONTRACTIONS = {}
 
 
@@ -315,7 +335,7 @@ def _host_contraction(key, *args):
 class _Contraction:
     def __init__(self, rt, host):
         self.host = host
-        self.dst_space = host.dst_space
+        self.dst_space = getattr(host, "dst_space", None)
         self.flops = host.flops
         self.desc = host.desc
         art = C.c_void_p()
@@ -382,6 +402,19 @@ class Runtime:
         if c is None:
             host = _host_contraction(key, a, b, list(axes_a), list(axes_b), list(out_perm), out_nout, adj_a, adj_b)
             c = _Contraction(self, host)
+            self._contractions[key] = c
+        return c
+
+    def contraction_batch(self, a, b, axes_a, axes_b, out_perm, out_nout, copies, shared, adj_a=False, adj_b=False):
+        """Device artifact of `copies` stacked instances of a contraction (one
+        tnt_contract call runs them all); see _HostBatch for `shared`."""
+        key = (a.key(), b.key(), tuple(axes_a), tuple(axes_b), tuple(out_perm), out_nout, adj_a, adj_b,
+               "batch", copies, shared)
+        c = self._contractions.get(key)
+        if c is None:
+            one = self.contraction(a, b, axes_a, axes_b, out_perm, out_nout, adj_a, adj_b)
+            c = _Contraction(self, _HostBatch(one.host, copies, shared))
+            c.dst_space = one.dst_space  # the space of one copy
             self._contractions[key] = c
         return c
 
@@ -600,6 +633,25 @@ class TensorMap:
 
     def norm(self):
         return math.sqrt(max(self.inner(self), 0.0))
+
+    def _dot_tables(self):
+        """Region and weight arrays of the weighted inner product, per space."""
+        t = getattr(self.space, "_dot_tables", None)
+        if t is None:
+            secs = self.space.sectors
+            regs = (N.Region * max(1, len(secs)))(*[N.Region(o, r * k) for _, o, r, k, _ in secs])
+            ws = (C.c_double * max(1, len(secs)))(*[d for *_, d in secs])
+            t = self.space._dot_tables = (regs, ws, len(secs))
+        return t
+
+    def inner_async(self, other, scalars, index):
+        """Queue sum_c dim(c) <A_c, B_c> into scalars[index] on the device (no
+        host synchronisation; KrylovVector::dot_real, tenet-krylov/src/vector.rs:13)."""
+        if self.space != other.space:
+            raise TenetError(-3, "inner: spaces differ")
+        regs, ws, n = self._dot_tables()
+        self.rt._check(N.hip().tnt_weighted_dot_async(self.rt._ctx, self._buf, other._buf, regs, ws, n,
+                                                      scalars, index))
 
     def tr(self):
         """sum_c dim(c) tr(block_c) for codomain == domain (host reduction of a download)."""

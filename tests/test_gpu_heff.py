@@ -172,15 +172,21 @@ def test_dot_async_matches_sync_and_events(rt):
     assert h.tnt_weighted_dot_async(rt._ctx, a._buf, b._buf, regs, ws, len(secs), out._buf, 4) == N.E_OOB
 
 
-def test_serving_same_alone_and_in_mix():
+@pytest.mark.parametrize("rule,precision", [("U1", "split_bf16"), ("U1", "split_i8"), ("SU2", "split_i8"),
+                                            ("SU2", "f64")])
+def test_serving_same_alone_and_in_mix(rule, precision):
+    """Requests arriving together are batched (stacked chains of up to 4 copies)
+    and overlap on 3 streams; each result equals the request served alone
+    (batch of one) bit for bit."""
     buckets = [32, 64, 96]
-    srv = HeffServer("U1", buckets=buckets, n_streams=3, max_inflight=6, precision="split_bf16")
+    srv = HeffServer(rule, buckets=buckets, n_streams=3, max_inflight=12, precision=precision, max_batch=4)
     try:
         srv.warm()
         trace = poisson_trace(18, rate=1e6, median_d=64, sigma=0.6, dmin=32, dmax=96, seed=5, buckets=buckets)
         stats = srv.serve(trace, keep_results=True)
         mix = dict(srv.results)
         assert stats["requests"] == 18 and stats["p99_ms"] >= stats["p50_ms"] > 0
+        assert max(stats["batches"]) > 1, stats["batches"]  # batches formed
         for i, (_, D, seed) in enumerate(trace):
             srv.serve([(0.0, D, seed)], keep_results=True)
             alone = srv.results
@@ -192,8 +198,37 @@ def test_serving_same_alone_and_in_mix():
         th = TensorMap.rand(srv.rts[0], v.theta_space, seed=seed, normal=True)
         want = _o(th).inner(OTensor(ospace(v.theta_space), mix[(0, "y")]))
         assert abs(mix[0] - want) <= 1e-12 * max(1.0, abs(want))
+        # the matvec itself against the oracle chain (split GEMMs within 1e-10)
+        y_want = _oracle_apply(v, _o(th)).data
+        assert rel(mix[(0, "y")], y_want) < (1e-13 if precision == "f64" else 1e-10)
     finally:
         srv.close()
+
+
+def test_serving_hbm_gate_serialises_and_keeps_results():
+    """With an HBM budget below two requests' buffers the server admits one
+    request at a time (never more than the budget in flight, batching off by
+    necessity); the results equal those of an unconstrained run."""
+    buckets = [64]
+    trace = poisson_trace(8, rate=1e6, median_d=64, sigma=0.1, dmin=64, dmax=64, seed=3, buckets=buckets)
+    res = {}
+    for gated in (False, True):
+        srv = HeffServer("U1", buckets=buckets, n_streams=2, max_inflight=8, precision="split_i8", max_batch=4)
+        try:
+            srv.warm()
+            if gated:
+                srv.budget = int(1.5 * srv.fp[64])
+            st = srv.serve(trace, keep_results=True)
+            res[gated] = (dict(srv.results), st, srv.fp[64])
+        finally:
+            srv.close()
+    (free, st_free, fp), (gate, st_gate, _) = res[False], res[True]
+    assert st_gate["max_inflight"] == 1 and st_gate["batches"] == {1: 8}
+    assert st_gate["max_footprint_bytes"] == fp  # one request's buffers at a time
+    assert st_free["max_inflight"] > 1 and st_free["max_footprint_bytes"] > 1.5 * fp
+    for i in range(len(trace)):
+        assert gate[i] == free[i], i
+        assert np.array_equal(gate[(i, "y")], free[(i, "y")]), i
 
 
 def test_serving_graph_issue_matches_eager_issue():
@@ -217,3 +252,43 @@ def test_serving_graph_issue_matches_eager_issue():
     for i in range(len(trace)):
         assert res[True][i] == res[False][i], i
         assert np.array_equal(res[True][(i, "y")], res[False][(i, "y")]), i
+
+
+def _cg_cases(H, sc):
+    from tenet_amd.krylov import CgOptions, DeviceVector
+    pos = lambda v: DeviceVector(H.gram(v.t, sc))  # noqa: E731
+    neg = lambda v: DeviceVector(H.gram(v.t, sc).copy().scale_(-1.0))  # noqa: E731
+    return [("converge", pos, CgOptions(rtol=1e-12, damping=0.5)),
+            ("max_iter", pos, CgOptions(rtol=1e-14, max_iter=3)),
+            ("non_positive_curvature", neg, CgOptions()),
+            ("unchecked_negative", neg, CgOptions(check_curvature=False, max_iter=4)),
+            ("invalid_options", pos, CgOptions(rtol=-1.0))]
+
+
+@pytest.mark.parametrize("rule_name,D", [("U1", 24), ("SU2", 24)])
+def test_cg_device_loop_equals_host_synchronous_loop(rt, rule_name, D):
+    """The stream-ordered CG (scalars on the device, one read per iteration)
+    takes the same decisions on the same FP64 values as the loop with a host
+    value per dot: identical statistics and a bit-identical solution, in
+    convergence, iteration-limit, curvature-breakdown and invalid-option cases."""
+    from tenet_amd.krylov import DeviceVector, cg_device, cg_generic
+    H = TwoSiteHeff(rt, rule_name, D, seed=5)
+    b = TensorMap.rand(rt, H.theta_space, seed=13, normal=True)
+    for name, op, opt in _cg_cases(H, {}):
+        dev = cg_device(op, DeviceVector(b), opt)
+        host = cg_generic(op, DeviceVector(b), opt)
+        for f in ("iterations", "matvecs", "converged", "initial_residual", "final_residual", "tolerance",
+                  "breakdown"):
+            assert getattr(dev.stats, f) == getattr(host.stats, f), (name, f, dev.stats, host.stats)
+        assert np.array_equal(dev.solution.t.to_numpy(), host.solution.t.to_numpy()), name
+        assert dev.stats.host_syncs == 1 + dev.stats.iterations + (
+            1 if dev.stats.breakdown is not None and dev.stats.breakdown.value.endswith("Curvature") else 0), name
+    assert cg_device(_cg_cases(H, {})[2][1], DeviceVector(b)).solution.t.norm() == 0.0  # rejected step
+
+
+def test_lanczos_one_host_sync_per_iteration(rt):
+    H = TwoSiteHeff(rt, "U1", 24, seed=5)
+    v0 = TensorMap.rand(rt, H.theta_space, seed=9, normal=True)
+    st = {}
+    ev, res, iters = lanczos(lambda v: H.gram(v, {}), v0, iters=12, tol=0.0, stats=st)
+    assert iters == 12 and st["host_syncs"] == 12
