@@ -69,7 +69,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, D, budget_s=20.0, wl=None):
+def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap"):
     """Time the oracle (numpy FP64, host cores) on the same workload.  With the
     device workload `wl` given, the oracle's operands are its downloaded
     payloads, so the parity check compares results of identical inputs."""
@@ -123,11 +123,18 @@ def cpu_baseline(workload, D, budget_s=20.0, wl=None):
         _, V = su2_fmove_space(8, 8)  # bounded sample: multiplicity floor(8/(2j+1))
         oV = OL(list(zip(V.ids, V.degs)))
         sp = OH(orule, [oV, oV], [oV, oV])
-        A, B = OTensor.rand(sp, 1), OTensor.rand(sp, 2)
+        B = OTensor.rand(sp, 2)
+        if fmove == "swap":
+            A = OTensor.rand(OH(orule, [oV, oV, oV], [oV]), 1)
 
-        def step():
-            return A.contract([1, 3], B, [2, 0], [0, 1, 2, 3], 2)
-        sample = "SU(2) F-move contraction at multiplicity floor(8/(2j+1)) (bounded sample)"
+            def step():
+                return A.permute([0, 2, 1], [3]).contract([2, 3], B, [2, 0], [0, 1, 2, 3], 2)
+        else:
+            A = OTensor.rand(sp, 1)
+
+            def step():
+                return A.contract([1, 3], B, [2, 0], [0, 1, 2, 3], 2)
+        sample = f"SU(2) F-move ({fmove} form) + contraction at multiplicity floor(8/(2j+1)) (bounded sample)"
     times = []
     out = None
     t_start = time.perf_counter()
@@ -259,6 +266,109 @@ def run_serve(args):
     srv.close()
 
 
+# The reference's published warm microbench (µs / iteration; TeNeT column,
+# Apple M4 Max, Accelerate, one thread), benchmarks/history/
+# contraction-microbench-2026-07-11.md:51-59 (d = 4) and :68-76 (d = 16).
+MICROBENCH_PUBLISHED = {("U1", 4): (4.10, 9.72, 12.43), ("SU2", 4): (7.19, 30.02, 44.63),
+                        ("U1", 16): (1886.1, 2645.3, 3281.0), ("SU2", 16): (7741.3, 11915.0, 15075.6)}
+# microbench_fusion.rs:24-50: (name, lhs axes, rhs axes, output axes)
+MICROBENCH_WORKLOADS = (("compose", [2, 3], [0, 1], [0, 1, 2, 3]), ("swap", [3, 2], [0, 1], [0, 1, 2, 3]),
+                        ("swap+out", [3, 2], [0, 1], [1, 0, 2, 3]))
+
+
+def run_microbench(args):
+    """The reference's contraction microbench (tenet-tensors/examples/
+    microbench_fusion.rs): A, B in V (x) V <- V (x) V, uniform degeneracy d,
+    U(1) {-1, 0, 1} and SU(2) {0, 1/2, 1}, fills (i % 17) 0.25 - 2 and
+    (i % 13) 0.5 - 3, compose / swap / swap+out, warm.  Per cell: µs per
+    iteration of back-to-back calls (the device stream keeps the calls in
+    order; timed to a device synchronisation) and of calls each followed by
+    a synchronisation, the checksum (sum of the destination), and the
+    reference's published TeNeT figure beside it.  One JSON line."""
+    import math as _m
+
+    import numpy as np
+
+    from tenet_amd import _native as N
+    from tenet_amd.tensor import GradedSpace, HomSpace, Rule, Runtime, TensorMap
+    rt = Runtime(0)
+    h = N.hip()
+    kn, kms, kw = C.c_size_t(), C.c_double(), C.c_double()
+    cells = []
+    for d in (4, 16):
+        for sym, pairs in (("U1", [(-1, d), (0, d), (1, d)]), ("SU2", [(0, d), (1, d), (2, d)])):
+            rule = Rule(sym)
+            V = GradedSpace(rule, pairs)
+            sp = HomSpace(rule, [V, V], [V, V])
+            n = sp.len
+            A = TensorMap.from_numpy(rt, sp, (np.arange(n) % 17) * 0.25 - 2.0)
+            B = TensorMap.from_numpy(rt, sp, (np.arange(n) % 13) * 0.5 - 3.0)
+            for (name, la, lb, perm), ref_us in zip(MICROBENCH_WORKLOADS, MICROBENCH_PUBLISHED[(sym, d)]):
+                t = time.perf_counter()
+                out = A.contract(la, B, lb, perm, 2)
+                rt.synchronize()
+                cold = time.perf_counter() - t
+                for _ in range(3):
+                    A.contract(la, B, lb, perm, 2, out=out)
+                rt.synchronize()
+                iters, t0 = 0, time.perf_counter()
+                while time.perf_counter() - t0 < args.min_ms * 1e-3 or iters < 10:
+                    A.contract(la, B, lb, perm, 2, out=out)
+                    iters += 1
+                rt.synchronize()
+                us = (time.perf_counter() - t0) * 1e6 / iters
+                ns, t1 = 0, time.perf_counter()
+                while time.perf_counter() - t1 < args.min_ms * 1e-3 or ns < 10:
+                    A.contract(la, B, lb, perm, 2, out=out)
+                    rt.synchronize()
+                    ns += 1
+                us_sync = (time.perf_counter() - t1) * 1e6 / ns
+                # the same calls straight through the C ABI (what a compiled
+                # caller pays: no Python frontend), and the device time of the
+                # kernels of one call (HIP events around each kernel)
+                c = rt.contraction(A.space, B.space, la, lb, perm, 2)
+                fn, args_ = h.tnt_contract, (rt._ctx, c.art, out._buf, A._buf, B._buf, 1.0, 0.0, 0)
+                nn, t3 = 0, time.perf_counter()
+                while time.perf_counter() - t3 < args.min_ms * 1e-3 or nn < 10:
+                    fn(*args_)
+                    nn += 1
+                rt.synchronize()
+                us_native = (time.perf_counter() - t3) * 1e6 / nn
+                rt._check(h.tnt_ctx_reset_stats(rt._ctx))
+                rt._check(h.tnt_ctx_set_timing(rt._ctx, 1))
+                for _ in range(20):
+                    fn(*args_)
+                rt.synchronize()
+                rt._check(h.tnt_ctx_set_timing(rt._ctx, 0))
+                dev_ms = 0.0
+                for kind in (0, 1):  # recoupling passes + FP64 GEMM table runs
+                    rt._check(h.tnt_ctx_kernel_stats(rt._ctx, kind, C.byref(kn), C.byref(kms), C.byref(kw)))
+                    dev_ms += kms.value
+                checksum = float(out.to_numpy().sum())
+                cells.append({"symmetry": sym, "d": d, "workload": name, "us_per_iter": us,
+                              "us_per_call_synchronised": us_sync, "us_per_call_c_abi": us_native,
+                              "device_us_per_call": dev_ms * 1e3 / 20, "iters": iters, "checksum": checksum,
+                              "cold_ms": cold * 1e3, "reference_us": ref_us, "ratio_vs_reference": us / ref_us})
+                log(f"[bench] microbench {sym} d={d} {name}: {us:.2f} us/iter ({us_sync:.2f} synchronised, "
+                    f"{us_native:.2f} through the C ABI, {dev_ms * 1e3 / 20:.2f} on the device), "
+                    f"checksum {checksum:.6f}; reference {ref_us} us")
+    rt.close()
+    gm = _m.exp(sum(_m.log(c["us_per_iter"]) for c in cells) / len(cells))
+    gm_ref = _m.exp(sum(_m.log(c["reference_us"]) for c in cells) / len(cells))
+    rec = {"metric": "contraction microbench us/iteration (warm), geometric mean over cells",
+           "value": gm, "unit": "us", "n_gpus": 1, "steps": len(cells), "warmup": 3,
+           "ms_per_step": gm * 1e-3, "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": gm / gm_ref, "dtype": "f64",
+           "data": "synthetic (the reference's fills (i%17)*0.25-2 and (i%13)*0.5-3)",
+           "config": {"workload": "microbench_fusion", "symmetries": "U1 {-1,0,1}, SU2 {0,1/2,1}",
+                      "degeneracies": [4, 16], "workloads": [w[0] for w in MICROBENCH_WORKLOADS],
+                      "min_ms_per_cell": args.min_ms},
+           "baseline_source": "benchmarks/history/contraction-microbench-2026-07-11.md (TeNeT column, "
+                              "Apple M4 Max, 1 thread)",
+           "cells": cells}
+    print(json.dumps(rec), flush=True)
+
+
 def run_gemm_vs_vendor(args):
     """The core GEMM jobs of the U(1) environment update (L.A and Adag.T2 at
     bond dimension D) on two FP64 routes over the same operand sizes: this
@@ -353,12 +463,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="u1_env",
                     choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
-                             "gemm_vs_vendor"])
+                             "gemm_vs_vendor", "microbench"])
+    ap.add_argument("--min-ms", type=float, default=200.0, help="microbench: timed milliseconds per cell")
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16", "split_i8"],
                     help="GEMM precision (default: split_i8 for serve, f64 otherwise)")
     ap.add_argument("--D", type=int, default=4096)
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
+    ap.add_argument("--fmove", default="swap", choices=["swap", "cross"],
+                    help="su2_fmove form: codomain swap of a 3<-1 tensor then contract, or A[1,3].B[2,0]")
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--rule", default="U1", choices=["U1", "SU2"], help="serve: symmetry of the chain")
     ap.add_argument("--requests", type=int, default=4000, help="serve: requests in the timed trace")
@@ -390,6 +503,9 @@ def main():
         return
     if args.workload == "gemm_vs_vendor":
         run_gemm_vs_vendor(args)
+        return
+    if args.workload == "microbench":
+        run_microbench(args)
         return
 
     from tenet_amd import _native as N
@@ -476,6 +592,13 @@ def main():
         else:
             wl.capture()
         log(f"[bench] batched: {wl.n_jobs} sector GEMMs, {args.precision} rel. Frobenius error vs FP64 {batched_err:.3e}")
+    elif args.fmove == "swap":
+        from tenet_amd.workloads import SU2FMoveSwapContract
+        wl = SU2FMoveSwapContract(rt, args.mult, 8, args.precision)
+        cfg = {"workload": "su2_fmove_swap_contract", "spins": "2j=0..8", "multiplicity": f"floor({args.mult}/(2j+1))",
+               "contraction": "A' = permute(A, (0,2,1), (3,)) [V(x)V(x)V <- V, inner line recoupled], "
+                              "C = A'[2,3] . B[2,0]",
+               "fmove_groups": {"single": wl.transform.n_single, "multi": wl.transform.n_multi}}
     else:
         wl = SU2FMoveContract(rt, args.mult, 8, args.precision)
         cfg = {"workload": "su2_fmove_contract", "spins": "2j=0..8", "multiplicity": f"floor({args.mult}/(2j+1))",
@@ -553,12 +676,38 @@ def main():
         f"{xf_launches} launches ({xf_ms / args.steps:.3f} ms/step); dominant kernel {dom['kernel']}: "
         f"{dom['launches']} launches, {dom['ms'] / max(1, dom['launches']):.4f} ms avg")
 
+    r_swap = None
+    if args.workload == "su2_fmove":
+        # the plain 2 <- 2 codomain swap: an R-move only (Single groups), timed
+        # apart from the step, on a tensor of the same leg space
+        from tenet_amd.tensor import TensorMap as _TM
+        X = _TM.rand(rt, [wl.V, wl.V], [wl.V, wl.V], seed=7)
+        tr = rt.transform(X.space, (1, 0), (2, 3))
+        Y = X.permute((1, 0), (2, 3))
+        rt.synchronize()
+        rt._check(h.tnt_ctx_reset_stats(rt._ctx))
+        rt._check(h.tnt_ctx_set_timing(rt._ctx, 1))
+        for _ in range(args.steps):
+            X.permute((1, 0), (2, 3), out=Y)
+        rt.synchronize()
+        rt._check(h.tnt_ctx_set_timing(rt._ctx, 0))
+        rl, rms, rbytes = stats(0)
+        r_gbs = rbytes / (rms * 1e-3) / 1e9 if rms else 0.0
+        r_swap = {"transform": "permute(A, (1,0), (2,3)) on V(x)V <- V(x)V", "single": tr.n_single,
+                  "multi": tr.n_multi, "launches": rl, "ms_per_call": rms / args.steps,
+                  "bytes_per_call": rbytes / args.steps, "achieved": r_gbs, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": r_gbs / HBM_PEAK_GBS}
+        log(f"[bench] R-only 2<-2 swap: {tr.n_single} single / {tr.n_multi} multi groups, "
+            f"{rms / args.steps:.3f} ms per call, {r_gbs:.0f} GB/s")
+        del X, Y
+
     cpu = None
     parity = None
     if args.cpu_baseline == "on":
         import numpy as np
         t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D,
-                                                       wl=wl if args.workload in ("u1_env", "su2_env") else None)
+                                                       wl=wl if args.workload in ("u1_env", "su2_env") else None,
+                                                       fmove=args.fmove)
         if args.workload == "batched":
             nsample = len(out)
             flops_sample = sum(c.flops for *_, c in wl.pairs[:nsample])
@@ -570,8 +719,8 @@ def main():
         elif args.workload == "su2_fmove":
             flops_sample = None
             from tenet_amd.tensor import Runtime as _R  # noqa: F401
-            from tenet_amd.workloads import SU2FMoveContract as _S
-            small = _S(rt, 8, 8, "f64")
+            from tenet_amd.workloads import SU2FMoveContract as _S, SU2FMoveSwapContract as _S3
+            small = (_S3 if args.fmove == "swap" else _S)(rt, 8, 8, "f64")
             flops_sample = small.flops
             got = small.run().to_numpy()
             parity = float(np.linalg.norm(got - out.data) / np.linalg.norm(out.data))
@@ -609,6 +758,7 @@ def main():
                        "avg_launch_ms": xf_ms / max(1, xf_launches),
                        "bytes_per_step": xf_bytes / max(1, args.steps)},
         "cpu_baseline": cpu,
+        "r_only_swap": r_swap,
         "parity_rel_err_vs_oracle": parity,
         "cold_compile_s": cold,
         "tile_configs": tuned,

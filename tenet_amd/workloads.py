@@ -9,10 +9,16 @@
   remainder), P = {0, 1}, M = {0:3, +1:1, -1:1} (Heisenberg S^z / S^+- channels).
   SU(2): V spins with multiplicities from the same Gaussian rule so that
   sum_j m_j (2j+1) = D, P = spin 1/2, M = {0, 1, 0} (dimension 5).
-* ``su2_fmove``: A, B in V (x) V <- V (x) V with spins 2j = 0..8 and
-  multiplicity floor(64 / (2j+1)); C = contract(A, [1, 3], B, [2, 0]) -- both
-  operand transforms cross the codomain/domain boundary, so inner fusion
-  lines recouple (Multi groups with 6j coefficients) before the core GEMM.
+* ``su2_fmove`` (spins 2j = 0..8, multiplicity floor(64 / (2j+1))):
+  - swap form: A in V (x) V (x) V <- V; A' = permute(A, (0, 2, 1), (3,))
+    braids codomain legs 1 and 2, which recouples the inner line j01 (Multi
+    groups with 6j coefficients); then C = contract(A', [2, 3], B, [2, 0])
+    with B in V (x) V <- V (x) V.
+  - cross form: A, B in V (x) V <- V (x) V, C = contract(A, [1, 3], B, [2, 0])
+    -- both operand transforms cross the codomain/domain boundary, so inner
+    lines recouple before the core GEMM.
+  - the plain 2 <- 2 codomain swap permute(A, (1, 0), (2, 3)) is an R-move
+    only (Single groups, coefficient +-1), timed on its own.
 
 Entries are seeded on the device: normal(0,1) (Box-Muller on splitmix64) for
 the environment, the reference's uniform [-1, 1) fill for the F-move case.
@@ -207,6 +213,32 @@ class SU2FMoveContract(GraphStep):
     def run(self):
         self.out = self.A.contract([1, 3], self.B, [2, 0], [0, 1, 2, 3], 2, out=self.out,
                                    precision=self.precision)
+        return self.out
+
+
+class SU2FMoveSwapContract(GraphStep):
+    """A' = permute(A, (0, 2, 1), (3,)) then C = contract(A', [2, 3], B, [2, 0])."""
+
+    SWAP = ((0, 2, 1), (3,))
+    AXES = ([2, 3], [2, 0], [0, 1, 2, 3], 2)
+
+    def __init__(self, rt, mult=64, twoj_max=8, precision="f64"):
+        self.rt = rt
+        self.rule, V = su2_fmove_space(mult, twoj_max)
+        self.V = V
+        self.precision = precision
+        self.A = TensorMap.rand(rt, [V, V, V], [V], seed=1)
+        self.B = TensorMap.rand(rt, [V, V], [V, V], seed=2)
+        self.Ap = None
+        self.out = None
+        self.run()
+        self.transform = rt.transform(self.A.space, *self.SWAP)
+        self.contraction = rt.contraction(self.Ap.space, self.B.space, *self.AXES)
+        self.flops = self.contraction.flops
+
+    def run(self):
+        self.Ap = self.A.permute(*self.SWAP, out=self.Ap)
+        self.out = self.Ap.contract(self.AXES[0], self.B, *self.AXES[1:], out=self.out, precision=self.precision)
         return self.out
 
 

@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 from tenet_amd.tensor import Runtime, TensorMap
-from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract
+from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract, SU2FMoveSwapContract
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +114,40 @@ def test_su2_permutation_roundtrip_full_size(rt, su2_fmove):
     del P
     err = dev_rel(Q, A)
     assert err < 1e-14, err
+
+
+@pytest.fixture(scope="module")
+def su2_swap(rt):
+    return SU2FMoveSwapContract(rt, mult=64, twoj_max=8, precision="f64")
+
+
+def test_su2_swap_form_full_size_route_invariance(rt, su2_swap):
+    """C = permute(A, (0,2,1), (3,))[2,3] . B[2,0] is A[1,3] . B[2,0]: the
+    explicit F-move and the fused operand transform agree at 342.6 M
+    elements per operand."""
+    wl = su2_swap
+    c1 = wl.run().copy()
+    c2 = wl.A.contract([1, 3], wl.B, [2, 0], [0, 1, 2, 3], 2)
+    assert c2.space == c1.space
+    err = dev_rel(c2, c1)
+    del c2
+    assert err < 1e-12, err
+
+
+def test_su2_swap_is_an_involution_full_size(rt, su2_swap):
+    """Braiding codomain legs 1 and 2 twice is the identity (symmetric
+    braiding), through two rounds of Multi groups."""
+    Q = su2_swap.Ap.permute([0, 2, 1], [3])
+    assert Q.space == su2_swap.A.space
+    err = dev_rel(Q, su2_swap.A)
+    del Q
+    assert err < 1e-14, err
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_i8"])
+def test_su2_swap_form_full_size_split_within_1e10(rt, su2_swap, prec):
+    ref = su2_swap.run().copy()
+    got = su2_swap.Ap.contract([2, 3], su2_swap.B, [2, 0], [0, 1, 2, 3], 2, precision=prec)
+    err = dev_rel(got, ref)
+    del got, ref
+    assert err < 1e-10, err

@@ -12,7 +12,7 @@ import pytest
 from descs import ospace
 from oracle.tensor import OTensor
 from tenet_amd.tensor import Runtime
-from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract
+from tenet_amd.workloads import MPSEnvUpdate, SU2FMoveContract, SU2FMoveSwapContract
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,41 @@ def test_fmove_contract_matches_oracle(rt):
     got = wl.run().to_numpy()
     want = _o(wl.A).contract([1, 3], _o(wl.B), [2, 0], [0, 1, 2, 3], 2).data
     assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "split_bf16", "split_i8"])
+def test_fmove_swap_contract_matches_oracle(rt, prec):
+    """The 3 <- 1 codomain swap (inner line recoupled: Multi groups) then the
+    contraction, against the oracle's permute and contract of the same
+    payloads."""
+    wl = SU2FMoveSwapContract(rt, mult=6, twoj_max=4, precision=prec)
+    assert wl.transform.n_multi > 0
+    got = wl.run().to_numpy()
+    ap = _o(wl.A).permute([0, 2, 1], [3])
+    assert np.abs(wl.Ap.to_numpy() - ap.data).max() <= 1e-14 * np.abs(ap.data).max()
+    want = ap.contract([2, 3], _o(wl.B), [2, 0], [0, 1, 2, 3], 2).data
+    tol = {"f64": 1e-13, "f32": 1e-5, "split_bf16": 1e-10, "split_i8": 1e-10}[prec]
+    assert rel(got, want) < tol, (prec, rel(got, want))
+
+
+def test_r_only_codomain_swap_matches_oracle(rt):
+    """A 2 <- 2 codomain swap is an R-move only: Single groups whose
+    coefficient is +-1.  Like the reference (fusion_tree.rs:3318-3372), the
+    plan bends every leg into the codomain, braids, and bends back, so the
+    coefficient is +-1 only to within rounding of the bend factors; the
+    device result matches the oracle to one rounding per element."""
+    from tenet_amd.tensor import TensorMap
+    from tenet_amd.workloads import su2_fmove_space
+    _, V = su2_fmove_space(6, 4)
+    X = TensorMap.rand(rt, [V, V], [V, V], seed=7)
+    tr = rt.transform(X.space, (1, 0), (2, 3))
+    assert tr.n_multi == 0 and tr.n_single > 0
+    d = tr.desc.contents
+    co = np.array([d.coeffs[i] for i in range(d.n_coeffs)])
+    assert np.abs(np.abs(co) - 1.0).max() <= 4 * np.finfo(float).eps
+    got = X.permute((1, 0), (2, 3)).to_numpy()
+    want = _o(X).permute([1, 0], [2, 3]).data
+    assert np.all(np.abs(got - want) <= 8 * np.finfo(float).eps * np.abs(want))
 
 
 @pytest.mark.parametrize("rule_name,D", [("U1", 128), ("SU2", 256)])

@@ -196,6 +196,24 @@ def test_u1_contract_equals_dense_einsum():
     np.testing.assert_allclose(C.to_dense(), np.transpose(ref, [2, 0, 1, 3]), atol=1e-12)
 
 
+def test_su2_inner_line_swap_invariants():
+    """The 3 <- 1 codomain swap (0, 2, 1) recouples the inner line (Multi
+    groups): it is an involution, keeps the dimension-weighted norm (the F/R
+    moves are unitary), and contracting the swapped tensor equals contracting
+    the original with the swapped axis -- the fused operand transform and the
+    explicit F-move take different group plans to the same result."""
+    su2 = make_rule("SU2")
+    V = leg(su2, [(0, 3), (1, 2), (2, 2), (3, 1)])
+    A = OTensor.rand(HomSpace(su2, [V, V, V], [V]), 1)
+    B = OTensor.rand(HomSpace(su2, [V, V], [V, V]), 2)
+    Ap = A.permute([0, 2, 1], [3])
+    assert np.abs(Ap.permute([0, 2, 1], [3]).data - A.data).max() < 1e-14
+    assert abs(Ap.norm() - A.norm()) < 1e-12 * A.norm()
+    C1 = Ap.contract([2, 3], B, [2, 0], [0, 1, 2, 3], 2)
+    C2 = A.contract([1, 3], B, [2, 0], [0, 1, 2, 3], 2)
+    assert np.linalg.norm(C1.data - C2.data) < 1e-13 * np.linalg.norm(C2.data)
+
+
 def test_layout_closed_form_z2():
     """tenet-core/src/tests.rs:189-260: strides [1,2,12,24], exact coverage."""
     z2 = make_rule("Z2")
