@@ -39,15 +39,46 @@ GEMM_KERNELS = ["gemm_kernel<{}, 128, 128>", "gemm_kernel<{}, 64, 64>", "gemm_ke
 SPLIT_PRODUCTS = 15  # bf16 MFMA products per FP64 product at five digit planes
 
 
-def roofline(precision, dom):
+TRAFFIC_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.txt")
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of a dominant kernel from the committed counter
+    table (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same bench command,
+    KiB per dispatch scaled by the copy calibration), or (None, None) when no
+    row matches the workload, precision and kernel."""
+    try:
+        with open(TRAFFIC_TABLE) as fh:
+            lines = fh.read().splitlines()
+    except OSError:
+        return None, None
+    scale = {}
+    for ln in lines:
+        if ln.startswith("# scale"):
+            _, _, name, val = ln.split()
+            scale[name] = float(val)
+        if not ln.strip() or ln.startswith("#"):
+            continue
+        f = [x.strip() for x in ln.split("|")]
+        if len(f) == 6 and f[0] == workload and f[1] == precision and f[2] == kernel:
+            fetch = float(f[3]) * 1024 * scale.get("FETCH_SIZE", 1.0)
+            write = float(f[4]) * 1024 * scale.get("WRITE_SIZE", 1.0)
+            return fetch + write, f[5]
+    return None, None
+
+
+def roofline(precision, dom, workload=None):
     """Roofline record of the dominant GEMM kernel: algorithmic flops per launch
     over its average launch time (HIP events on the backend stream)."""
     avg_s = dom["ms"] * 1e-3 / max(1, dom["launches"])
     per_launch = dom["flops"] / max(1, dom["launches"])
     f64_eq = per_launch / avg_s / 1e12 if avg_s else 0.0
-    rec = {"bound": "mfma", "unit": "TFLOP/s", "traffic": None, "kernel": dom["kernel"],
+    traffic, source = measured_traffic(workload, precision, dom["kernel"])
+    rec = {"bound": "mfma", "unit": "TFLOP/s", "traffic": traffic, "kernel": dom["kernel"],
            "launches": dom["launches"], "avg_launch_ms": avg_s * 1e3,
            "flops_per_launch": per_launch}
+    if traffic is not None:
+        rec["traffic_source"] = source
     if dom.get("bytes"):
         # memory-bound batch of tiny GEMMs: operands read once, results written once
         gbs = dom["bytes"] / max(1, dom["launches"]) / avg_s / 1e9 if avg_s else 0.0
@@ -69,7 +100,59 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap"):
+def cpu_port_baseline(workload, wl, budget_s=30.0):
+    """The C++/OpenMP CPU port of the host route (oracle/cpu_port.cpp) on the
+    device workload's own compiled descriptors and downloaded payloads, at the
+    full workload size: warm (descriptors compiled and workspaces allocated
+    before the clock starts), the median of the runs that fit in budget_s (at
+    least one, at most five).  Returns (seconds, flops, threads, sample, runs,
+    result payload(s))."""
+    import numpy as np
+
+    from oracle import cpu_port as P
+    if workload in ("u1_env", "su2_env"):
+        fs = [P.Contract(c.desc, c.dst_space.len) for c in wl.contractions]
+        L, A, W = wl.L.to_numpy(), wl.A.to_numpy(), wl.W.to_numpy()
+
+        def step():
+            return fs[2](A, fs[1](fs[0](L, A), W))
+        sample = "full environment update (L.A, .W, A^dagger.) at the bench size"
+    elif workload == "batched":
+        ops = [(P.Contract(c.desc, c.dst_space.len), a.to_numpy(), b.to_numpy()) for a, b, _, c in wl.pairs]
+
+        def step():
+            return [f(a, b) for f, a, b in ops]
+        sample = f"all {len(ops)} merges in FP64"
+    elif hasattr(wl, "transform"):
+        xf = wl.transform
+        A, B = wl.A.to_numpy(), wl.B.to_numpy()
+        ap = np.empty(xf.dst_space.len)
+        f = P.Contract(wl.contraction.desc, wl.contraction.dst_space.len)
+
+        def step():
+            P.transform(xf.desc, A, dst=ap)
+            return f(ap, B)
+        sample = "full F-move (3<-1 codomain swap) + contraction"
+    else:
+        A, B = wl.A.to_numpy(), wl.B.to_numpy()
+        f = P.Contract(wl.contraction.desc, wl.contraction.dst_space.len)
+
+        def step():
+            return f(A, B)
+        sample = "full contraction A[1,3].B[2,0]"
+    times, out = [], None
+    t0 = time.perf_counter()
+    while len(times) < 5:
+        t = time.perf_counter()
+        out = step()
+        times.append(time.perf_counter() - t)
+        if time.perf_counter() - t0 + times[-1] > budget_s:
+            break
+    times.sort()
+    return times[len(times) // 2], wl.flops, P.threads(), sample, len(times), out
+
+
+def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap", min_runs=3):
     """Time the oracle (numpy FP64, host cores) on the same workload.  With the
     device workload `wl` given, the oracle's operands are its downloaded
     payloads, so the parity check compares results of identical inputs."""
@@ -138,9 +221,9 @@ def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap"):
     times = []
     out = None
     t_start = time.perf_counter()
-    # at least 3 runs and about 10 s of CPU work, at most budget_s
-    while len(times) < 3 or (time.perf_counter() - t_start) < min(10.0, budget_s):
-        if len(times) >= 3 and (time.perf_counter() - t_start) >= budget_s:
+    # at least min_runs runs and about 10 s of CPU work, at most budget_s
+    while len(times) < min_runs or (time.perf_counter() - t_start) < min(10.0, budget_s):
+        if len(times) >= min_runs and (time.perf_counter() - t_start) >= budget_s:
             break
         t = time.perf_counter()
         out = step()
@@ -703,11 +786,29 @@ def main():
 
     cpu = None
     parity = None
+    parity_port = None
     if args.cpu_baseline == "on":
         import numpy as np
+
+        from oracle import cpu_port as P
+        # the C++/OpenMP port at full size: the baseline value, and a full-size
+        # parity check of the device result (same descriptors, same payloads)
+        pt, pflops, pthreads, psample, pruns, pout = cpu_port_baseline(args.workload, wl)
+        if args.workload == "batched":
+            got = wl.results()
+            parity_port = (sum(float(np.linalg.norm(g - o) ** 2) for g, o in zip(got, pout))
+                           / sum(float(np.linalg.norm(o) ** 2) for o in pout)) ** 0.5
+        else:
+            got = wl.run().to_numpy()
+            parity_port = float(np.linalg.norm(got - pout) / np.linalg.norm(pout))
+        del pout, got
+        log(f"[bench] cpu port: {pflops / pt / 1e9:.1f} GFLOP/s on {pthreads} threads ({P.cpu_model()}), "
+            f"median of {pruns} run(s) of {pt:.2f}s; GPU vs port rel err {parity_port:.2e}")
+        # the numpy oracle (an independent restatement of structure and replay):
+        # the parity checker; timed once beside the port
         t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D,
                                                        wl=wl if args.workload in ("u1_env", "su2_env") else None,
-                                                       fmove=args.fmove)
+                                                       fmove=args.fmove, budget_s=0.0, min_runs=1)
         if args.workload == "batched":
             nsample = len(out)
             flops_sample = sum(c.flops for *_, c in wl.pairs[:nsample])
@@ -729,9 +830,13 @@ def main():
             got = wl.run().to_numpy()
             parity = float(np.linalg.norm(got - out.data) / np.linalg.norm(out.data))
             cpu_val = wl.flops / t_cpu / 1e12
-        cpu = {"value": cpu_val, "unit": "TFLOP/s", "cores": int(cores), "kind": "port",
-               "sample": f"{sample}; median of {reps} run(s), {t_cpu:.2f}s each (numpy FP64 oracle)"}
-        log(f"[bench] cpu baseline {cpu_val * 1e3:.2f} GFLOP/s on {cores} threads; GPU vs oracle rel err {parity:.2e}")
+        cpu = {"value": pflops / pt / 1e12, "unit": "TFLOP/s", "cores": int(pthreads), "kind": "port",
+               "sample": f"{psample}; C++/OpenMP port of the host route (oracle/cpu_port.cpp), FP64, "
+                         f"median of {pruns} warm run(s) of {pt:.2f}s",
+               "cpu_model": P.cpu_model(), "gemm": "cache-blocked DGEMM of the port (AVX2+FMA), no system BLAS",
+               "numpy_oracle": {"value": cpu_val, "unit": "TFLOP/s", "blas_threads": int(cores),
+                                "sample": f"{sample}; one run of {t_cpu:.2f}s"}}
+        log(f"[bench] numpy oracle {cpu_val * 1e3:.2f} GFLOP/s on {cores} BLAS threads; GPU vs oracle rel err {parity:.2e}")
 
     rec = {
         "metric": "contract TFLOP/s (FP64 block-sparse contraction incl. recoupling)",
@@ -748,7 +853,8 @@ def main():
                   "split_i8": "split-int8 (f64 in/out)"}[args.precision],
         "data": "synthetic (seeded normal(0,1) on device)",
         "config": cfg,
-        "roofline": roofline(args.precision, dom),
+        "roofline": roofline(args.precision, dom, cfg["workload"] + (
+            f" D={args.D}" if "bond_dim" in cfg else f" mult={args.mult}" if "multiplicity" in cfg else "")),
         "gemm_all_launches": {"tflops": gemm_tflops, "launches": gemm_launches,
                               "ms_per_step": gemm_ms / args.steps},
         "ms_per_step_with_kernel_events": ms_events,
@@ -760,6 +866,7 @@ def main():
         "cpu_baseline": cpu,
         "r_only_swap": r_swap,
         "parity_rel_err_vs_oracle": parity,
+        "parity_rel_err_vs_cpu_port_full_size": parity_port,
         "cold_compile_s": cold,
         "tile_configs": tuned,
         "rel_err_vs_f64": batched_err if args.workload == "batched" else None,

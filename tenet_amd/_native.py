@@ -212,9 +212,11 @@ def _load(name, api):
 
 
 def host():
+    """The host structure library; TENET_AMD_HOST_LIB names another build of
+    it in lib/ (the sanitizer build libtenet_host_san.so)."""
     global _host
     if _host is None:
-        _host = _load("libtenet_host.so", HOST_API)
+        _host = _load(os.environ.get("TENET_AMD_HOST_LIB", "libtenet_host.so"), HOST_API)
     return _host
 
 

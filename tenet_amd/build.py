@@ -1,7 +1,11 @@
 """Build the native libraries of tenet_amd in-tree.
 
-  lib/libtenet_host.so  host structure compiler (g++, no GPU code)
-  lib/libtenet_hip.so   gfx950 backend (hipcc, --offload-arch=gfx950)
+  lib/libtenet_host.so      host structure compiler (g++, no GPU code)
+  lib/libtenet_hip.so       gfx950 backend (hipcc, --offload-arch=gfx950)
+  lib/libtenet_host_san.so  the host library again with AddressSanitizer and
+                            UndefinedBehaviorSanitizer (selected with
+                            TENET_AMD_HOST_LIB=libtenet_host_san.so and the
+                            ASan runtime preloaded; CPU checking only)
 
 Usage: python -m tenet_amd.build   (or tenet_amd.build.build())
 """
@@ -41,6 +45,14 @@ def build_host():
     return out
 
 
+def build_host_sanitized():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libtenet_host_san.so")
+    _run(["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared", "-Wall", "-fno-omit-frame-pointer",
+          "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-o", out] + HOST_SRC)
+    return out
+
+
 def build_hip(jobs=8):
     os.makedirs(LIB, exist_ok=True)
     objdir = os.path.join(HERE, "_build")
@@ -75,6 +87,7 @@ def build_hip(jobs=8):
 
 def build():
     build_host()
+    build_host_sanitized()
     build_hip()
 
 
