@@ -300,9 +300,7 @@ class HeffServer:
             for D in sorted(waiting, key=lambda d: waiting[d][0][1][0]):  # oldest bucket first
                 items = waiting[D]
                 while items and slots and not blocked:
-                    # batch only while every stream is busy: an idle stream
-                    # serves a lone request sooner than a batch would
-                    limit = min(len(items), len(slots)) if len(inflight) >= len(self.rts) else 1
+                    limit = min(len(items), len(slots))
                     # the largest batch whose buffers fit the HBM budget; one
                     # request is admitted over budget only when nothing runs
                     fits = [b for b in self._sizes(D) if b <= limit and used + b * self.fp[D] <= self.budget]
