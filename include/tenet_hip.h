@@ -248,8 +248,15 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* art, int cfg);
  * payload to the destination whose coefficients are gathered from the rhs
  * payload, instead of transform + GEMM + transform (HBM-bound either way; the
  * fused pass moves each element once).  TENET_AMD_FUSE_NARROW=0 in the
- * environment at upload keeps the general route. */
-enum tnt_route { TNT_ROUTE_TRANSFORM_GEMM = 0, TNT_ROUTE_FUSED_NARROW = 1 };
+ * environment at upload keeps the general route.
+ * A tiny contraction -- at most 16 Mi flops, every payload at most 2^20
+ * elements, Single-only transforms, no core element feeding two destination
+ * elements -- runs its FP64 calls as ONE launch: the operand transforms are
+ * folded into the GEMM's operand loads (element maps built at upload) and the
+ * output transform into its stores (TNT_ROUTE_TINY_GATHERED); other
+ * precisions keep the general route.  TENET_AMD_TINY_FLOPS=0 at upload keeps
+ * the general route. */
+enum tnt_route { TNT_ROUTE_TRANSFORM_GEMM = 0, TNT_ROUTE_FUSED_NARROW = 1, TNT_ROUTE_TINY_GATHERED = 2 };
 int tnt_contract_route(const tnt_artifact* art, int* route);
 int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_buf* lhs,
                  const tnt_buf* rhs, double alpha, double beta, int precision);

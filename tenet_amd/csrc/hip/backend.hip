@@ -214,6 +214,18 @@ struct tnt_artifact {
   int64_t* g_idx = nullptr;
   double* g_scale = nullptr;
   uint32_t n_gather = 0;
+  // gathered small-GEMM route of tiny contractions (FP64 calls): element maps
+  // of the Single-only transforms and one tile list (32 x 32) over every job
+  bool tiny = false;
+  TinyEntry* t_lmap = nullptr;
+  TinyEntry* t_rmap = nullptr;
+  TinyEntry* t_omap = nullptr;
+  int64_t* t_fix = nullptr;
+  uint32_t t_n_fix = 0;
+  GJob* t_jobs = nullptr;
+  GTile* t_tiles = nullptr;
+  uint32_t t_n_tiles = 0;
+  double t_flops = 0;
 };
 
 // The fused narrow route is on unless TENET_AMD_FUSE_NARROW=0 (for A/B runs
@@ -221,6 +233,15 @@ struct tnt_artifact {
 static bool fusion_enabled() {
   const char* e = std::getenv("TENET_AMD_FUSE_NARROW");
   return !(e && e[0] == '0');
+}
+
+// Contractions up to this many flops and payload elements whose transforms
+// are all Single blocks run as one gathered small-GEMM launch (FP64 calls);
+// TENET_AMD_TINY_FLOPS overrides at upload (0: off).  Per call they pay one
+// launch instead of one per transform class and tile shape.
+static double tiny_max_flops() {
+  const char* e = std::getenv("TENET_AMD_TINY_FLOPS");
+  return e ? std::atof(e) : kTinyMaxFlops;
 }
 
 struct tnt_graph {
@@ -494,6 +515,15 @@ static bool layout_in_bounds(const tnt_layout& l, int64_t len, std::string& err)
   return true;
 }
 
+// Work items per item class that a plan aims for (TENET_AMD_MIN_ITEMS
+// overrides at upload; 0 keeps the fixed item sizes).  A small plan cut into
+// the fixed sizes leaves most CUs idle: a 190K-element Multi pass is 46 items,
+// each swept serially by one workgroup.
+static int64_t min_plan_items() {
+  const char* e = std::getenv("TENET_AMD_MIN_ITEMS");
+  return e ? std::max<int64_t>(0, std::atoll(e)) : kMinPlanItems;
+}
+
 static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out) {
   if (!d) FAIL(ctx, TNT_E_INVALID, "null transform descriptor");
   std::vector<XBlock> blocks;
@@ -502,6 +532,28 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   std::vector<XFlat> flat;
   std::string err;
   double bytes = 0;
+  // item sizes: Single kSingleItem elements, Multi ~kMultiItemWork element
+  // updates (elements x layouts / 2), both shrunk for small plans so that each
+  // class has about min_plan_items() items (never below 256 elements)
+  int64_t single_target = kSingleItem, multi_work_target = kMultiItemWork;
+  {
+    int64_t single_elems = 0, multi_work = 0;
+    for (size_t bi = 0; bi < d->n_blocks; ++bi) {
+      const tnt_transform_block& tb = d->blocks[bi];
+      if (tb.dst_layout >= d->n_layouts) continue;  // rejected below
+      const tnt_layout& L = d->layouts[tb.dst_layout];
+      if (L.rank < 0 || L.rank > TNT_MAX_RANK) continue;
+      int64_t n = 1;
+      for (int i = 0; i < L.rank; ++i) n *= std::max<int64_t>(0, L.shape[i]);
+      if (tb.kind == TNT_BLOCK_SINGLE) single_elems += n;
+      else multi_work += n * std::max<int64_t>(1, ((int64_t)tb.dst_count + tb.src_count) / 2);
+    }
+    const int64_t want = min_plan_items();
+    if (want > 0 && single_elems < want * kSingleItem)
+      single_target = std::max<int64_t>(256, ((single_elems / want + 63) / 64) * 64);
+    if (want > 0 && multi_work < want * kMultiItemWork)
+      multi_work_target = std::max<int64_t>(256, multi_work / want);
+  }
   auto add_block = [&](int kind, const std::vector<uint32_t>& dl, const std::vector<uint32_t>& sl,
                        uint32_t coeff) -> int {
     std::vector<NLayout> ls;
@@ -558,9 +610,10 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     for (size_t i = 2; i < dims.size(); ++i) outer *= dims[i];
     int64_t elems = plane * outer;
     bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
-    // Single blocks: kSingleItem-element items (kSingleUnroll per thread) swept in
-    // order by a persistent grid; Multi: ~8K element-updates per item, weighted by
-    // its layouts.  Single items of one outer index go to the flattened class.
+    // Single blocks: items of up to kSingleItem elements (kSingleUnroll per
+    // thread) swept in order by a persistent grid; Multi: ~kMultiItemWork
+    // element-updates per item, weighted by its layouts (both smaller for small
+    // plans).  Single items of one outer index go to the flattened class.
     struct ItemList {
       std::vector<XItem>& items;
       std::vector<XItem>& flat_items;
@@ -606,7 +659,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
       return TNT_OK;
     }
     int64_t weight = std::max<int64_t>(1, (int64_t)(dl.size() + sl.size()) / 2);
-    int64_t target = kind == XK_SINGLE ? kSingleItem : std::max<int64_t>(256, 8192 / weight);
+    int64_t target = kind == XK_SINGLE ? single_target : std::max<int64_t>(256, multi_work_target / weight);
     if (plane >= target) {
       int64_t chunk = ((target + 63) / 64) * 64;
       for (int64_t o = 0; o < outer; ++o)
@@ -1237,6 +1290,122 @@ int tnt_tree_transform(tnt_ctx* ctx, const tnt_plan* plan, tnt_buf* dst, const t
   return launch_status(ctx);
 }
 
+// Element maps of the tiny route.  Layouts were bounds-checked by build_plan.
+// Visits every element of a layout pair in column-major order of the shape.
+extern "C++" template <class F>
+static void for_each_element(const tnt_layout& ld, const tnt_layout& ls, F&& f) {
+  int64_t n = 1;
+  for (int i = 0; i < ld.rank; ++i) n *= ld.shape[i];
+  if (n <= 0) return;
+  int64_t c[TNT_MAX_RANK] = {0};
+  for (int64_t e = 0; e < n; ++e) {
+    int64_t od = ld.offset, os = ls.offset;
+    for (int i = 0; i < ld.rank; ++i) {
+      od += c[i] * ld.strides[i];
+      os += c[i] * ls.strides[i];
+    }
+    f(od, os);
+    for (int i = 0; i < ld.rank; ++i) {
+      if (++c[i] < ld.shape[i]) break;
+      c[i] = 0;
+    }
+  }
+}
+
+static bool single_only(const tnt_transform_desc* d) {
+  for (size_t i = 0; i < d->n_blocks; ++i)
+    if (d->blocks[i].kind != TNT_BLOCK_SINGLE) return false;
+  return true;
+}
+
+// Gather map of an operand transform replayed with overwrite: transformed
+// element e = c * src[idx] (c = 0, idx = 0 where no block writes e).
+static bool tiny_gather_map(const tnt_transform_desc* d, std::vector<TinyEntry>& m) {
+  if (d->src_len < 1) return false;
+  m.assign((size_t)d->dst_len, TinyEntry{0, 0.0});
+  for (size_t i = 0; i < d->n_blocks; ++i) {
+    const tnt_transform_block& b = d->blocks[i];
+    const double c = d->coeffs[b.coeff];
+    for_each_element(d->layouts[b.dst_layout], d->layouts[b.src_layout],
+                     [&](int64_t od, int64_t os) { m[(size_t)od] = TinyEntry{os, c}; });
+  }
+  return true;
+}
+
+// Scatter map of the output transform: core element e goes to dst[idx] scaled
+// by c (idx = -1: read by no block).  Fails when a core element feeds two
+// destination elements.  `fix` receives the destination elements of inactive
+// layouts (beta * dst).
+static bool tiny_scatter_map(const tnt_transform_desc* d, std::vector<TinyEntry>& m, std::vector<int64_t>& fix) {
+  m.assign((size_t)d->src_len, TinyEntry{-1, 0.0});
+  bool ok = true;
+  for (size_t i = 0; i < d->n_blocks && ok; ++i) {
+    const tnt_transform_block& b = d->blocks[i];
+    const double c = d->coeffs[b.coeff];
+    for_each_element(d->layouts[b.dst_layout], d->layouts[b.src_layout], [&](int64_t od, int64_t os) {
+      if (m[(size_t)os].idx >= 0) ok = false;
+      m[(size_t)os] = TinyEntry{od, c};
+    });
+  }
+  for (size_t i = 0; i < d->n_inactive && ok; ++i) {
+    const tnt_layout& l = d->layouts[d->inactive[i]];
+    for_each_element(l, l, [&](int64_t od, int64_t) { fix.push_back(od); });
+  }
+  return ok;
+}
+
+// Decides and builds the tiny route of an uploaded artifact (after the general
+// route, which stays for the other precisions).  Returns non-zero only on an
+// allocation or copy failure.
+static int build_tiny_route(tnt_ctx* ctx, const tnt_contract_desc* d, const std::vector<GJob>& gj,
+                            tnt_artifact* a) {
+  const double max_flops = tiny_max_flops();
+  if (max_flops <= 0) return TNT_OK;
+  double flops = 0;
+  for (const auto& j : gj) flops += 2.0 * j.m * (double)j.k * j.n;
+  if (flops > max_flops) return TNT_OK;
+  const int64_t dst_len = d->out ? d->out->dst_len : d->core_len;
+  for (int64_t len : {d->lhs_len, d->rhs_len, d->core_len, dst_len,
+                      d->lhs ? d->lhs->src_len : int64_t(0), d->rhs ? d->rhs->src_len : int64_t(0)})
+    if (len > kTinyMaxElems) return TNT_OK;
+  if ((d->lhs && !single_only(d->lhs)) || (d->rhs && !single_only(d->rhs)) || (d->out && !single_only(d->out)))
+    return TNT_OK;
+  std::vector<TinyEntry> lm, rm, om;
+  std::vector<int64_t> fix;
+  if (d->lhs && !tiny_gather_map(d->lhs, lm)) return TNT_OK;
+  if (d->rhs && !tiny_gather_map(d->rhs, rm)) return TNT_OK;
+  if (d->out) {
+    if (!tiny_scatter_map(d->out, om, fix)) return TNT_OK;
+    // core elements of uncovered regions are zero: their destinations get beta * dst
+    for (size_t i = 0; i < d->n_uncovered; ++i)
+      for (int64_t e = d->uncovered[i].offset; e < d->uncovered[i].offset + d->uncovered[i].len; ++e)
+        if (om[(size_t)e].idx >= 0) fix.push_back(om[(size_t)e].idx);
+  } else {
+    for (size_t i = 0; i < d->n_uncovered; ++i)
+      for (int64_t e = d->uncovered[i].offset; e < d->uncovered[i].offset + d->uncovered[i].len; ++e)
+        fix.push_back(e);
+  }
+  std::vector<GTile> tiles;
+  for (size_t ji = 0; ji < gj.size(); ++ji) {
+    const GJob& j = gj[ji];
+    for (int tm = 0; tm < (j.m + 31) / 32; ++tm)
+      for (int tn = 0; tn < (j.n + 31) / 32; ++tn) tiles.push_back(GTile{(int32_t)ji, tm, tn, TILE_SMALL});
+  }
+  int rc = TNT_OK;
+  if (!rc && d->lhs) rc = upload_vec(ctx, lm, &a->t_lmap, a->allocs);
+  if (!rc && d->rhs) rc = upload_vec(ctx, rm, &a->t_rmap, a->allocs);
+  if (!rc && d->out) rc = upload_vec(ctx, om, &a->t_omap, a->allocs);
+  if (!rc) rc = upload_vec(ctx, fix, &a->t_fix, a->allocs);
+  if (!rc) rc = upload_vec(ctx, gj, &a->t_jobs, a->allocs);
+  if (!rc) rc = upload_vec(ctx, tiles, &a->t_tiles, a->allocs);
+  if (rc) return rc;
+  a->t_n_fix = (uint32_t)fix.size();
+  a->t_n_tiles = (uint32_t)tiles.size();
+  a->t_flops = flops;
+  a->tiny = true;
+  return TNT_OK;
+}
+
 int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact** out) {
   if (!ctx || !d || !out) return TNT_E_INVALID;
   if (!valid_op(d->lhs_op) || !valid_op(d->rhs_op)) FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
@@ -1303,6 +1472,7 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
       }
     }
   }
+  if (int rc = build_tiny_route(ctx, d, gj, a)) return fail(rc);
   void* p = nullptr;
   if (a->lhs) {
     if (int rc = pool_alloc(ctx, 8 * (size_t)std::max<int64_t>(a->lhs_len, 1), &p)) return fail(rc);
@@ -1354,7 +1524,7 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
 
 int tnt_contract_route(const tnt_artifact* a, int* route) {
   if (!a || !route) return TNT_E_INVALID;
-  *route = a->fused ? TNT_ROUTE_FUSED_NARROW : TNT_ROUTE_TRANSFORM_GEMM;
+  *route = a->fused ? TNT_ROUTE_FUSED_NARROW : a->tiny ? TNT_ROUTE_TINY_GATHERED : TNT_ROUTE_TRANSFORM_GEMM;
   return TNT_OK;
 }
 
@@ -1390,6 +1560,14 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   if (a->fused) {
     launch_coeff_gather(a->fused->d.coeffs, R, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
     run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
+    return launch_status(ctx);
+  }
+  if (a->tiny && precision == TNT_PREC_F64) {
+    TimedScope ts(ctx, 1 + precision, ctx->timing ? a->t_flops : 0.0);
+    TimedScope tk(ctx, timing_kind_kernel(precision, TILE_SMALL), ctx->timing ? a->t_flops : 0.0);
+    GemmArgs g{a->t_jobs, a->t_tiles, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta};
+    TinyArgs t{a->t_lmap, a->t_rmap, a->t_omap, a->t_fix, a->t_n_fix, 0u, static_cast<double*>(dst->ptr)};
+    launch_gemm_tiny(g, t, a->t_n_tiles, ctx->stream);
     return launch_status(ctx);
   }
   if (a->lhs) {

@@ -26,6 +26,7 @@
 //            FP32 (v_mfma_f32_16x16x4f32):  i = 4 * (lane / 16) + r
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace tnt_dev {
@@ -390,6 +391,197 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, uint32_t n_
         }
       }
     }
+}
+
+// The gathered small GEMM of a tiny contraction: gemm_small_kernel's tile
+// walk and MFMA sequence (so a job's sums are formed in the same order), with
+// an operand element e of a transformed operand read as map[e].c *
+// source[map[e].idx] -- the value the Single-block replay would have written
+// to the transformed operand -- and a result element stored through the output
+// map, alpha * c * acc (+ beta * dst), as the output replay would.  Workgroups
+// past the tile workgroups set the fix-up elements to beta * dst.  Indices are
+// clamped into each job's operands as in gemm_small_kernel, and every map
+// index was checked against its payload at upload.
+template <bool LM, bool RM, bool OM, bool BETA>
+__global__ __launch_bounds__(256) void gemm_tiny_kernel(GemmArgs g, TinyArgs t, uint32_t n_tiles) {
+  if (blockIdx.x >= t.tile_blocks) {
+    const uint32_t nb = gridDim.x - t.tile_blocks;
+    for (uint32_t i = (blockIdx.x - t.tile_blocks) * 256u + threadIdx.x; i < t.n_fix; i += nb * 256u) {
+      double* p = t.dst + t.fix[i];
+      *p = BETA ? g.beta * *p : 0.0;
+    }
+    return;
+  }
+  constexpr int MT = 2, NT = 2;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, q4 = lane >> 4;
+  const uint32_t ti = blockIdx.x * 4u + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ti >= n_tiles) return;
+  const GTile tile = g.tiles[ti];
+  const GJob job = g.jobs[tile.job];
+  const int m = job.m, n = job.n, k = job.k;
+  const int m0 = tile.tm * 32, n0 = tile.tn * 32;
+  const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
+  const int64_t sAi = a_adj ? job.lda : 1, sAk = a_adj ? 1 : job.lda;
+  const int64_t sBj = b_adj ? 1 : job.ldb, sBk = b_adj ? job.ldb : 1;
+  const gptr<const double> A = gmem<const double>(g.abase);
+  const gptr<const double> B = gmem<const double>(g.bbase);
+  int64_t ea[MT], eb[NT];  // element offsets of the fragment rows / columns
+#pragma unroll
+  for (int i = 0; i < MT; ++i) ea[i] = job.a_off + (int64_t)min(m0 + 16 * i + l16, m - 1) * sAi;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) eb[j] = job.b_off + (int64_t)min(n0 + 16 * j + l16, n - 1) * sBj;
+  auto lval = [&](int64_t e) -> double {
+    if constexpr (LM) {
+      const TinyEntry te = t.lmap[e];
+      return te.c * A[te.idx];
+    } else {
+      return A[e];
+    }
+  };
+  auto rval = [&](int64_t e) -> double {
+    if constexpr (RM) {
+      const TinyEntry te = t.rmap[e];
+      return te.c * B[te.idx];
+    } else {
+      return B[e];
+    }
+  };
+  f64x4 acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f64x4{0, 0, 0, 0};
+  double ra[2][2][MT], rb[2][2][NT];  // [stage][p][fragment]
+  const int nb = (k + 7) / 8;
+  auto load = [&](auto sc, int blk) {
+    constexpr int s = decltype(sc)::value;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int kk = min(blk * 8 + q4 + 4 * p, k - 1);
+#pragma unroll
+      for (int i = 0; i < MT; ++i) ra[s][p][i] = lval(ea[i] + kk * sAk);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) rb[s][p][j] = rval(eb[j] + kk * sBk);
+    }
+  };
+  auto compute = [&](auto sc, int blk) {
+    constexpr int s = decltype(sc)::value;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const bool ok = blk * 8 + q4 + 4 * p < k;
+      double af[MT], bf[NT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) af[i] = ok ? ra[s][p][i] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bf[j] = ok ? rb[s][p][j] : 0.0;
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<double>::run(af[i], bf[j], acc[i][j]);
+    }
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  if (k > 0 && k <= 8 * kTinyKBlocks) {
+    // short K: every operand fragment of the job in flight at once (one
+    // memory round trip instead of one per 16 k); loads past K re-read the
+    // last k (clamped) and their products are skipped, so the sums are the
+    // pipelined loop's
+    double fa[kTinyKBlocks][2][MT], fb[kTinyKBlocks][2][NT];
+#pragma unroll
+    for (int blk = 0; blk < kTinyKBlocks; ++blk)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int kk = min(blk * 8 + q4 + 4 * p, k - 1);
+#pragma unroll
+        for (int i = 0; i < MT; ++i) fa[blk][p][i] = lval(ea[i] + kk * sAk);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) fb[blk][p][j] = rval(eb[j] + kk * sBk);
+      }
+#pragma unroll
+    for (int blk = 0; blk < kTinyKBlocks; ++blk) {
+      if (blk >= nb) break;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const bool ok = blk * 8 + q4 + 4 * p < k;
+        double af[MT], bf[NT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) af[i] = ok ? fa[blk][p][i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) bf[j] = ok ? fb[blk][p][j] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<double>::run(af[i], bf[j], acc[i][j]);
+      }
+    }
+  } else {
+    if (k > 0) load(S0{}, 0);
+    for (int tt = 0; tt < nb; tt += 2) {
+      load(S1{}, tt + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(S0{}, tt);
+      __builtin_amdgcn_sched_barrier(0);
+      load(S0{}, tt + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(S1{}, tt + 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  const double alpha = g.alpha, beta = g.beta;
+  const gptr<double> D = gmem<double>(g.dbase);
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int col = n0 + 16 * j + l16;
+      if (col >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + 16 * i + MfmaOp<double>::row(lane, r);
+        if (row >= m) continue;
+        const int64_t e = job.d_off + row + (int64_t)col * job.ldd;
+        if constexpr (OM) {
+          const TinyEntry te = t.omap[e];
+          if (te.idx >= 0) {
+            double* p = t.dst + te.idx;
+            const double c = alpha * te.c;
+            const double v = acc[i][j][r];
+            *p = BETA ? c * v + beta * *p : c * v;
+          }
+        } else {
+          const double v = alpha * acc[i][j][r];
+          D[e] = BETA ? v + beta * D[e] : v;
+        }
+      }
+    }
+}
+
+template <bool LM, bool RM, bool OM>
+static void launch_tiny3(const GemmArgs& g, const TinyArgs& t, uint32_t grid, uint32_t n_tiles, hipStream_t s) {
+  if (g.beta != 0.0)
+    hipLaunchKernelGGL((gemm_tiny_kernel<LM, RM, OM, true>), dim3(grid), dim3(256), 0, s, g, t, n_tiles);
+  else
+    hipLaunchKernelGGL((gemm_tiny_kernel<LM, RM, OM, false>), dim3(grid), dim3(256), 0, s, g, t, n_tiles);
+}
+
+void launch_gemm_tiny(const GemmArgs& g, const TinyArgs& t0, uint32_t n_tiles, hipStream_t s) {
+  TinyArgs t = t0;
+  t.tile_blocks = (n_tiles + 3) / 4;
+  const uint32_t fix_blocks = t.n_fix ? std::min<uint32_t>(64u, (t.n_fix + 255) / 256) : 0u;
+  const uint32_t grid = t.tile_blocks + fix_blocks;
+  if (grid == 0) return;
+  const int key = (t.lmap ? 1 : 0) | (t.rmap ? 2 : 0) | (t.omap ? 4 : 0);
+  switch (key) {
+    case 0: launch_tiny3<false, false, false>(g, t, grid, n_tiles, s); break;
+    case 1: launch_tiny3<true, false, false>(g, t, grid, n_tiles, s); break;
+    case 2: launch_tiny3<false, true, false>(g, t, grid, n_tiles, s); break;
+    case 3: launch_tiny3<true, true, false>(g, t, grid, n_tiles, s); break;
+    case 4: launch_tiny3<false, false, true>(g, t, grid, n_tiles, s); break;
+    case 5: launch_tiny3<true, false, true>(g, t, grid, n_tiles, s); break;
+    case 6: launch_tiny3<false, true, true>(g, t, grid, n_tiles, s); break;
+    default: launch_tiny3<true, true, true>(g, t, grid, n_tiles, s); break;
+  }
 }
 
 template <typename T, bool BETA, typename S>

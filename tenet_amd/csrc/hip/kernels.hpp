@@ -23,6 +23,8 @@ __device__ __forceinline__ gptr<T> gmem(uintptr_t addr) {
 constexpr int kMaxMultiSrc = 32;  // source blocks per Multi group handled in registers
 constexpr int kSingleUnroll = 8;  // elements per thread of a Single work item
 constexpr int kSingleItem = 256 * kSingleUnroll;  // plane elements per Single work item
+constexpr int64_t kMultiItemWork = 8192;  // element updates per Multi work item
+constexpr int64_t kMinPlanItems = 1024;   // items per class a small plan is cut into
 
 // XK_TRANSPOSE: a Single block whose source is contiguous along another axis
 // than the destination (dims[0] destination-contiguous, dims[1]
@@ -110,7 +112,11 @@ void launch_xform_single(const XformArgs& a, uint32_t begin, uint32_t n, uint32_
 // All items of a plan in one launch (any class).
 void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s);
 // Plans that move fewer bytes than this run as one launch of every class.
-constexpr double kSmallPlanBytes = 1.0 * (1 << 20);
+constexpr double kSmallPlanBytes = 16.0 * (1 << 20);
+// Tiny-contraction route limits (flops of all core jobs; elements of every payload).
+constexpr double kTinyMaxFlops = 16.0 * (1 << 20);
+constexpr int64_t kTinyMaxElems = int64_t(1) << 20;
+constexpr int kTinyKBlocks = 8;  // jobs with K <= 8 * this load all of K at once
 // coeffs[t] = idx[t] >= 0 ? scale[t] * src[idx[t]] : 0 (coefficients of a fused
 // narrow contraction gathered from its small operand).
 void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,
@@ -138,6 +144,30 @@ struct GemmArgs {
   uintptr_t dbase, abase, bbase;  // byte addresses; job offsets are in elements
   double alpha, beta;
 };
+
+// Gathered small-GEMM route of tiny contractions (one launch per call): the
+// operand transforms are folded into the MFMA operand loads and the output
+// transform into the epilogue.  One entry per element of a transformed
+// operand (value = c * source[idx], c = 0 for elements no block writes) or of
+// the core result (stored to dst[idx] scaled by c; idx < 0: stored nowhere).
+struct TinyEntry {
+  int64_t idx;
+  double c;
+};
+
+struct TinyArgs {
+  const TinyEntry* lmap;  // per element of the transformed lhs (nullptr: lhs used as is)
+  const TinyEntry* rmap;  // ... of the transformed rhs
+  const TinyEntry* omap;  // per core element (nullptr: the core result is the destination)
+  const int64_t* fix;     // destination elements that receive beta * dst (0 when beta == 0)
+  uint32_t n_fix;
+  uint32_t tile_blocks;   // workgroups of GEMM tiles; the rest sweep `fix`
+  double* dst;
+};
+
+// gemm_small_kernel's 32 x 32 one-wave tiles with the maps above; tiles of all
+// jobs in one launch, FP64.
+void launch_gemm_tiny(const GemmArgs& g, const TinyArgs& t, uint32_t n_tiles, hipStream_t s);
 
 // Tile shapes (rows x cols per workgroup) available to the autotuner.
 // TILE_SKINNY: narrow outputs (n <= 16), 256 rows x all columns per workgroup

@@ -346,10 +346,12 @@ class _Contraction:
     @property
     def route(self):
         """"fused_narrow" (one recoupling pass, coefficients gathered from the
-        rhs) or "transform_gemm" (operand transforms, grouped GEMM, output transform)."""
+        rhs), "tiny_gathered" (FP64 calls: one launch, transforms folded into
+        the GEMM's loads and stores) or "transform_gemm" (operand transforms,
+        grouped GEMM, output transform)."""
         r = C.c_int()
         self.rt._check(N.hip().tnt_contract_route(self.art, C.byref(r)))
-        return "fused_narrow" if r.value == 1 else "transform_gemm"
+        return {1: "fused_narrow", 2: "tiny_gathered"}.get(r.value, "transform_gemm")
 
     def __del__(self):
         try:

@@ -159,23 +159,31 @@ def test_fused_mpo_step_equals_general_route(rt, prec):
     general route (TENET_AMD_FUSE_NARROW=0 at upload) gives the same bits, since
     U(1) recoupling coefficients are 1 and both sum the k products in order."""
     import os
-    wl = MPSEnvUpdate(rt, "U1", 256, prec)
-    assert wl.contractions[1].route == "fused_narrow"
-    assert wl.contractions[0].route == wl.contractions[2].route == "transform_gemm"
-    out = wl.run().to_numpy().copy()
-    t2 = wl._tmp["T2"].to_numpy().copy()
-    os.environ["TENET_AMD_FUSE_NARROW"] = "0"
-    rt2 = Runtime(0)
-    wl2 = None
+    # both runtimes with the tiny route off: this compares the fused pass with
+    # the general route (the tiny route has tests of its own)
+    os.environ["TENET_AMD_TINY_FLOPS"] = "0"
+    rt1 = Runtime(0)
+    rt2 = None
+    wl = wl2 = None
     try:
+        wl = MPSEnvUpdate(rt1, "U1", 256, prec)
+        assert wl.contractions[1].route == "fused_narrow"
+        assert wl.contractions[0].route == wl.contractions[2].route == "transform_gemm"
+        out = wl.run().to_numpy().copy()
+        t2 = wl._tmp["T2"].to_numpy().copy()
+        os.environ["TENET_AMD_FUSE_NARROW"] = "0"
+        rt2 = Runtime(0)
         wl2 = MPSEnvUpdate(rt2, "U1", 256, prec)
         assert wl2.contractions[1].route == "transform_gemm"
         assert np.array_equal(wl2._tmp["T2"].to_numpy(), t2)
         assert np.array_equal(wl2.run().to_numpy(), out)
     finally:
-        del os.environ["TENET_AMD_FUSE_NARROW"]
-        del wl2
-        rt2.close()
+        os.environ.pop("TENET_AMD_FUSE_NARROW", None)
+        del os.environ["TENET_AMD_TINY_FLOPS"]
+        del wl, wl2
+        if rt2 is not None:
+            rt2.close()
+        rt1.close()
 
 
 def test_fused_route_accumulates_and_reads_adjoint_mpo(rt):
