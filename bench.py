@@ -539,6 +539,72 @@ def run_gemm_vs_vendor(args):
     rt.close()
 
 
+# The other configurations, measured after the headline one in the default
+# run (each in a child process with its own time limit, so a failure there
+# leaves the headline record intact): the SU(2) environment update at
+# D = 4096, the SU(2) F-move (FP64 and split-INT8), the batched merges, the
+# served H_eff trace (U(1) and SU(2)) and the reference's microbench.
+SECONDARY = (
+    ("su2_env_f64", ["--workload", "su2_env"], 240),
+    ("su2_fmove_f64", ["--workload", "su2_fmove", "--steps", "10", "--warmup", "2"], 400),
+    ("su2_fmove_split_i8", ["--workload", "su2_fmove", "--precision", "split_i8", "--steps", "10",
+                            "--warmup", "2", "--cpu-baseline", "off"], 300),
+    ("batched_f32", ["--workload", "batched", "--precision", "f32"], 300),
+    ("serve_u1", ["--workload", "serve", "--rule", "U1", "--cpu-baseline", "off"], 300),
+    ("serve_su2", ["--workload", "serve", "--rule", "SU2", "--cpu-baseline", "off"], 300),
+    ("microbench", ["--workload", "microbench", "--min-ms", "100"], 300),
+)
+
+
+def _summary(rec):
+    """The figures of a child bench record kept in the headline record."""
+    keep = ("metric", "value", "unit", "ms_per_step", "dtype", "parity_rel_err_vs_oracle",
+            "parity_rel_err_vs_cpu_port_full_size", "rel_err_vs_f64", "latency_ms", "offered_rate", "tflops")
+    out = {k: rec[k] for k in keep if k in rec and rec[k] is not None}
+    out["workload"] = rec.get("config", {}).get("workload")
+    r = rec.get("roofline")
+    if r:
+        out["roofline"] = {k: r.get(k) for k in ("kernel", "achieved", "peak", "unit", "frac", "traffic")}
+    if rec.get("recoupling"):
+        out["recoupling_gbs"] = rec["recoupling"]["achieved"]
+    c = rec.get("cpu_baseline")
+    if c:
+        out["cpu_baseline"] = {k: c.get(k) for k in ("value", "unit", "cores", "kind", "cpu_model")}
+    if rec.get("capacity"):
+        out["capacity_matvecs_per_s"] = rec["capacity"]["matvecs_per_s"]
+    if rec.get("r_only_swap"):
+        out["r_only_swap_gbs"] = rec["r_only_swap"]["achieved"]
+    if rec.get("cells"):
+        out["cells"] = [{"case": f"{x['symmetry']} d={x['d']} {x['workload']}", "us": round(x["us_per_iter"], 2),
+                         "us_c_abi": round(x["us_per_call_c_abi"], 2), "reference_us": x["reference_us"],
+                         "checksum": x["checksum"]} for x in rec["cells"]]
+    return out
+
+
+def run_secondary():
+    """Run the SECONDARY configurations one after another; {name: summary or error}."""
+    import subprocess
+    out = {}
+    here = os.path.dirname(os.path.abspath(__file__))
+    for name, extra, limit in SECONDARY:
+        log(f"[bench] secondary: {name}")
+        cmd = [sys.executable, os.path.join(here, "bench.py"), "--secondary", "off"] + extra
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=limit)
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": f"timed out after {limit}s"}
+            log(f"[bench] secondary {name}: timed out")
+            continue
+        lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not lines:
+            out[name] = {"error": f"exit {p.returncode}: {p.stderr.strip().splitlines()[-1:]}"}
+            log(f"[bench] secondary {name}: failed ({p.returncode})")
+            continue
+        out[name] = _summary(json.loads(lines[-1]))
+        log(f"[bench] secondary {name}: {out[name].get('value')} {out[name].get('unit')}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -570,7 +636,12 @@ def main():
                     help="replay the step from a hipGraph in the throughput region (env / F-move workloads)")
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
                     help="pick GEMM tile shapes per contraction by measurement before timing")
+    ap.add_argument("--secondary", default=None, choices=["on", "off"],
+                    help="after the headline record, measure the other configurations in child processes "
+                         "and add their figures under 'secondary' (default: on for the default workload)")
     args = ap.parse_args()
+    if args.secondary is None:
+        args.secondary = "on" if args.workload == "u1_env" else "off"
     if args.precision is None:
         # the serving configuration runs split-INT8 GEMMs with FP64 dot products
         args.precision = "split_i8" if args.workload == "serve" else "f64"
@@ -871,6 +942,11 @@ def main():
         "tile_configs": tuned,
         "rel_err_vs_f64": batched_err if args.workload == "batched" else None,
     }
+    if args.secondary == "on":
+        rt.close()
+        rec["secondary"] = run_secondary()
+        print(json.dumps(rec), flush=True)
+        return
     print(json.dumps(rec), flush=True)
     rt.close()
 

@@ -14,6 +14,12 @@ test_gpu_workloads.py):
   a permutation followed by its inverse returns the payload within 1e-14;
   split-BF16 and split-INT8 within 1e-10 of FP64.
 
+* The batched workload at its full size (256 U(1)xZ2 merges, bond dimensions
+  64..1024, 6656 sector GEMMs in one grouped launch): every merge against the
+  C++/OpenMP CPU port replaying the same descriptors on the downloaded
+  payloads (the port is pinned against the oracle in tests/test_cpu_port.py),
+  in FP64, FP32 and both split precisions, eager and as a hipGraph replay.
+
 Norms are taken on the device (FP64 weighted inner products), so nothing of
 gigabyte size crosses PCIe.
 """
@@ -151,3 +157,35 @@ def test_su2_swap_form_full_size_split_within_1e10(rt, su2_swap, prec):
     err = dev_rel(got, ref)
     del got, ref
     assert err < 1e-10, err
+
+
+@pytest.fixture(scope="module")
+def batched_ref(rt):
+    """The FP64 CPU-port results of the 256 full-size merges (seeded inputs)."""
+    from oracle import cpu_port as P
+    from tenet_amd.workloads import BatchedMerge
+    wl = BatchedMerge(rt, 256, 64, 1024, precision="f64")
+    refs = [P.Contract(c.desc, c.dst_space.len)(a.to_numpy(), b.to_numpy()) for a, b, _, c in wl.pairs]
+    del wl
+    return refs
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("split_bf16", 1e-10), ("split_i8", 1e-10)])
+def test_batched_full_size_matches_cpu_port(rt, batched_ref, prec, tol):
+    from tenet_amd.workloads import BatchedMerge
+    wl = BatchedMerge(rt, 256, 64, 1024, precision=prec)
+    assert wl.n_jobs > 6000
+    wl.run()
+    rt.synchronize()
+    got = wl.results()
+    num = sum(float(np.linalg.norm(g - r) ** 2) for g, r in zip(got, batched_ref))
+    den = sum(float(np.linalg.norm(r) ** 2) for r in batched_ref)
+    worst = max(float(np.linalg.norm(g - r) / np.linalg.norm(r)) for g, r in zip(got, batched_ref))
+    assert math.sqrt(num / den) < tol and worst < 4 * tol, (prec, math.sqrt(num / den), worst)
+    wl.capture()
+    for _, _, Cm, _ in wl.pairs:
+        Cm.scale_(0.0)
+    rt.synchronize()
+    wl.run()
+    rt.synchronize()
+    assert all(np.array_equal(x, y) for x, y in zip(wl.results(), got))
