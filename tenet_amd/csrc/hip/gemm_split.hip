@@ -242,15 +242,43 @@ __device__ __forceinline__ int stage_col(int row, int col) { return col ^ (row &
 // The same for a 16 x 32 BF16-step tile, read 8 k per lane.
 __device__ __forceinline__ int stage_col32(int row, int col) { return col ^ (((row & 1) << 4) | ((row >> 1) & 7)); }
 
+// The S BF16 digit planes of one k step: lane l holds 8 k of line l % 16 (v,
+// scaled by the line chunk's exponent e here) and writes 16 bytes of each
+// plane frag + p.
+template <int S>
+__device__ __forceinline__ void bf16_step_digits(double (&v)[8], int e, __bf16* planes, int64_t frag, int lane) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = ldexp(v[q], -e);
+#pragma unroll
+  for (int p = 0; p < S; ++p) {
+    // digit d is an integer with |d| <= 256: int -> float -> x 2^-8p -> bf16 is
+    // exact at every step (a direct double -> bf16 conversion would be
+    // compiled as a round-to-odd sequence of eight instructions)
+    const float scale = __builtin_amdgcn_ldexpf(1.0f, -8 * p);
+    bf16x8 o;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      v[q] *= 256.0;
+      const double d = rint(v[q]);
+      v[q] -= d;
+      o[q] = (__bf16)((float)(int)d * scale);
+    }
+    *reinterpret_cast<bf16x8*>(planes + ((frag + p) * FRAG + lane * 8)) = o;
+  }
+}
+
 // Wave w handles the chunk's k steps w and w + 4; lane l holds line l % 16
 // and k = 8 (l / 16) .. + 7 of a step -- the fragment order, so each lane
 // writes 16 bytes per plane.  For sources contiguous along lines, the 16 lanes
-// of one k read 128 contiguous bytes.
+// of one k read 128 contiguous bytes.  A job whose K fits one k step (ns == 1)
+// has one item per four 16-line blocks instead, one block per wave: the same
+// exponent and digits without the seven idle steps of a 256-k chunk.
 template <int S>
 __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   static_assert(CHUNK_STEPS == 8, "two k steps per wave");
   __shared__ double red[4][4][16];
   __shared__ int sexp[16];
+  __shared__ double stage[4][16][32];  // one 16 x 32 step per wave (its LDS accesses complete in order)
   const PrepItem it = prep_item(x);
   const GJob g = x.jobs[it.job];
   const SplitJob s = x.sj[it.job];
@@ -260,10 +288,46 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   const gptr<const double> M = gmem<const double>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
+  if (s.ns == 1) {
+    const int lb = it.lb * 4 + wave;
+    if (lb * 16 >= lines) return;  // whole waves; no workgroup barrier on this path
+    const int ln = lb * 16 + li;
+    double v[8];
+    if (k_contiguous(g, it.side)) {
+      double w[4][2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) w[r][c] = load_elem(M, g, it.side, lb * 16 + kg + 4 * r, 16 * c + li, lines, K);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) stage[wave][kg + 4 * r][stage_col32(kg + 4 * r, 16 * c + li)] = w[r][c];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = stage[wave][li][stage_col32(li, kg * 8 + q)];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = load_elem(M, g, it.side, ln, kg * 8 + q, lines, K);
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) mx = fmax(mx, fabs(v[q]));
+    red[wave][kg][li] = mx;
+    __builtin_amdgcn_wave_barrier();
+    double m2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[wave][c][li]);
+    int e = EXP_NONE;
+    if (m2 > 0.0) frexp(m2, &e);
+    if (e == EXP_NONE) e = 0;
+    if (kg == 0 && ln < lines) sd.exps[(it.side == 0 ? s.ea0 : s.eb0) + ln] = e;
+    bf16_step_digits<S>(v, e, static_cast<__bf16*>(sd.planes), (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)lb * S, lane);
+    return;
+  }
   const int line = it.lb * 16 + li;
   double v[2][8];
   if (k_contiguous(g, it.side)) {
-    __shared__ double stage[4][16][32];  // one 16 x 32 step per wave (its LDS accesses complete in order)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
@@ -317,24 +381,7 @@ __global__ __launch_bounds__(256) void split_digits(PrepArgs x) {
   for (int h = 0; h < 2; ++h) {
     const int st = it.kc * CHUNK_STEPS + wave + 4 * h;
     if (st >= s.ns) continue;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[h][q] = ldexp(v[h][q], -e);
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-      // digit d is an integer with |d| <= 256: int -> float -> x 2^-8p -> bf16 is
-      // exact at every step (a direct double -> bf16 conversion would be
-      // compiled as a round-to-odd sequence of eight instructions)
-      const float scale = __builtin_amdgcn_ldexpf(1.0f, -8 * p);
-      bf16x8 o;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        v[h][q] *= 256.0;
-        const double d = rint(v[h][q]);
-        v[h][q] -= d;
-        o[q] = (__bf16)((float)(int)d * scale);
-      }
-      *reinterpret_cast<bf16x8*>(static_cast<__bf16*>(sd.planes) + ((f0 + (int64_t)st * S + p) * FRAG + lane * 8)) = o;
-    }
+    bf16_step_digits<S>(v[h], e, static_cast<__bf16*>(sd.planes), f0 + (int64_t)st * S, lane);
   }
 }
 
@@ -425,10 +472,59 @@ __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
   }
 }
 
+// The S INT8 digit planes of one 64-k step: lane l holds 16 k of line l % 16
+// (v) and writes 16 bytes of each plane frag + p; e is the line's exponent
+// over all of K.  a = v 2^(7-e) is cut into d0 = rint(a) and S-1 base-256
+// digits by round-to-nearest; a digit of 128 becomes -128 with a carry into
+// the digit above, from the last digit up, so every digit fits INT8.
+template <int S>
+__device__ __forceinline__ void i8_step_digits(const double (&v)[16], int e, void* planes, int64_t frag, int lane) {
+  uint32_t packed[S][4];
+#pragma unroll
+  for (int p = 0; p < S; ++p)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    double a = ldexp(v[q], 7 - e);  // |a| < 126.5
+    int d[S];
+    double r = rint(a);
+    d[0] = (int)r;
+    a -= r;                         // |a| <= 1/2, exact
+#pragma unroll
+    for (int p = 1; p < S; ++p) {
+      a *= 256.0;
+      r = rint(a);
+      d[p] = (int)r;                // in [-128, 128]
+      a -= r;
+    }
+    int carry = 0;
+#pragma unroll
+    for (int p = S - 1; p >= 1; --p) {
+      d[p] += carry;
+      carry = d[p] > 127 ? 1 : 0;
+      d[p] -= carry << 8;
+    }
+    d[0] += carry;
+#pragma unroll
+    for (int p = 0; p < S; ++p) packed[p][q >> 2] |= (uint32_t)(d[p] & 0xff) << (8 * (q & 3));
+  }
+#pragma unroll
+  for (int p = 0; p < S; ++p) {
+    const i32x4 o{(int)packed[p][0], (int)packed[p][1], (int)packed[p][2], (int)packed[p][3]};
+    *reinterpret_cast<i32x4*>(static_cast<char*>(planes) + (frag + p) * FRAG_BYTES + lane * 16) = o;
+  }
+}
+
+// One item per (side, job, 16-line block, 512-k chunk), as the BF16 pre-pass;
+// a job whose K fits one 64-k step (ns == 1) has one item per four line
+// blocks, one block per wave.
 template <int S>
 __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   static_assert(S >= 2 && S <= 6, "digit count");
   __shared__ int sexp[16];
+  __shared__ double red[4][4][16];
+  __shared__ double stage[4][16][64];  // one 16 x 64 step per wave (its LDS accesses complete in order)
   const PrepItem it = prep_item(x);
   const GJob g = x.jobs[it.job];
   const SplitJob s = x.sj[it.job];
@@ -438,6 +534,45 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   const gptr<const double> M = gmem<const double>(sd.base) + (it.side == 0 ? g.a_off : g.b_off);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int li = lane & 15, kg = lane >> 4;
+  if (s.ns == 1) {  // (then nkc == 1 too: the exponent comes from this step)
+    const int lb = it.lb * 4 + wave;
+    if (lb * 16 >= lines) return;  // whole waves; no workgroup barrier on this path
+    const int ln = lb * 16 + li;
+    double v[16];
+    if (k_contiguous(g, it.side)) {
+      double w[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) w[r][c] = load_elem(M, g, it.side, lb * 16 + kg + 4 * r, 16 * c + li, lines, K);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) stage[wave][kg + 4 * r][stage_col(kg + 4 * r, 16 * c + li)] = w[r][c];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = stage[wave][li][stage_col(li, kg * 16 + q)];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = load_elem(M, g, it.side, ln, kg * 16 + q, lines, K);
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(v[q]));
+    red[wave][kg][li] = mx;
+    __builtin_amdgcn_wave_barrier();
+    double m2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m2 = fmax(m2, red[wave][c][li]);
+    int e = 0;  // all-zero line: its digits are zero
+    if (m2 > 0.0) {
+      frexp(m2, &e);
+      if (ldexp(m2, 7 - e) >= 126.5) ++e;
+    }
+    if (kg == 0 && ln < lines) sd.rows[(it.side == 0 ? s.ra0 : s.rb0) + ln] = e;
+    i8_step_digits<S>(v, e, sd.planes, (it.side == 0 ? s.fa0 : s.fb0) + (int64_t)lb * S, lane);
+    return;
+  }
   const int line = it.lb * 16 + li;
   double v[2][16];
   if (k_contiguous(g, it.side)) {
@@ -445,7 +580,6 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
     // (lane / 16, lane % 16) take k = 16 c + lane % 16 of lines lane / 16 +
     // 4 r, four whole 128-byte lines per load), then fragment-order reads.
     // The tile belongs to one wave, whose LDS accesses complete in order.
-    __shared__ double stage[4][16][64];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
@@ -474,7 +608,6 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
     }
   }
   if (s.nkc == 1) {  // the line's exponent from this chunk (no exponent pass ran)
-    __shared__ double red[4][4][16];
     double mx = 0.0;
 #pragma unroll
     for (int h = 0; h < 2; ++h)
@@ -515,41 +648,7 @@ __global__ __launch_bounds__(256) void split_i8_digits(PrepArgs x) {
   for (int h = 0; h < 2; ++h) {
     const int st = it.kc * (I8_KCHUNK / I8_KSTEP) + wave + 4 * h;
     if (st >= s.ns) continue;
-    uint32_t packed[S][4];
-#pragma unroll
-    for (int p = 0; p < S; ++p)
-#pragma unroll
-      for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      double a = ldexp(v[h][q], 7 - e);  // |a| < 126.5
-      int d[S];
-      double r = rint(a);
-      d[0] = (int)r;
-      a -= r;                         // |a| <= 1/2, exact
-#pragma unroll
-      for (int p = 1; p < S; ++p) {
-        a *= 256.0;
-        r = rint(a);
-        d[p] = (int)r;                // in [-128, 128]
-        a -= r;
-      }
-      int carry = 0;
-#pragma unroll
-      for (int p = S - 1; p >= 1; --p) {
-        d[p] += carry;
-        carry = d[p] > 127 ? 1 : 0;
-        d[p] -= carry << 8;
-      }
-      d[0] += carry;
-#pragma unroll
-      for (int p = 0; p < S; ++p) packed[p][q >> 2] |= (uint32_t)(d[p] & 0xff) << (8 * (q & 3));
-    }
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-      const i32x4 o{(int)packed[p][0], (int)packed[p][1], (int)packed[p][2], (int)packed[p][3]};
-      *reinterpret_cast<i32x4*>(static_cast<char*>(sd.planes) + (f0 + (int64_t)st * S + p) * FRAG_BYTES + lane * 16) = o;
-    }
+    i8_step_digits<S>(v[h], e, sd.planes, f0 + (int64_t)st * S, lane);
   }
 }
 
@@ -1016,7 +1115,10 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob
       }
       x.cfg = (j.m > 0 && j.n > 0) ? split_shape_for(j, small_tiles, S) : SPLIT_64x64;
       // operands of a job with an empty output are never read: no pre-pass items
-      const int64_t ia = (j.n > 0) ? (int64_t)x.lba * x.nkc : 0, ib = (j.m > 0) ? (int64_t)x.lbb * x.nkc : 0;
+      // one-step jobs (ns == 1): one item per four 16-line blocks (one per wave)
+      const bool one_step = x.ns == 1;
+      const int64_t ia = (j.n > 0) ? (one_step ? (x.lba + 3) / 4 : (int64_t)x.lba * x.nkc) : 0;
+      const int64_t ib = (j.m > 0) ? (one_step ? (x.lbb + 3) / 4 : (int64_t)x.lbb * x.nkc) : 0;
       sj[i] = x;
       pre_a.push_back(pre_a.back() + ia);
       pre_b.push_back(pre_b.back() + ib);
