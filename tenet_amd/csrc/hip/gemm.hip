@@ -167,21 +167,28 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
     store(0);
   }
   __syncthreads();
+  // A wave whose part of the tile lies wholly outside the job (in a right or
+  // bottom edge tile) skips its LDS reads and MFMAs -- the waves of the other
+  // workgroups on its SIMD get the matrix core -- but keeps loading its share
+  // of the slabs and meets every barrier; its epilogue stores nothing.
+  const bool live = __builtin_amdgcn_readfirstlane((m0 + wm < m) && (n0 + wn < n));
   for (int t = 0; t < nk; ++t) {
     const int cur = t & 1;
     if (t + 1 < nk) load((t + 1) * BK);
+    if (live) {
 #pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
-      const int kk = ks * 4 + (lane >> 4);
-      T af[MT], bf[NT];
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        const int kk = ks * 4 + (lane >> 4);
+        T af[MT], bf[NT];
 #pragma unroll
-      for (int i = 0; i < MT; ++i) af[i] = As[cur][kk][wm + i * 16 + (lane & 15)];
+        for (int i = 0; i < MT; ++i) af[i] = As[cur][kk][wm + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bf[j] = Bs[cur][kk][wn + j * 16 + (lane & 15)];
+        for (int j = 0; j < NT; ++j) bf[j] = Bs[cur][kk][wn + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < MT; ++i)
+        for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+      }
     }
     if (t + 1 < nk) store(cur ^ 1);
     __syncthreads();
