@@ -333,6 +333,19 @@ int tnt_event_destroy(tnt_ctx* ctx, tnt_event* ev);
  * No reference counterpart (the reference's profiles are host wall-clock,
  * tenet-operations/src/profile.rs:16-59). */
 int tnt_ctx_set_timing(tnt_ctx* ctx, int on);
+/* Debug shadow write counts (no reference counterpart; the reference relies
+ * on its disjoint-destination invariant, tenet-dense/src/executor.rs:8-13).
+ * When on, every recoupling replay, FP64/FP32 block-GEMM table run (with the
+ * uncovered core regions) and tiny-route call is followed by a pass that
+ * counts, per destination element, the writes of the work decomposition that
+ * ran (plan items, GEMM tiles, regions, output-map stores): `writes` sums the
+ * counts, `doubles` counts elements written more than once, `expected` sums
+ * the writes the compiled plans prescribe, `passes` the checked
+ * destinations, all since the last tnt_ctx_set_shadow(ctx, 1).  A correct
+ * decomposition gives writes == expected and doubles == 0.  Not during graph
+ * capture; allocates a 4-byte counter per destination element. */
+int tnt_ctx_set_shadow(tnt_ctx* ctx, int on);
+int tnt_ctx_shadow_stats(tnt_ctx* ctx, uint64_t* writes, uint64_t* expected, uint64_t* doubles, uint64_t* passes);
 int tnt_ctx_kernel_stats(tnt_ctx* ctx, int kind, size_t* launches, double* total_ms, double* total_work);
 int tnt_ctx_reset_stats(tnt_ctx* ctx);
 

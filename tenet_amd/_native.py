@@ -123,6 +123,9 @@ HIP_API = {
     "tnt_event_query": (C.c_int, [VP, VP]),
     "tnt_event_destroy": (C.c_int, [VP, VP]),
     "tnt_ctx_set_timing": (C.c_int, [VP, C.c_int]),
+    "tnt_ctx_set_shadow": (C.c_int, [VP, C.c_int]),
+    "tnt_ctx_shadow_stats": (C.c_int, [VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]),
     "tnt_ctx_kernel_stats": (C.c_int, [VP, C.c_int, SZP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tnt_ctx_reset_stats": (C.c_int, [VP]),
     "tnt_set_tile_config": (C.c_int, [VP, C.c_int]),

@@ -96,6 +96,14 @@ struct tnt_ctx {
     double work;  // flops (GEMM) or bytes (recoupling)
   };
   std::vector<Timed> timed;
+  // debug shadow write counts (tnt_ctx_set_shadow): per-element counts of one
+  // destination, and on the device the running totals (writes, elements
+  // written more than once); on the host the writes the plans expect
+  bool shadow = false;
+  uint32_t* shadow_cnt = nullptr;
+  size_t shadow_cap = 0;
+  unsigned long long* shadow_stats = nullptr;
+  uint64_t shadow_expected = 0, shadow_passes = 0;
 };
 
 static hipEvent_t take_event(tnt_ctx* ctx) {
@@ -164,6 +172,7 @@ struct tnt_plan {
   std::vector<void*> allocs;
   double bytes_moved = 0;  // algorithmic bytes of one replay (read src + write dst)
   bool one_launch = false;  // items of several classes, small: one launch of every class
+  uint64_t dst_writes = 0;  // destination element writes of one replay (shadow check)
 };
 
 // Plans of several item classes that move fewer bytes than this (default
@@ -226,6 +235,8 @@ struct tnt_artifact {
   GTile* t_tiles = nullptr;
   uint32_t t_n_tiles = 0;
   double t_flops = 0;
+  uint64_t t_writes = 0;     // destination elements the tiny route writes per call
+  uint64_t core_writes = 0;  // core elements written by the GEMM jobs and the uncovered regions
 };
 
 // The fused narrow route is on unless TENET_AMD_FUSE_NARROW=0 (for A/B runs
@@ -532,6 +543,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   std::vector<XFlat> flat;
   std::string err;
   double bytes = 0;
+  uint64_t dst_writes = 0;
   // item sizes: Single kSingleItem elements, Multi ~kMultiItemWork element
   // updates (elements x layouts / 2), both shrunk for small plans so that each
   // class has about min_plan_items() items (never below 256 elements)
@@ -610,6 +622,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     for (size_t i = 2; i < dims.size(); ++i) outer *= dims[i];
     int64_t elems = plane * outer;
     bytes += 8.0 * double(elems) * double(dl.size() + sl.size());
+    dst_writes += (uint64_t)elems * dl.size();
     // Single blocks: items of up to kSingleItem elements (kSingleUnroll per
     // thread) swept in order by a persistent grid; Multi: ~kMultiItemWork
     // element-updates per item, weighted by its layouts (both smaller for small
@@ -703,6 +716,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   p->dst_len = d->dst_len;
   p->src_len = d->src_len;
   p->bytes_moved = bytes;
+  p->dst_writes = dst_writes;
   std::vector<double> coeffs(d->coeffs, d->coeffs + d->n_coeffs);
   std::vector<XItem> items;
   for (int c = 0; c < XC_COUNT; ++c) {
@@ -729,17 +743,47 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   return TNT_OK;
 }
 
+// Shadow check of one destination (debug option): counts the writes of a
+// work decomposition (launch(cnt)) over `len` elements and adds them to the
+// context's totals, with the writes the decomposition is planned to make.
+// Not while capturing a graph.  Returns non-zero only when the count buffer
+// cannot be allocated.
+extern "C++" template <class F>
+static int shadow_pass(tnt_ctx* ctx, int64_t len, uint64_t expected, F&& launch) {
+  if (!ctx->shadow || ctx->capturing || len <= 0) return TNT_OK;
+  if ((size_t)len > ctx->shadow_cap) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->shadow_cnt);
+    ctx->shadow_cnt = nullptr;
+    ctx->shadow_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->shadow_cnt, sizeof(uint32_t) * (size_t)len));
+    ctx->shadow_cap = (size_t)len;
+  }
+  launch_zero_f32(reinterpret_cast<float*>(ctx->shadow_cnt), len, ctx->stream);
+  launch(ctx->shadow_cnt);
+  launch_shadow_reduce(ctx->shadow_cnt, len, ctx->shadow_stats, ctx->stream);
+  ctx->shadow_expected += expected;
+  ctx->shadow_passes += 1;
+  return TNT_OK;
+}
+
 static void run_plan(tnt_ctx* ctx, const tnt_plan* p, double* dst, const double* src, double alpha,
                      double beta, int overwrite) {
   XformArgs a{p->d.blocks, p->d.layouts, p->d.items, p->d.flat, p->d.coeffs, dst, src, alpha, beta, overwrite};
-  TimedScope ts(ctx, 0, p->bytes_moved);
-  if (p->one_launch) {
-    launch_xform_any(a, p->d.n_items, ctx->stream);
-    return;
+  {
+    TimedScope ts(ctx, 0, p->bytes_moved);
+    if (p->one_launch) {
+      launch_xform_any(a, p->d.n_items, ctx->stream);
+    } else {
+      launch_xform_single(a, p->d.cls_begin[XC_SINGLE], p->d.cls_count[XC_SINGLE], p->d.cls_count[XC_FLAT],
+                          ctx->stream);
+      for (int c = 0; c < XC_COUNT; ++c)
+        if (c != XC_SINGLE && c != XC_FLAT) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
+    }
   }
-  launch_xform_single(a, p->d.cls_begin[XC_SINGLE], p->d.cls_count[XC_SINGLE], p->d.cls_count[XC_FLAT], ctx->stream);
-  for (int c = 0; c < XC_COUNT; ++c)
-    if (c != XC_SINGLE && c != XC_FLAT) launch_xform(a, c, p->d.cls_begin[c], p->d.cls_count[c], ctx->stream);
+  (void)shadow_pass(ctx, p->dst_len, p->dst_writes, [&](uint32_t* cnt) {
+    launch_shadow_plan(p->d.blocks, p->d.layouts, p->d.items, p->d.n_items, cnt, ctx->stream);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,6 +1073,8 @@ int tnt_ctx_destroy(tnt_ctx* ctx) {
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   (void)hipFree(ctx->dev_scalar);
   (void)hipHostFree(ctx->host_scalar);
+  (void)hipFree(ctx->shadow_cnt);
+  (void)hipFree(ctx->shadow_stats);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return TNT_OK;
@@ -1401,6 +1447,11 @@ static int build_tiny_route(tnt_ctx* ctx, const tnt_contract_desc* d, const std:
   if (rc) return rc;
   a->t_n_fix = (uint32_t)fix.size();
   a->t_n_tiles = (uint32_t)tiles.size();
+  a->t_writes = fix.size();
+  for (const GJob& j : gj)
+    for (int col = 0; col < j.n; ++col)
+      for (int row = 0; row < j.m; ++row)
+        if (om.empty() || om[(size_t)(j.d_off + row + (int64_t)col * j.ldd)].idx >= 0) ++a->t_writes;
   a->t_flops = flops;
   a->tiny = true;
   return TNT_OK;
@@ -1455,6 +1506,8 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     ul.push_back(d->uncovered[i].len);
   }
   a->n_unc = (uint32_t)uo.size();
+  for (const GJob& j : gj) a->core_writes += (uint64_t)std::max(j.m, 0) * (uint64_t)std::max(j.n, 0);
+  for (int64_t l : ul) a->core_writes += (uint64_t)l;
   if (int rc = upload_vec(ctx, uo, &a->unc_off, a->allocs)) return fail(rc);
   if (int rc = upload_vec(ctx, ul, &a->unc_len, a->allocs)) return fail(rc);
   if (fusion_enabled()) {
@@ -1522,6 +1575,36 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
   return TNT_OK;
 }
 
+int tnt_ctx_set_shadow(tnt_ctx* ctx, int on) {
+  if (!ctx) return TNT_E_INVALID;
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "the shadow check cannot change during graph capture");
+  if (int rc = set_device(ctx)) return rc;
+  if (on) {
+    if (!ctx->shadow_stats) HIPCHK(ctx, hipMalloc(&ctx->shadow_stats, 2 * sizeof(unsigned long long)));
+    launch_fill(reinterpret_cast<double*>(ctx->shadow_stats), 2, 0.0, ctx->stream);  // all-zero bits
+    HIPCHK(ctx, hipGetLastError());
+    ctx->shadow_expected = ctx->shadow_passes = 0;
+  }
+  ctx->shadow = on != 0;
+  return TNT_OK;
+}
+
+int tnt_ctx_shadow_stats(tnt_ctx* ctx, uint64_t* writes, uint64_t* expected, uint64_t* doubles, uint64_t* passes) {
+  if (!ctx || !writes || !expected || !doubles || !passes) return TNT_E_INVALID;
+  if (ctx->capturing) FAIL(ctx, TNT_E_INVALID, "tnt_ctx_shadow_stats synchronises and cannot be captured");
+  if (int rc = set_device(ctx)) return rc;
+  unsigned long long st[2] = {0ull, 0ull};
+  if (ctx->shadow_stats) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(st, ctx->shadow_stats, sizeof(st), hipMemcpyDeviceToHost));
+  }
+  *writes = st[0];
+  *doubles = st[1];
+  *expected = ctx->shadow_expected;
+  *passes = ctx->shadow_passes;
+  return TNT_OK;
+}
+
 int tnt_contract_route(const tnt_artifact* a, int* route) {
   if (!a || !route) return TNT_E_INVALID;
   *route = a->fused ? TNT_ROUTE_FUSED_NARROW : a->tiny ? TNT_ROUTE_TINY_GATHERED : TNT_ROUTE_TRANSFORM_GEMM;
@@ -1568,6 +1651,10 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     GemmArgs g{a->t_jobs, a->t_tiles, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta};
     TinyArgs t{a->t_lmap, a->t_rmap, a->t_omap, a->t_fix, a->t_n_fix, 0u, static_cast<double*>(dst->ptr)};
     launch_gemm_tiny(g, t, a->t_n_tiles, ctx->stream);
+    if (int rc = shadow_pass(ctx, a->out ? a->out->dst_len : a->core_len, a->t_writes, [&](uint32_t* cnt) {
+          launch_shadow_tiny(a->t_jobs, a->t_tiles, a->t_n_tiles, a->t_omap, a->t_fix, a->t_n_fix, cnt, ctx->stream);
+        }))
+      return rc;
     return launch_status(ctx);
   }
   if (a->lhs) {
@@ -1579,15 +1666,27 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     R = a->scratch_r;
   }
   int rc;
+  // shadow check of the core destination: the GEMM table's tiles (FP64 / FP32
+  // tables; the split precisions run their own tiles) and the uncovered regions
+  auto shadow_core = [&]() -> int {
+    if (precision != TNT_PREC_F64 && precision != TNT_PREC_F32) return TNT_OK;
+    const uint32_t n_tiles = (uint32_t)a->gemm.host_tiles.size();
+    return shadow_pass(ctx, a->core_len, a->core_writes, [&](uint32_t* cnt) {
+      launch_shadow_tiles(a->gemm.jobs, a->gemm.tiles, n_tiles, cnt, ctx->stream);
+      launch_shadow_regions(a->unc_off, a->unc_len, a->n_unc, cnt, ctx->stream);
+    });
+  };
   if (a->out) {
     launch_scale_regions(a->scratch_c, a->unc_off, a->unc_len, a->n_unc, 0.0, ctx->stream);
     rc = run_gemm_table(ctx, a->gemm, (uintptr_t)a->scratch_c, (uintptr_t)L, (uintptr_t)R, 1.0, 0.0, precision);
     if (rc) return rc;
+    if ((rc = shadow_core())) return rc;
     run_plan(ctx, a->out, static_cast<double*>(dst->ptr), a->scratch_c, alpha, beta, 0);
   } else {
     launch_scale_regions(static_cast<double*>(dst->ptr), a->unc_off, a->unc_len, a->n_unc, beta, ctx->stream);
     rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision);
     if (rc) return rc;
+    if ((rc = shadow_core())) return rc;
   }
   return launch_status(ctx);
 }

@@ -186,11 +186,11 @@ enum TileCfg : int32_t {
   TILE_SMALL = 6,
   TILE_CFG_COUNT = 7
 };
-inline int tile_rows(int cfg) {
+__host__ __device__ inline int tile_rows(int cfg) {
   if (cfg == TILE_SMALL) return 32;
   return cfg == TILE_SKINNY ? 256 : (cfg == TILE_128x128 || cfg == TILE_128x64 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
-inline int tile_cols(int cfg) {
+__host__ __device__ inline int tile_cols(int cfg) {
   if (cfg == TILE_SMALL) return 32;
   return cfg == TILE_SKINNY ? 16 : (cfg == TILE_128x128 || cfg == TILE_64x128 || cfg == TILE_128x128_W8) ? 128 : 64;
 }
@@ -249,5 +249,21 @@ void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,
                           double beta, hipStream_t s);
+
+// Shadow write counts (debug option tnt_ctx_set_shadow): every destination
+// element that a work decomposition writes is counted with a 32-bit atomic
+// into cnt -- the items of a recoupling plan (each element of each of a
+// block's destination layouts), the tiles of a GEMM table (the valid rows and
+// columns of each tile), contiguous regions, and the tiny route's stores
+// (through its output map) and fix-ups.  launch_shadow_reduce adds the sum of
+// cnt[0, n) to stats[0] and the number of elements counted more than once to
+// stats[1].
+void launch_shadow_plan(const XBlock* blocks, const XLayout* layouts, const XItem* items, uint32_t n_items,
+                        uint32_t* cnt, hipStream_t s);
+void launch_shadow_tiles(const GJob* jobs, const GTile* tiles, uint32_t n_tiles, uint32_t* cnt, hipStream_t s);
+void launch_shadow_regions(const int64_t* off, const int64_t* len, uint32_t n, uint32_t* cnt, hipStream_t s);
+void launch_shadow_tiny(const GJob* jobs, const GTile* tiles, uint32_t n_tiles, const TinyEntry* omap,
+                        const int64_t* fix, uint32_t n_fix, uint32_t* cnt, hipStream_t s);
+void launch_shadow_reduce(const uint32_t* cnt, int64_t n, unsigned long long* stats, hipStream_t s);
 
 }  // namespace tnt_dev

@@ -21,6 +21,8 @@
 // outer indices, in the same launch.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace tnt_dev {
 
 static_assert(sizeof(XItem) == 24, "XItem layout");
@@ -497,4 +499,125 @@ void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipSt
   }
 }
 
+// ---------------------------------------------------------------------------
+// Shadow write counts (debug): the element sets of the replay above, counted.
+// A plan item covers plane elements [e_begin, e_begin + e_count) of outer
+// indices [o_begin, o_begin + o_count) in every destination layout of its
+// block; a transpose item is one 64 x 64 tile of one plane.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void shadow_plan_kernel(const XBlock* blocks, const XLayout* layouts,
+                                                         const XItem* items, uint32_t n, uint32_t* cnt) {
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const XItem it = items[i];
+    const XBlock& b = blocks[it.block];
+    const int64_t n0 = b.dims[0], n1 = b.rank > 1 ? b.dims[1] : 1;
+    int64_t idx[kMaxRank];
+    if (b.kind == XK_TRANSPOSE) {
+      const int64_t T = kTransposeTile, ntx = (n0 + T - 1) / T;
+      const int64_t x0 = (int64_t)(it.e_begin % ntx) * T, y0 = (int64_t)(it.e_begin / ntx) * T;
+      outer_index(b, it.o_begin, idx);
+      const XLayout& L = layouts[b.lay];
+      const int64_t base = base_offset(L, idx, b.rank);
+      for (int e = threadIdx.x; e < T * T; e += 256) {
+        const int64_t x = x0 + e % T, y = y0 + e / T;
+        if (x < n0 && y < n1) atomicAdd(cnt + base + x * L.strides[0] + y * L.strides[1], 1u);
+      }
+      continue;
+    }
+    for (uint64_t o = it.o_begin; o < it.o_begin + it.o_count; ++o) {
+      outer_index(b, o, idx);
+      for (int j = 0; j < b.ndst; ++j) {
+        const XLayout& L = layouts[b.lay + j];
+        const int64_t base = base_offset(L, idx, b.rank);
+        const int64_t s1 = b.rank > 1 ? L.strides[1] : 0;
+        for (uint32_t e = it.e_begin + threadIdx.x; e < it.e_begin + it.e_count; e += 256) {
+          const int64_t x = e % n0, y = e / n0;
+          atomicAdd(cnt + base + x * L.strides[0] + y * s1, 1u);
+        }
+      }
+    }
+  }
+}
+
+// A tile: rows [tm bm, +bm) x columns [tn bn, +bn) of its job, the valid part.
+// With an output map, element e of the core goes to omap[e].idx (if >= 0).
+__global__ __launch_bounds__(256) void shadow_tiles_kernel(const GJob* jobs, const GTile* tiles, uint32_t n,
+                                                          const TinyEntry* omap, uint32_t* cnt) {
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const GTile t = tiles[i];
+    if (t.job < 0) continue;
+    const GJob j = jobs[t.job];
+    const int bm = tile_rows(t.cfg), bn = tile_cols(t.cfg);
+    const int m0 = t.tm * bm, n0 = t.tn * bn;
+    for (int e = threadIdx.x; e < bm * bn; e += 256) {
+      const int row = m0 + e % bm, col = n0 + e / bm;
+      if (row >= j.m || col >= j.n) continue;
+      const int64_t c = j.d_off + row + (int64_t)col * j.ldd;
+      const int64_t d = omap ? omap[c].idx : c;
+      if (d >= 0) atomicAdd(cnt + d, 1u);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void shadow_regions_kernel(const int64_t* off, const int64_t* len, uint32_t n,
+                                                            uint32_t* cnt) {
+  for (uint32_t r = blockIdx.x; r < n; r += gridDim.x)
+    for (int64_t e = threadIdx.x; e < len[r]; e += 256) atomicAdd(cnt + off[r] + e, 1u);
+}
+
+__global__ __launch_bounds__(256) void shadow_list_kernel(const int64_t* idx, uint32_t n, uint32_t* cnt) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) atomicAdd(cnt + idx[i], 1u);
+}
+
+__global__ __launch_bounds__(256) void shadow_reduce_kernel(const uint32_t* cnt, int64_t n, unsigned long long* stats) {
+  __shared__ unsigned long long part[2];
+  if (threadIdx.x < 2) part[threadIdx.x] = 0ull;
+  __syncthreads();
+  unsigned long long sum = 0, dup = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const uint32_t c = cnt[i];
+    sum += c;
+    dup += c > 1u ? 1ull : 0ull;
+  }
+  atomicAdd(&part[0], sum);
+  atomicAdd(&part[1], dup);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(stats, part[0]);
+    atomicAdd(stats + 1, part[1]);
+  }
+}
+
+static uint32_t shadow_grid(uint64_t n) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(n, 1), 4096); }
+
+void launch_shadow_plan(const XBlock* blocks, const XLayout* layouts, const XItem* items, uint32_t n_items,
+                        uint32_t* cnt, hipStream_t s) {
+  if (n_items == 0) return;
+  hipLaunchKernelGGL(shadow_plan_kernel, dim3(shadow_grid(n_items)), dim3(256), 0, s, blocks, layouts, items, n_items, cnt);
+}
+
+void launch_shadow_tiles(const GJob* jobs, const GTile* tiles, uint32_t n_tiles, uint32_t* cnt, hipStream_t s) {
+  if (n_tiles == 0) return;
+  hipLaunchKernelGGL(shadow_tiles_kernel, dim3(shadow_grid(n_tiles)), dim3(256), 0, s, jobs, tiles, n_tiles,
+                     (const TinyEntry*)nullptr, cnt);
+}
+
+void launch_shadow_regions(const int64_t* off, const int64_t* len, uint32_t n, uint32_t* cnt, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(shadow_regions_kernel, dim3(shadow_grid(n)), dim3(256), 0, s, off, len, n, cnt);
+}
+
+void launch_shadow_tiny(const GJob* jobs, const GTile* tiles, uint32_t n_tiles, const TinyEntry* omap,
+                        const int64_t* fix, uint32_t n_fix, uint32_t* cnt, hipStream_t s) {
+  if (n_tiles) hipLaunchKernelGGL(shadow_tiles_kernel, dim3(shadow_grid(n_tiles)), dim3(256), 0, s, jobs, tiles, n_tiles, omap, cnt);
+  if (n_fix) hipLaunchKernelGGL(shadow_list_kernel, dim3((n_fix + 255) / 256), dim3(256), 0, s, fix, n_fix, cnt);
+}
+
+void launch_shadow_reduce(const uint32_t* cnt, int64_t n, unsigned long long* stats, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(shadow_reduce_kernel, dim3(shadow_grid((uint64_t)(n + 255) / 256)), dim3(256), 0, s, cnt, n, stats);
+}
+
 }  // namespace tnt_dev
+
