@@ -544,16 +544,17 @@ def run_gemm_vs_vendor(args):
 # leaves the headline record intact): the SU(2) environment update at
 # D = 4096, the SU(2) F-move (FP64 and split-INT8), the batched merges, the
 # served H_eff trace (U(1) and SU(2)) and the reference's microbench.
-SECONDARY = (
-    ("su2_env_f64", ["--workload", "su2_env"], 240),
-    ("su2_fmove_f64", ["--workload", "su2_fmove", "--steps", "10", "--warmup", "2"], 400),
+SECONDARY = (  # (name, arguments, time limit in seconds; each takes 5-25 s)
+    ("su2_env_f64", ["--workload", "su2_env"], 90),
+    ("su2_fmove_f64", ["--workload", "su2_fmove", "--steps", "10", "--warmup", "2"], 150),
     ("su2_fmove_split_i8", ["--workload", "su2_fmove", "--precision", "split_i8", "--steps", "10",
-                            "--warmup", "2", "--cpu-baseline", "off"], 300),
-    ("batched_f32", ["--workload", "batched", "--precision", "f32"], 300),
-    ("serve_u1", ["--workload", "serve", "--rule", "U1", "--cpu-baseline", "off"], 300),
-    ("serve_su2", ["--workload", "serve", "--rule", "SU2", "--cpu-baseline", "off"], 300),
-    ("microbench", ["--workload", "microbench", "--min-ms", "100"], 300),
+                            "--warmup", "2", "--cpu-baseline", "off"], 90),
+    ("batched_f32", ["--workload", "batched", "--precision", "f32"], 90),
+    ("serve_u1", ["--workload", "serve", "--rule", "U1", "--cpu-baseline", "off"], 90),
+    ("serve_su2", ["--workload", "serve", "--rule", "SU2", "--cpu-baseline", "off"], 90),
+    ("microbench", ["--workload", "microbench", "--min-ms", "100"], 90),
 )
+SECONDARY_BUDGET_S = 300.0  # configurations still waiting when this is spent are skipped
 
 
 def _summary(rec):
@@ -586,7 +587,13 @@ def run_secondary():
     import subprocess
     out = {}
     here = os.path.dirname(os.path.abspath(__file__))
+    t0 = time.perf_counter()
     for name, extra, limit in SECONDARY:
+        left = SECONDARY_BUDGET_S - (time.perf_counter() - t0)
+        if left < 10.0:
+            out[name] = {"error": "skipped: the secondary time budget is spent"}
+            continue
+        limit = min(limit, left)
         log(f"[bench] secondary: {name}")
         cmd = [sys.executable, os.path.join(here, "bench.py"), "--secondary", "off"] + extra
         try:
