@@ -612,6 +612,114 @@ def run_secondary():
     return out
 
 
+def run_complex_gemm(args):
+    """The core GEMM jobs of the U(1) environment update (as gemm_vs_vendor)
+    on complex (C64) payloads: one tnt_gemm_batch_complex call per
+    contraction (expand pass, real m x 2k x 2n grouped MFMA GEMM, combine pass)
+    against the vendor complex GEMM behind torch.mm (one call per job), on the
+    same device buffers.  Complex flops = 8 mkn.  One JSON line."""
+    import torch
+
+    from tenet_amd import _native as N
+    from tenet_amd.tensor import HomSpace, Runtime, _HostContraction
+    from tenet_amd.workloads import MPSEnvUpdate, mps_env_spaces
+    r, V, P, M = mps_env_spaces("U1", args.D)
+    L, A, W = HomSpace(r, [V, M], [V]), HomSpace(r, [V], [P, V]), HomSpace(r, [M, P], [P, M])
+    c1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False)
+    c2 = _HostContraction(c1.dst_space, W, *MPSEnvUpdate.T2_AXES, False, False)
+    c3 = _HostContraction(A, c2.dst_space, *MPSEnvUpdate.LP_AXES, True, False)
+    groups = []
+    for c in (c1, c3):
+        d = c.desc.contents
+        groups.append([(d.jobs[i].rows, d.jobs[i].contracted, d.jobs[i].cols) for i in range(d.n_jobs)])
+    shapes = [s_ for g_ in groups for s_ in g_]
+    flops = float(sum(8 * m * k * n for m, k, n in shapes))
+    precision = args.precision or "f64"
+    prec = {"f64": 0, "f32": 1, "split_bf16": 2, "split_i8": 3}[precision]
+    rt = Runtime(0)
+    h = N.hip()
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(1)
+    calls = []
+    for grp in groups:
+        nl = sum(m * k for m, k, n in grp)
+        nr = sum(k * n for m, k, n in grp)
+        nd = sum(m * n for m, k, n in grp)
+        tl = torch.randn(nl, dtype=torch.complex128, device=dev, generator=g)
+        tr = torch.randn(nr, dtype=torch.complex128, device=dev, generator=g)
+        td = torch.zeros(nd, dtype=torch.complex128, device=dev)
+        js, views, lo, ro, do = [], [], 0, 0, 0
+        for m, k, n in grp:
+            js.append(N.GemmJob(do, lo, ro, m, k, n))
+            # column-major (m x k) A, (k x n) B, (m x n) C are row-major (k, m),
+            # (n, k), (n, m) tensors, and torch.mm(B^T, A^T) = C^T
+            views.append((tl[lo:lo + m * k].view(k, m), tr[ro:ro + k * n].view(n, k),
+                          td[do:do + m * n].view(n, m)))
+            lo, ro, do = lo + m * k, ro + k * n, do + m * n
+        bufs = []
+        for t_ in (td, tl, tr):
+            b = C.c_void_p()
+            rt._check(h.tnt_buf_wrap(rt._ctx, C.c_void_p(t_.data_ptr()), t_.numel(), N.C64, C.byref(b)))
+            bufs.append(b)
+        calls.append(((N.GemmJob * len(js))(*js), len(js), bufs, views, td))
+    torch.cuda.synchronize()
+
+    def one(c):
+        arr, nj, (bd, bl, br), _, _ = c
+        rt._check(h.tnt_gemm_batch_complex(rt._ctx, bd, bl, br, arr, nj, None, 0, 0, 0, 1.0, 0.0, 0.0, 0.0, prec))
+
+    def vendor():
+        for _, _, _, views, _ in calls:
+            for a, b, cm in views:
+                torch.mm(b, a, out=cm)
+
+    def timed(fn, sync):
+        for _ in range(args.warmup):
+            fn()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        sync()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_vendor = timed(vendor, torch.cuda.synchronize)
+    ref = [c[4].clone() for c in calls]
+    for c in calls:
+        c[4].zero_()
+    torch.cuda.synchronize()  # torch's stream before the backend's
+    # tile shape per contraction picked by measurement, as the autotuner does
+    t_ours, cfgs = 0.0, []
+    for c in calls:
+        best = None
+        for cfg in (-1, 0, 1, 2, 3, 5, 6):
+            h.tnt_set_tile_config(rt._ctx, cfg)
+            t = timed(lambda: one(c), rt.synchronize)
+            if best is None or t < best[0]:
+                best = (t, cfg)
+        t_ours += best[0]
+        cfgs.append(GEMM_KERNELS[best[1]].format("double") if best[1] >= 0 else "size rule")
+    h.tnt_set_tile_config(rt._ctx, -1)
+    err = max(float(torch.linalg.norm(c[4] - r_) / torch.linalg.norm(r_)) for c, r_ in zip(calls, ref))
+    log(f"[bench] {len(shapes)} C64 sector GEMMs, {flops / 1e9:.2f} GFLOP (8mkn): tnt_gemm_batch_complex "
+        f"({precision}) {t_ours * 1e3:.3f} ms ({flops / t_ours / 1e12:.1f} TFLOP/s; tiles {cfgs}), torch.mm per "
+        f"job {t_vendor * 1e3:.3f} ms ({flops / t_vendor / 1e12:.1f} TFLOP/s); rel Frobenius diff {err:.1e}")
+    rec = {"metric": "C64 sector GEMM TFLOP/s (8mkn; tnt_gemm_batch_complex vs vendor torch.mm)",
+           "value": flops / t_ours / 1e12, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": t_ours * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c64", "data": "synthetic (torch complex normal operands)",
+           "config": {"workload": "u1_env_core_gemms_c64", "bond_dim": args.D, "jobs": len(shapes),
+                      "precision": precision, "tiles": cfgs},
+           "vendor": {"route": "torch.mm per job (complex128, hipBLASLt / rocBLAS)",
+                      "tflops": flops / t_vendor / 1e12, "ms_per_step": t_vendor * 1e3},
+           "rel_frobenius_diff_vs_vendor": err}
+    print(json.dumps(rec), flush=True)
+    for c in calls:
+        for b in c[2]:
+            h.tnt_buf_free(rt._ctx, b)
+    rt.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -619,7 +727,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="u1_env",
                     choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
-                             "gemm_vs_vendor", "microbench"])
+                             "gemm_vs_vendor", "complex_gemm", "microbench"])
     ap.add_argument("--min-ms", type=float, default=200.0, help="microbench: timed milliseconds per cell")
     ap.add_argument("--count", type=int, default=256, help="batched: number of contractions")
     ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16", "split_i8"],
@@ -664,6 +772,9 @@ def main():
         return
     if args.workload == "gemm_vs_vendor":
         run_gemm_vs_vendor(args)
+        return
+    if args.workload == "complex_gemm":
+        run_complex_gemm(args)
         return
     if args.workload == "microbench":
         run_microbench(args)

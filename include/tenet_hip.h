@@ -71,16 +71,17 @@ enum tnt_status {
 /* Payload element types: DenseRead / DenseWrite::{F64, F32, C32, C64}
  * (tenet-dense/src/executor.rs:228-285).  FP64 payloads are accepted by every
  * entry point (the reference's device subset is FP64,
- * tenet-dense/src/cuda_adapter.rs:108-140).  FP32 payloads are accepted by
- * tnt_buf_alloc / _wrap / _upload / _download / _zero and by tnt_gemm_batch
- * (precision TNT_PREC_F64 or TNT_PREC_F32; the result is rounded to FP32 once);
- * every other entry point returns TNT_E_UNSUPPORTED for them.  Complex
- * payloads (TNT_C32, TNT_C64) are rejected with TNT_E_UNSUPPORTED by every
- * entry point, tnt_buf_alloc and tnt_buf_wrap included. */
+ * tenet-dense/src/cuda_adapter.rs:108-140).  FP32 and complex payloads
+ * (TNT_C32: two FP32, TNT_C64: two FP64 per element, real part first) are
+ * accepted by tnt_buf_alloc / _wrap / _upload / _download / _zero / _info /
+ * _free and by tnt_gemm_batch / tnt_gemm_batch_complex; lengths and offsets
+ * count elements of the payload type.  Every other entry point returns
+ * TNT_E_UNSUPPORTED for them. */
 enum tnt_dtype { TNT_F64 = 0, TNT_F32 = 1, TNT_C32 = 2, TNT_C64 = 3 };
 
 /* MatrixOp {Identity, Transpose, Adjoint} (tenet-dense/src/executor.rs:24-31),
- * same order.  On real payloads Transpose and Adjoint are the same operation. */
+ * same order.  On real payloads Transpose and Adjoint are the same operation;
+ * on complex payloads Adjoint is the conjugate transpose. */
 enum tnt_op { TNT_OP_IDENTITY = 0, TNT_OP_TRANSPOSE = 1, TNT_OP_ADJOINT = 2 };
 
 /* Payload arithmetic of the block GEMM. */
@@ -212,17 +213,31 @@ int tnt_buf_fill_random(tnt_ctx* ctx, tnt_buf* buf, uint64_t seed, int normal);
 int tnt_buf_info(const tnt_buf* buf, void** device_ptr, size_t* len, int* dtype, int* device);
 
 /* ---- batched block GEMM ---------------------------------------------------- */
-/* runs: optional strided-batch run partition (executor.rs:473-488); it is a
+/* DenseExecutor::matmul_batch_axpby_with_ops_into (tenet-dense/src/executor.rs:292-312;
+ * matmul_batch_axpby_into, :217-287, is lhs_op = rhs_op = Identity).
+ * runs: optional strided-batch run partition (executor.rs:473-488); it is a
  * scheduling hint only and may be NULL.  lhs_op / rhs_op are tnt_op values
  * (MatrixOp); any other value is TNT_E_INVALID.  dst, lhs and rhs must share
- * one payload type: three FP64 payloads (every precision) or three FP32
- * payloads (TNT_PREC_F64: FP64 arithmetic, result rounded to FP32 once;
- * TNT_PREC_F32: FP32 MFMA); mixed types are TNT_E_INVALID, split precisions
- * on FP32 payloads and complex payloads TNT_E_UNSUPPORTED.  Every check runs
- * before anything is queued. */
+ * one payload type (mixed types are TNT_E_INVALID):
+ *   F64 / C64: every precision;
+ *   F32 / C32: TNT_PREC_F64 (FP64 arithmetic, result rounded to FP32 once) or
+ *              TNT_PREC_F32 (FP32 MFMA); split precisions TNT_E_UNSUPPORTED.
+ * Complex jobs run as real m x 2k x 2n block GEMMs on FP64 planes of the
+ * operands ([Re A | Im A] . [[Re B, Im B], [-Im B, Re B]]) in a grow-only
+ * workspace of the context (2mk + 4kn + 2mn doubles over the batch), with
+ * contracted and column counts up to 2^30 - 1.  Every check runs before
+ * anything is queued.  tnt_gemm_batch passes real alpha and beta. */
 int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
                    const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
                    int lhs_op, int rhs_op, double alpha, double beta, int precision);
+/* The same with DenseScalar::{C32, C64} alpha and beta (tenet-dense/src/scalar.rs:5-11):
+ * D = alpha op(A) op(B) + beta D in complex arithmetic, D not read when
+ * beta = 0.  On real payloads alpha_im and beta_im must be 0 (else
+ * TNT_E_INVALID). */
+int tnt_gemm_batch_complex(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                           const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
+                           int lhs_op, int rhs_op, double alpha_re, double alpha_im, double beta_re,
+                           double beta_im, int precision);
 /* Grouped GEMM over explicit pointers: all jobs in one persistent launch. */
 int tnt_gemm_grouped(tnt_ctx* ctx, const tnt_gemm_ptr_job* jobs, size_t n_jobs, double alpha,
                      double beta, int precision);

@@ -102,6 +102,8 @@ HIP_API = {
     "tnt_buf_info": (C.c_int, [VP, PP, SZP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tnt_gemm_batch": (C.c_int, [VP, VP, VP, VP, C.POINTER(GemmJob), SZ, SZP, SZ, C.c_int,
                                  C.c_int, C.c_double, C.c_double, C.c_int]),
+    "tnt_gemm_batch_complex": (C.c_int, [VP, VP, VP, VP, C.POINTER(GemmJob), SZ, SZP, SZ, C.c_int,
+                                         C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "tnt_gemm_grouped": (C.c_int, [VP, C.POINTER(GemmPtrJob), SZ, C.c_double, C.c_double, C.c_int]),
     "tnt_plan_upload": (C.c_int, [VP, C.POINTER(TransformDesc), PP]),
     "tnt_plan_destroy": (C.c_int, [VP, VP]),

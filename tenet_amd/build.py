@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOST_SRC = ["csrc/host/structure.cpp", "csrc/host/snapshot.cpp", "csrc/host/host_capi.cpp",
             "csrc/common/narrow_fusion.cpp", "csrc/common/batch_desc.cpp"]
 HIP_SRC = ["csrc/hip/backend.hip", "csrc/hip/xform.hip", "csrc/hip/gemm.hip",
-           "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip", "csrc/common/narrow_fusion.cpp"]
+           "csrc/hip/gemm_split.hip", "csrc/hip/reduce.hip", "csrc/hip/complex.hip",
+           "csrc/common/narrow_fusion.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              # no packed FP32 instructions anywhere in device code.  hipcc has
              # no device-only form of a cc1 target feature (-Xarch_device does

@@ -90,6 +90,9 @@ struct tnt_ctx {
   // stream-ordered partials of tnt_weighted_dot_async
   void* dot_parts = nullptr;
   size_t dot_parts_len = 0;
+  // real planes of complex GEMM batches (grow-only, stream-ordered)
+  void* cplx_ws = nullptr;
+  size_t cplx_ws_bytes = 0;
   struct Timed {
     int kind;
     hipEvent_t a, b;
@@ -1002,12 +1005,12 @@ static bool valid_precision(int p) { return p >= TNT_PREC_F64 && p <= TNT_PREC_S
 
 // Payload type check of tnt_buf_alloc / tnt_buf_wrap.
 static int check_dtype(tnt_ctx* ctx, int dtype) {
-  if (dtype == TNT_F64 || dtype == TNT_F32) return TNT_OK;
-  if (dtype == TNT_C32 || dtype == TNT_C64) FAIL(ctx, TNT_E_UNSUPPORTED, "complex payloads (C32, C64) are not supported");
+  if (dtype == TNT_F64 || dtype == TNT_F32 || dtype == TNT_C32 || dtype == TNT_C64) return TNT_OK;
   FAIL(ctx, TNT_E_INVALID, "unknown payload type");
 }
 
-static size_t dtype_bytes(int dtype) { return dtype == TNT_F32 ? 4 : 8; }
+static bool is_complex(int dtype) { return dtype == TNT_C32 || dtype == TNT_C64; }
+static size_t dtype_bytes(int dtype) { return dtype == TNT_F32 ? 4 : dtype == TNT_C64 ? 16 : 8; }
 
 // The job of a DenseGemmBatchJob against whole flat payloads (ld = row count
 // of the stored operand).  Dimensions must fit the kernels' 32-bit indices.
@@ -1153,7 +1156,8 @@ static int check_buf(tnt_ctx* ctx, const tnt_buf* b, const char* what) {
   if (!b) FAIL(ctx, TNT_E_INVALID, std::string("null buffer: ") + what);
   if (b->device != ctx->device) FAIL(ctx, TNT_E_DEVICE_MISMATCH, std::string(what) + " lives on another device");
   if (b->dtype != TNT_F64)
-    FAIL(ctx, TNT_E_UNSUPPORTED, std::string(what) + " must be an FP64 payload (FP32 payloads: tnt_gemm_batch only)");
+    FAIL(ctx, TNT_E_UNSUPPORTED,
+         std::string(what) + " must be an FP64 payload (FP32 and complex payloads: tnt_gemm_batch only)");
   return TNT_OK;
 }
 
@@ -1190,8 +1194,9 @@ int tnt_buf_zero(tnt_ctx* ctx, tnt_buf* buf) {
   if (!ctx || !buf) return TNT_E_INVALID;
   if (int rc = check_any_buf(ctx, buf, "buf")) return rc;
   if (int rc = set_device(ctx)) return rc;
-  if (buf->dtype == TNT_F32) launch_zero_f32(static_cast<float*>(buf->ptr), (int64_t)buf->len, ctx->stream);
-  else launch_fill(static_cast<double*>(buf->ptr), (int64_t)buf->len, 0.0, ctx->stream);
+  if (buf->dtype == TNT_F32 || buf->dtype == TNT_C32)
+    launch_zero_f32(static_cast<float*>(buf->ptr), (int64_t)buf->len * (buf->dtype == TNT_C32 ? 2 : 1), ctx->stream);
+  else launch_fill(static_cast<double*>(buf->ptr), (int64_t)buf->len * (buf->dtype == TNT_C64 ? 2 : 1), 0.0, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
   return TNT_OK;
 }
@@ -1215,8 +1220,8 @@ int tnt_buf_info(const tnt_buf* buf, void** ptr, size_t* len, int* dtype, int* d
   return TNT_OK;
 }
 
-// Validation shared by tnt_gemm_batch and its allocation-free variant: every
-// check before anything is queued; fills `gj`.
+// Validation of tnt_gemm_batch / tnt_gemm_batch_complex: every check before
+// anything is queued; fills `gj` (element offsets of the payload type).
 static int gemm_batch_jobs(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
                            const tnt_gemm_job* jobs, size_t n_jobs, int lhs_op, int rhs_op, int precision,
                            std::vector<GJob>& gj) {
@@ -1228,8 +1233,8 @@ static int gemm_batch_jobs(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const
   if (n_jobs && !jobs) FAIL(ctx, TNT_E_INVALID, "null job list");
   if (!valid_op(lhs_op) || !valid_op(rhs_op)) FAIL(ctx, TNT_E_INVALID, "unknown matrix op");
   if (!valid_precision(precision)) FAIL(ctx, TNT_E_INVALID, "unknown precision");
-  if (dst->dtype == TNT_F32 && precision != TNT_PREC_F64 && precision != TNT_PREC_F32)
-    FAIL(ctx, TNT_E_UNSUPPORTED, "split precisions need FP64 payloads");
+  if ((dst->dtype == TNT_F32 || dst->dtype == TNT_C32) && precision != TNT_PREC_F64 && precision != TNT_PREC_F32)
+    FAIL(ctx, TNT_E_UNSUPPORTED, "split precisions need FP64 or C64 payloads");
   gj.resize(n_jobs);
   for (size_t i = 0; i < n_jobs; ++i) {
     if (int rc = make_seam_job(ctx, jobs[i], lhs_op, rhs_op, &gj[i])) return rc;
@@ -1239,14 +1244,148 @@ static int gemm_batch_jobs(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const
   return TNT_OK;
 }
 
+// Complex payloads, two forms (layouts in complex.hip).  C64 with op(B) =
+// identity and a real beta: R(alpha op(A)) planes in the context's workspace,
+// then one real 2m x 2k x n GEMM that reads B and writes D in their real
+// views (beta applied by the GEMM).  Otherwise each job becomes one real
+// m x 2k x 2n job on planes of both operands -- expand pass, real block GEMM
+// (alpha 1, beta 0), combine pass with the complex alpha and beta.
+static int gemm_batch_complex(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                              const std::vector<GJob>& seam, int lhs_op, int rhs_op, double ar, double ai,
+                              double br, double bi, int precision) {
+  const bool lhs_form = dst->dtype == TNT_C64 && rhs_op == TNT_OP_IDENTITY && bi == 0.0;
+  std::vector<CplxJob> cj;
+  std::vector<GJob> rj;
+  int64_t ws = 0;
+  uint64_t a_items = 0, b_items = 0, d_items = 0;
+  auto items = [](int64_t elems) { return (uint64_t)((elems + kCplxItem - 1) / kCplxItem); };
+  for (const GJob& g : seam) {
+    if (g.m == 0 || g.n == 0) continue;  // writes nothing
+    if (2 * (int64_t)g.m > INT32_MAX || 2 * (int64_t)g.k > INT32_MAX || 2 * (int64_t)g.n > INT32_MAX)
+      FAIL(ctx, TNT_E_UNSUPPORTED, "complex job: a dimension exceeds 2^30 - 1");
+    const int64_t mk = (int64_t)g.m * g.k, kn = (int64_t)g.k * g.n, mn = (int64_t)g.m * g.n;
+    if (lhs_form) {
+      CplxJob c{};
+      c.a_src = g.a_off;
+      c.m = g.m;
+      c.k = g.k;
+      c.n = g.n;
+      c.ops = ((g.ops & 1) ? CPLX_A_T : 0) | (lhs_op == TNT_OP_ADJOINT ? CPLX_A_CONJ : 0);
+      c.a_ws = ws;
+      ws += 4 * mk;
+      c.a_item = (uint32_t)a_items;
+      a_items += items(mk);
+      if (a_items > UINT32_MAX) FAIL(ctx, TNT_E_UNSUPPORTED, "complex batch too large for one pass");
+      cj.push_back(c);
+      GJob r{};
+      r.a_off = c.a_ws;          // workspace
+      r.b_off = 2 * g.b_off;     // rhs payload, real view
+      r.d_off = 2 * g.d_off;     // dst payload, real view
+      r.m = 2 * g.m;
+      r.k = 2 * g.k;
+      r.n = g.n;
+      r.lda = (g.ops & 1) ? std::max(r.k, 1) : std::max(r.m, 1);
+      r.ldb = std::max(r.k, 1);
+      r.ldd = std::max(r.m, 1);
+      r.ops = g.ops & 1;
+      rj.push_back(r);
+      continue;
+    }
+    CplxJob c{};
+    c.a_src = g.a_off;
+    c.b_src = g.b_off;
+    c.d_dst = g.d_off;
+    c.m = g.m;
+    c.k = g.k;
+    c.n = g.n;
+    c.ops = ((g.ops & 1) ? CPLX_A_T : 0) | (lhs_op == TNT_OP_ADJOINT ? CPLX_A_CONJ : 0) |
+            ((g.ops & 2) ? CPLX_B_T : 0) | (rhs_op == TNT_OP_ADJOINT ? CPLX_B_CONJ : 0);
+    c.a_ws = ws;
+    c.b_ws = c.a_ws + 2 * mk;
+    c.c_ws = c.b_ws + 4 * kn;
+    ws = c.c_ws + 2 * mn;
+    c.a_item = (uint32_t)a_items;
+    c.b_item = (uint32_t)b_items;
+    c.d_item = (uint32_t)d_items;
+    a_items += items(mk);
+    b_items += items(kn);
+    d_items += items(mn);
+    if (a_items + b_items > UINT32_MAX || d_items > UINT32_MAX)
+      FAIL(ctx, TNT_E_UNSUPPORTED, "complex batch too large for one pass");
+    cj.push_back(c);
+    GJob r{};
+    r.a_off = c.a_ws;
+    r.b_off = c.b_ws;
+    r.d_off = c.c_ws;
+    r.m = g.m;
+    r.k = 2 * g.k;
+    r.n = 2 * g.n;
+    r.lda = (g.ops & 1) ? std::max(r.k, 1) : std::max(r.m, 1);
+    r.ldb = (g.ops & 2) ? std::max(r.n, 1) : std::max(r.k, 1);
+    r.ldd = std::max(r.m, 1);
+    r.ops = g.ops;
+    rj.push_back(r);
+  }
+  if (cj.empty()) return TNT_OK;
+  if (int rc = set_device(ctx)) return rc;
+  begin_call(ctx);
+  const size_t bytes = 8 * (size_t)ws;
+  if (ctx->cplx_ws_bytes < bytes) {
+    if (ctx->capturing)
+      FAIL(ctx, TNT_E_INVALID, "complex GEMM workspace must grow during graph capture; run the call once before capturing");
+    if (ctx->cplx_ws) retire(ctx, ctx->cplx_ws);
+    ctx->cplx_ws = nullptr;
+    ctx->cplx_ws_bytes = 0;
+    if (int rc = pool_alloc(ctx, bytes, &ctx->cplx_ws)) return rc;
+    ctx->cplx_ws_bytes = bytes;
+  }
+  void* dcj = nullptr;
+  int rc = cached_table(ctx, cj.data(), sizeof(CplxJob) * cj.size(), "CJ", &dcj);
+  GemmTable t;
+  build_gemm_table(t, rj, ctx->knobs);
+  void* dj = nullptr;
+  void* dt = nullptr;
+  if (!rc) rc = cached_table(ctx, t.host_jobs.data(), sizeof(GJob) * std::max<size_t>(t.host_jobs.size(), 1), "J", &dj);
+  if (!rc) rc = cached_table(ctx, t.host_tiles.data(), sizeof(GTile) * std::max<size_t>(t.host_tiles.size(), 1), "T", &dt);
+  if (rc) return rc;
+  t.jobs = static_cast<GJob*>(dj);
+  t.tiles = static_cast<GTile*>(dt);
+  const bool c32 = dst->dtype == TNT_C32;
+  double* wsp = static_cast<double*>(ctx->cplx_ws);
+  const CplxJob* jobs = static_cast<const CplxJob*>(dcj);
+  if (lhs_form) {
+    launch_cplx_expand_lhs(jobs, (int)cj.size(), (uint32_t)a_items, lhs->ptr, wsp, ar, ai, ctx->stream);
+    if (int lrc = launch_status(ctx)) return lrc;
+    return run_gemm_table(ctx, t, (uintptr_t)dst->ptr, (uintptr_t)wsp, (uintptr_t)rhs->ptr, 1.0, br, precision);
+  }
+  launch_cplx_expand(jobs, (int)cj.size(), (uint32_t)a_items, (uint32_t)b_items, lhs->ptr, rhs->ptr, wsp, c32,
+                     ctx->stream);
+  if (int lrc = launch_status(ctx)) return lrc;
+  if ((rc = run_gemm_table(ctx, t, (uintptr_t)wsp, (uintptr_t)wsp, (uintptr_t)wsp, 1.0, 0.0, precision))) return rc;
+  launch_cplx_combine(jobs, (int)cj.size(), (uint32_t)d_items, wsp, dst->ptr, ar, ai, br, bi, c32, ctx->stream);
+  return launch_status(ctx);
+}
+
 int tnt_gemm_batch(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
                    const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
                    int lhs_op, int rhs_op, double alpha, double beta, int precision) {
+  return tnt_gemm_batch_complex(ctx, dst, lhs, rhs, jobs, n_jobs, runs, n_runs, lhs_op, rhs_op, alpha, 0.0, beta,
+                                0.0, precision);
+}
+
+int tnt_gemm_batch_complex(tnt_ctx* ctx, tnt_buf* dst, const tnt_buf* lhs, const tnt_buf* rhs,
+                           const tnt_gemm_job* jobs, size_t n_jobs, const size_t* runs, size_t n_runs,
+                           int lhs_op, int rhs_op, double alpha_re, double alpha_im, double beta_re,
+                           double beta_im, int precision) {
   (void)runs;
   (void)n_runs;
   if (!ctx) return TNT_E_INVALID;
   std::vector<GJob> gj;
   if (int rc = gemm_batch_jobs(ctx, dst, lhs, rhs, jobs, n_jobs, lhs_op, rhs_op, precision, gj)) return rc;
+  if (is_complex(dst->dtype))
+    return gemm_batch_complex(ctx, dst, lhs, rhs, gj, lhs_op, rhs_op, alpha_re, alpha_im, beta_re, beta_im, precision);
+  if (alpha_im != 0.0 || beta_im != 0.0) FAIL(ctx, TNT_E_INVALID, "complex alpha or beta on real payloads");
+  const double alpha = alpha_re, beta = beta_re;
   if (int rc = set_device(ctx)) return rc;
   begin_call(ctx);
   GemmTable t;

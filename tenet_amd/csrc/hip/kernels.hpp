@@ -247,6 +247,27 @@ struct ScalarProgram {
 void launch_scalars(double* sc, const ScalarProgram& prog, hipStream_t s);
 void launch_fill_unit(double* y, int64_t n, uint64_t seed, hipStream_t s);
 void launch_fill_normal(double* y, int64_t n, uint64_t seed, hipStream_t s);
+
+// Complex payloads at the batched GEMM seam (complex.hip): one complex job as
+// a real m x 2k x 2n job on planes in a workspace, with an expand pass before
+// and a combine pass after the real GEMM.
+constexpr int kCplxItem = 1024;  // stored elements per work item of the element passes
+enum CplxOps : int32_t { CPLX_A_T = 1, CPLX_A_CONJ = 2, CPLX_B_T = 4, CPLX_B_CONJ = 8 };
+struct CplxJob {
+  int64_t a_src, b_src, d_dst;       // complex element offsets in lhs, rhs, dst
+  int64_t a_ws, b_ws, c_ws;          // double offsets of the job's planes in the workspace
+  int32_t m, k, n, ops;              // ops: CplxOps of the stored operands
+  uint32_t a_item, b_item, d_item;   // first work item of the job's A / B / D elements
+  uint32_t pad;
+};
+void launch_cplx_expand(const CplxJob* jobs, int n_jobs, uint32_t a_items, uint32_t b_items, const void* lhs,
+                        const void* rhs, double* ws, bool c32, hipStream_t s);
+void launch_cplx_combine(const CplxJob* jobs, int n_jobs, uint32_t d_items, const double* ws, void* dst,
+                         double ar, double ai, double br, double bi, bool c32, hipStream_t s);
+// The left-expanded form (C64, op(B) = identity, real beta): R(alpha op(A))
+// planes only; the real GEMM reads B and writes D in their real views.
+void launch_cplx_expand_lhs(const CplxJob* jobs, int n_jobs, uint32_t a_items, const void* lhs, double* ws,
+                            double ar, double ai, hipStream_t s);
 void launch_scale_regions(double* y, const int64_t* off, const int64_t* len, uint32_t n,
                           double beta, hipStream_t s);
 
