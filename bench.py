@@ -96,6 +96,23 @@ def roofline(precision, dom, workload=None):
     return rec
 
 
+def settle(step, sync, ms):
+    """Run `step` back to back for `ms` milliseconds of wall time (the queue
+    is drained every few steps so it never runs far ahead of the clock);
+    returns the number of steps run."""
+    n = 0
+    if ms > 0:
+        sync()
+        t = time.perf_counter()
+        while (time.perf_counter() - t) * 1e3 < ms:
+            for _ in range(4):
+                step()
+            n += 4
+            sync()
+    sync()
+    return n
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -725,6 +742,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle-ms", type=float, default=100.0,
+                    help="back-to-back steps before each timed region so the GPU clock settles (0: none)")
     ap.add_argument("--workload", default="u1_env",
                     choices=["u1_env", "su2_env", "su2_fmove", "batched", "serve", "copy_calibration",
                              "gemm_vs_vendor", "complex_gemm", "microbench"])
@@ -884,6 +903,7 @@ def main():
     tuned = {"batched": batched_cfg} if batched_cfg is not None else None
     if args.autotune == "on" and args.workload != "batched":
         from tenet_amd.autotune import autotune
+        settle(wl.run, rt.synchronize, args.settle_ms / 2)
         conts = list(wl.contractions) if hasattr(wl, "contractions") else [wl.contraction]
         tuned = autotune(rt, conts, wl.run, reps=max(2, min(5, args.steps // 4)), log=log)
     # Timed region 1: K steps as a caller runs them -> value, ms_per_step.  The
@@ -896,7 +916,12 @@ def main():
         step = wl.replay
         for _ in range(2):
             step()
-    rt.synchronize()
+    # The GPU's clock follows its load over tens of milliseconds: after the
+    # host-side gaps of set-up and tuning, short steps would otherwise be timed
+    # while the clock is still ramping (measured: ~2.0 GHz in the first
+    # milliseconds of load, 2.38 GHz once settled).  Steps run back to back
+    # for --settle-ms first, as they do inside a sweep.
+    settle_steps = settle(step, rt.synchronize, args.settle_ms)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -905,6 +930,7 @@ def main():
     # Timed region 2: the same K steps with every kernel class bracketed by HIP
     # events on the backend stream -> roofline / recoupling (the events widen the
     # gaps between kernels, so this region's wall time is reported beside only).
+    settle(wl.run, rt.synchronize, args.settle_ms / 4)
     rt._check(h.tnt_ctx_reset_stats(rt._ctx))
     rt._check(h.tnt_ctx_set_timing(rt._ctx, 1))
     rt.synchronize()
@@ -1047,6 +1073,8 @@ def main():
         "gemm_all_launches": {"tflops": gemm_tflops, "launches": gemm_launches,
                               "ms_per_step": gemm_ms / args.steps},
         "ms_per_step_with_kernel_events": ms_events,
+        "clock_settle": {"ms": args.settle_ms, "steps": settle_steps,
+                         "note": "steps run back to back before the timed region (not timed, not counted in warmup)"},
         "launch": "hipGraph replay of the step" if use_graph else "eager launches",
         "recoupling": {"bound": "hbm", "achieved": xf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": xf_gbs / HBM_PEAK_GBS, "launches": xf_launches,

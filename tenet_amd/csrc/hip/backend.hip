@@ -191,10 +191,17 @@ struct GemmTable {
   GTile* tiles = nullptr;
   uint32_t n_tiles_cfg[TILE_CFG_COUNT] = {};
   uint32_t tile_start_cfg[TILE_CFG_COUNT] = {};
-  // the last n_narrow_cfg[c] tiles of each shape belong to narrow jobs
+  // tiles of narrow jobs among each shape's tiles (see part_narrow)
   uint32_t n_narrow_cfg[TILE_CFG_COUNT] = {};
   double flops_cfg[TILE_CFG_COUNT] = {};         // algorithmic flops per shape
   double flops_narrow_cfg[TILE_CFG_COUNT] = {};  // ... of its narrow jobs
+  // Within a shape the tiles run in parts of one op pair (GJob::ops), each
+  // part its jobs' wide tiles, then its narrow ones: part o of shape c is
+  // tiles [part_start[c][o], + part_n[c][o]), the last part_narrow[c][o] of
+  // them narrow.
+  uint32_t part_start[TILE_CFG_COUNT][4] = {};
+  uint32_t part_n[TILE_CFG_COUNT][4] = {};
+  uint32_t part_narrow[TILE_CFG_COUNT][4] = {};
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
   std::vector<GJob> split_jobs;  // host_jobs with the narrow jobs emptied
@@ -876,13 +883,23 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const 
       t.flops_cfg[c] += f;
       if (narrow_job(j)) t.flops_narrow_cfg[c] += f;
     }
-    // narrow jobs' tiles last (stable: larger jobs still lead)
-    auto mid = std::stable_partition(per[c].begin(), per[c].end(),
-                                     [&](const GTile& g) { return !narrow_job(jobs[g.job]); });
-    t.n_narrow_cfg[c] = (uint32_t)(per[c].end() - mid);
+    // parts by op pair; in each, narrow jobs' tiles last (stable: larger
+    // jobs still lead)
     t.tile_start_cfg[c] = (uint32_t)t.host_tiles.size();
     t.n_tiles_cfg[c] = (uint32_t)per[c].size();
-    t.host_tiles.insert(t.host_tiles.end(), per[c].begin(), per[c].end());
+    t.n_narrow_cfg[c] = 0;
+    for (int o = 0; o < 4; ++o) {
+      std::vector<GTile> part;
+      for (const GTile& g : per[c])
+        if ((jobs[g.job].ops & 3) == o) part.push_back(g);
+      auto mid = std::stable_partition(part.begin(), part.end(),
+                                       [&](const GTile& g) { return !narrow_job(jobs[g.job]); });
+      t.part_start[c][o] = (uint32_t)t.host_tiles.size();
+      t.part_n[c][o] = (uint32_t)part.size();
+      t.part_narrow[c][o] = (uint32_t)(part.end() - mid);
+      t.n_narrow_cfg[c] += t.part_narrow[c][o];
+      t.host_tiles.insert(t.host_tiles.end(), part.begin(), part.end());
+    }
   }
   t.split_jobs = jobs;
   for (auto& j : t.split_jobs)
@@ -909,20 +926,26 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
   if (storage == TNT_F32) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
       if (!t.n_tiles_cfg[c]) continue;
-      GemmArgs g{t.jobs, t.tiles + t.tile_start_cfg[c], d, a, b, alpha, beta};
       TimedScope tk(ctx, timing_kind_kernel(precision, c), ctx->timing ? t.flops_cfg[c] : 0.0);
-      if (precision == TNT_PREC_F32) launch_gemm_f32_on_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
-      else launch_gemm_f64_on_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
+      for (int o = 0; o < 4; ++o) {
+        if (!t.part_n[c][o]) continue;
+        GemmArgs g{t.jobs, t.tiles + t.part_start[c][o], d, a, b, alpha, beta};
+        if (precision == TNT_PREC_F32) launch_gemm_f32_on_f32(g, c, o, t.part_n[c][o], ctx->stream);
+        else launch_gemm_f64_on_f32(g, c, o, t.part_n[c][o], ctx->stream);
+      }
     }
     return launch_status(ctx);
   }
   if (precision == TNT_PREC_SPLIT_BF16 || precision == TNT_PREC_SPLIT_I8) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
       if (!t.n_narrow_cfg[c]) continue;
-      const uint32_t first = t.tile_start_cfg[c] + t.n_tiles_cfg[c] - t.n_narrow_cfg[c];
-      GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
       TimedScope tk(ctx, timing_kind_kernel(TNT_PREC_F64, c), ctx->timing ? t.flops_narrow_cfg[c] : 0.0);
-      launch_gemm_f64(g, c, t.n_narrow_cfg[c], ctx->stream);
+      for (int o = 0; o < 4; ++o) {
+        if (!t.part_narrow[c][o]) continue;
+        const uint32_t first = t.part_start[c][o] + t.part_n[c][o] - t.part_narrow[c][o];
+        GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
+        launch_gemm_f64(g, c, o, t.part_narrow[c][o], ctx->stream);
+      }
     }
     if (!ctx->split) ctx->split.reset(split_workspace_create());
     const int scheme = precision == TNT_PREC_SPLIT_I8 ? SPLIT_SCHEME_I8 : SPLIT_SCHEME_BF16;
@@ -933,10 +956,13 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
   }
   for (int c = 0; c < TILE_CFG_COUNT; ++c) {
     if (!t.n_tiles_cfg[c]) continue;
-    GemmArgs g{t.jobs, t.tiles + t.tile_start_cfg[c], d, a, b, alpha, beta};
     TimedScope tk(ctx, timing_kind_kernel(precision, c), ctx->timing ? t.flops_cfg[c] : 0.0);
-    if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, t.n_tiles_cfg[c], ctx->stream);
-    else launch_gemm_f64(g, c, t.n_tiles_cfg[c], ctx->stream);
+    for (int o = 0; o < 4; ++o) {
+      if (!t.part_n[c][o]) continue;
+      GemmArgs g{t.jobs, t.tiles + t.part_start[c][o], d, a, b, alpha, beta};
+      if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, o, t.part_n[c][o], ctx->stream);
+      else launch_gemm_f64(g, c, o, t.part_n[c][o], ctx->stream);
+    }
   }
   return launch_status(ctx);
 }

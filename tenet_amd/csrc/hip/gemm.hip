@@ -61,7 +61,24 @@ constexpr int BK = 16;
 // Register budget: 4-wave tiles at most 256 VGPRs+AGPRs per lane, so two
 // waves share each SIMD (one wave per SIMD leaves LDS and global latency
 // exposed); 8-wave tiles at most 128, four waves per SIMD.
-template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false, typename S = double>
+//
+// Every global load of the main loop is unconditional, so the loop has no
+// branch but its trip test and the compiler keeps each K slab's loads in
+// flight across the slab's MFMAs (predicated edge loads had made it wait for
+// the A slab right after issuing it):
+//  * a tile that would cross the bottom (right) edge of a job with m >= BM
+//    (n >= BN) is moved back inside the job and stores only the rows
+//    (columns) from its own start on, which no other tile stores; the rows it
+//    recomputes are the padding an edge tile computes anyway;
+//  * in a job smaller than the tile, line indices are clamped to the last
+//    line (clamped lines only feed outputs that are never stored);
+//  * the K remainder is the FIRST slab, loaded in the prologue with the
+//    elements past it zeroed, so every slab of the loop is a full BK.
+// The adjoint flags are template parameters (a launch covers tiles of one
+// op pair), and the LDS fragments of k-step ks+1 are read while the MFMAs of
+// k-step ks issue.
+template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false, typename S = double,
+          bool AADJ = false, bool BADJ = false>
 __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 ? 4 : 2))) void gemm_kernel(
     GemmArgs g) {
   constexpr int NTH = 64 * GM * GN;
@@ -76,8 +93,9 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   const GTile tile = g.tiles[blockIdx.x];
   const GJob job = g.jobs[tile.job];
   const int m = job.m, n = job.n, k = job.k;
-  const int m0 = tile.tm * BM, n0 = tile.tn * BN;
-  const bool a_adj = job.ops & 1, b_adj = job.ops & 2;
+  const int m0o = tile.tm * BM, n0o = tile.tn * BN;  // first row / column this tile stores
+  const int m0 = m >= BM ? min(m0o, m - BM) : 0;     // tile origin, inside the job when it fits
+  const int n0 = n >= BN ? min(n0o, n - BN) : 0;
   const gptr<const S> A = gmem<const S>(g.abase) + job.a_off;
   const gptr<const S> B = gmem<const S>(g.bbase) + job.b_off;
   const gptr<S> D = gmem<S>(g.dbase) + job.d_off;
@@ -97,108 +115,133 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   // ---- loader geometry (computed once) ----
   // A identity: contiguous along i (C = BM, O = BK); adjoint: along k (C = BK, O = BM)
   // B identity: contiguous along k (C = BK, O = BN); adjoint: along j (C = BN, O = BK)
-  const int a_c = a_adj ? tid % BK : tid % BM;
-  const int a_o = a_adj ? tid / BK : tid / BM;
-  const int a_step = a_adj ? NTH / BK : NTH / BM;
-  const int b_c = b_adj ? tid % BN : tid % BK;
-  const int b_o = b_adj ? tid / BN : tid / BK;
-  const int b_step = b_adj ? NTH / BN : NTH / BK;
-  // element (row i, k kk) of op(A): identity A[i + kk*lda], adjoint A[kk + i*lda]
-  gptr<const S> pa = a_adj ? A + (int64_t)a_c + (int64_t)(m0 + a_o) * lda : A + (int64_t)(m0 + a_c) + (int64_t)a_o * lda;
-  const int64_t a_rstride = (int64_t)a_step * lda;  // between consecutive loads r
-  const int64_t a_kadv = a_adj ? BK : (int64_t)BK * lda;
-  gptr<const S> pb = b_adj ? B + (int64_t)(n0 + b_c) + (int64_t)b_o * ldb : B + (int64_t)b_c + (int64_t)(n0 + b_o) * ldb;
-  const int64_t b_rstride = (int64_t)b_step * ldb;
-  const int64_t b_kadv = b_adj ? (int64_t)BK * ldb : BK;
-  // LDS destinations: As[buf][kk][i], Bs[buf][kk][j]
-  // A identity: (kk = a_o + a_step*r, i = a_c); adjoint: (kk = a_c, i = a_o + a_step*r)
-  // B identity: (kk = b_c, j = b_o + b_step*r);  adjoint: (kk = b_o + b_step*r, j = b_c)
-  const bool m_full = m0 + BM <= m, n_full = n0 + BN <= n;
+  // Load r of a thread reads element (line, kk) of the slab: A identity
+  // (i = a_c, kk = a_o + a_step r), adjoint (i = a_o + a_step r, kk = a_c);
+  // B identity (j = b_o + b_step r, kk = b_c), adjoint (j = b_c, kk = b_o + b_step r).
+  constexpr int a_step = AADJ ? NTH / BK : NTH / BM;
+  constexpr int b_step = BADJ ? NTH / BN : NTH / BK;
+  const int a_c = AADJ ? tid % BK : tid % BM;
+  const int a_o = AADJ ? tid / BK : tid / BM;
+  const int b_c = BADJ ? tid % BN : tid % BK;
+  const int b_o = BADJ ? tid / BN : tid / BK;
+  // element pointers at the load's own kk of slab 0: one per load where the
+  // line changes with r (adjoint A, identity B), else one per thread
+  constexpr int NPA = AADJ ? A_LOADS : 1, NPB = BADJ ? 1 : B_LOADS;
+  gptr<const S> pa[NPA], pb[NPB];
+  if constexpr (AADJ) {
+#pragma unroll
+    for (int r = 0; r < A_LOADS; ++r) pa[r] = A + (int64_t)min(m0 + a_o + a_step * r, m - 1) * lda + a_c;
+  } else {
+    pa[0] = A + min(m0 + a_c, m - 1) + (int64_t)a_o * lda;
+  }
+  if constexpr (BADJ) {
+    pb[0] = B + min(n0 + b_c, n - 1) + (int64_t)b_o * ldb;
+  } else {
+#pragma unroll
+    for (int r = 0; r < B_LOADS; ++r) pb[r] = B + (int64_t)min(n0 + b_o + b_step * r, n - 1) * ldb + b_c;
+  }
+  auto aptr = [&](int r) { return AADJ ? pa[r] : pa[0] + (int64_t)r * a_step * lda; };
+  auto bptr = [&](int r) { return BADJ ? pb[0] + (int64_t)r * b_step * ldb : pb[r]; };
+  auto akk = [&](int r) { return AADJ ? a_c : a_o + a_step * r; };
+  auto bkk = [&](int r) { return BADJ ? b_o + b_step * r : b_c; };
+  // element offset of k index kk from k index 0 of the same line
+  auto aoff = [&](int kk) { return AADJ ? (int64_t)kk : (int64_t)kk * lda; };
+  auto boff = [&](int kk) { return BADJ ? (int64_t)kk * ldb : (int64_t)kk; };
 
   S ra[A_LOADS], rb[B_LOADS];
-  auto load = [&](int k0) {
-    const bool k_full = k0 + BK <= k;
-    if (m_full && k_full) {
-#pragma unroll
-      for (int r = 0; r < A_LOADS; ++r) ra[r] = pa[r * a_rstride];
-    } else {
-#pragma unroll
-      for (int r = 0; r < A_LOADS; ++r) {
-        const int i = a_adj ? a_o + a_step * r : a_c;
-        const int kk = a_adj ? a_c : a_o + a_step * r;
-        const bool ok = (m0 + i < m) && (k0 + kk < k);
-        ra[r] = ok ? pa[r * a_rstride] : S(0);
-      }
-    }
-    if (n_full && k_full) {
-#pragma unroll
-      for (int r = 0; r < B_LOADS; ++r) rb[r] = pb[r * b_rstride];
-    } else {
-#pragma unroll
-      for (int r = 0; r < B_LOADS; ++r) {
-        const int j = b_adj ? b_c : b_o + b_step * r;
-        const int kk = b_adj ? b_o + b_step * r : b_c;
-        const bool ok = (n0 + j < n) && (k0 + kk < k);
-        rb[r] = ok ? pb[r * b_rstride] : S(0);
-      }
-    }
-    pa += a_kadv;
-    pb += b_kadv;
-  };
   auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < A_LOADS; ++r) {
-      if (!a_adj) As[buf][a_o + a_step * r][a_c] = (T)ra[r];
+      if (!AADJ) As[buf][akk(r)][a_c] = (T)ra[r];
       else As[buf][a_c][a_o + a_step * r] = (T)ra[r];
     }
 #pragma unroll
     for (int r = 0; r < B_LOADS; ++r) {
-      if (!b_adj) Bs[buf][b_c][b_o + b_step * r] = (T)rb[r];
-      else Bs[buf][b_o + b_step * r][b_c] = (T)rb[r];
+      if (!BADJ) Bs[buf][b_c][b_o + b_step * r] = (T)rb[r];
+      else Bs[buf][bkk(r)][b_c] = (T)rb[r];
     }
   };
+  T fa[2][MT], fb[2][NT];
+  auto frags = [&](int buf, int ks, int s) {
+    const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+    for (int i = 0; i < MT; ++i) fa[s][i] = As[buf][kk][wm + i * 16 + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) fb[s][j] = Bs[buf][kk][wn + j * 16 + (lane & 15)];
+  };
+  auto mfmas = [&](int s) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(fa[s][i], fb[s][j], acc[i][j]);
+  };
 
-  // Double-buffered LDS: slab t is read from buffer t&1 while slab t+1 is
-  // fetched into registers and then written to the other buffer; one
-  // barrier per slab.
-  const int nk = (k + BK - 1) / BK;
+  // Prologue: slab 0 holds k in [0, kf), kf = the K remainder (1..BK); its
+  // elements at kk >= kf are zero (their addresses clamped to kf - 1).
+  // Nothing contracted (k == 0): no slab, no load; the epilogue gives beta D.
+  const int kf = k > 0 ? k - (k - 1) / BK * BK : 0;
+  const int nk = k > 0 ? (k - kf) / BK + 1 : 0;
   if (nk > 0) {
-    load(0);
+#pragma unroll
+    for (int r = 0; r < A_LOADS; ++r) {
+      const int kk = akk(r);
+      const S v = aptr(r)[aoff(min(kk, kf - 1) - kk)];
+      ra[r] = kk < kf ? v : S(0);
+    }
+#pragma unroll
+    for (int r = 0; r < B_LOADS; ++r) {
+      const int kk = bkk(r);
+      const S v = bptr(r)[boff(min(kk, kf - 1) - kk)];
+      rb[r] = kk < kf ? v : S(0);
+    }
     store(0);
   }
   __syncthreads();
-  // A wave whose part of the tile lies wholly outside the job (in a right or
-  // bottom edge tile) skips its LDS reads and MFMAs -- the waves of the other
-  // workgroups on its SIMD get the matrix core -- but keeps loading its share
-  // of the slabs and meets every barrier; its epilogue stores nothing.
-  const bool live = __builtin_amdgcn_readfirstlane((m0 + wm < m) && (n0 + wn < n));
+  // Slab t >= 1 covers k in [kf + (t - 1) BK, kf + t BK).
+  int64_t ak = aoff(kf), bk = boff(kf);
+  const int64_t a_kadv = aoff(BK), b_kadv = boff(BK);
+  // A wave whose part of the tile stores nothing (wholly outside the job, or
+  // inside the rows / columns a moved edge tile only recomputes) skips its
+  // LDS reads and MFMAs -- the waves of the other workgroups on its SIMD get
+  // the matrix core -- but keeps loading its share of the slabs and meets
+  // every barrier.
+  const bool live = __builtin_amdgcn_readfirstlane((m0 + wm + WM > m0o) && (m0 + wm < m) &&
+                                                   (n0 + wn + WN > n0o) && (n0 + wn < n));
+  if (live && nk > 0) frags(0, 0, 0);
+  // Double-buffered LDS: slab t is read from buffer t&1 while slab t+1 is
+  // fetched into registers and then written to the other buffer; one
+  // barrier per slab.
   for (int t = 0; t < nk; ++t) {
     const int cur = t & 1;
-    if (t + 1 < nk) load((t + 1) * BK);
+    if (t + 1 < nk) {
+#pragma unroll
+      for (int r = 0; r < A_LOADS; ++r) ra[r] = aptr(r)[ak];
+#pragma unroll
+      for (int r = 0; r < B_LOADS; ++r) rb[r] = bptr(r)[bk];
+      ak += a_kadv;
+      bk += b_kadv;
+    }
     if (live) {
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
-        const int kk = ks * 4 + (lane >> 4);
-        T af[MT], bf[NT];
-#pragma unroll
-        for (int i = 0; i < MT; ++i) af[i] = As[cur][kk][wm + i * 16 + (lane & 15)];
-#pragma unroll
-        for (int j = 0; j < NT; ++j) bf[j] = Bs[cur][kk][wn + j * 16 + (lane & 15)];
-#pragma unroll
-        for (int i = 0; i < MT; ++i)
-#pragma unroll
-          for (int j = 0; j < NT; ++j) acc[i][j] = MfmaOp<T>::run(af[i], bf[j], acc[i][j]);
+        if (ks + 1 < BK / 4) frags(cur, ks + 1, (ks + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfmas(ks & 1);
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     if (t + 1 < nk) store(cur ^ 1);
     __syncthreads();
+    if (live && t + 1 < nk) frags(cur ^ 1, 0, 0);
   }
+  if (!live) return;
 
   // Epilogue: D = alpha * acc (+ beta * D in the BETA instance, launched for
-  // beta != 0).  There the D values of one row of fragments are read
-  // unconditionally (indices clamped into the job) before any is stored, so
-  // the wave waits once per fragment row rather than once per element; the
-  // beta == 0 instance keeps its registers for the main loop.
+  // beta != 0) over rows [m0o, m) and columns [n0o, n) of the tile.  There
+  // the D values of one row of fragments are read unconditionally (indices
+  // clamped into the job) before any is stored, so the wave waits once per
+  // fragment row rather than once per element; the beta == 0 instance keeps
+  // its registers for the main loop.
   const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
@@ -215,11 +258,11 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       const int col = n0 + wn + j * 16 + (lane & 15);
-      if (col >= n) continue;
+      if (col < n0o || col >= n) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
-        if (row < m) {
+        if (row >= m0o && row < m) {
           const double v = alpha * (double)acc[i][j][r];
           if constexpr (BETA) D[row + (int64_t)col * ldd] = (S)(v + beta * old[j][r]);
           else D[row + (int64_t)col * ldd] = (S)v;
@@ -591,29 +634,41 @@ void launch_gemm_tiny(const GemmArgs& g, const TinyArgs& t0, uint32_t n_tiles, h
   }
 }
 
-template <typename T, bool BETA, typename S>
+template <typename T, bool BETA, typename S, bool AADJ, bool BADJ>
 static void launch_tiles(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
   switch (cfg) {
     case TILE_128x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 2, BETA, S, AADJ, BADJ>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_128x64:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 64, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 64, 2, 2, BETA, S, AADJ, BADJ>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_64x128:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 128, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 128, 2, 2, BETA, S, AADJ, BADJ>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
     case TILE_128x128_W8:
-      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4, BETA, S>), dim3(n_tiles), dim3(512), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 128, 128, 2, 4, BETA, S, AADJ, BADJ>), dim3(n_tiles), dim3(512), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 2, 2, BETA, S>), dim3(n_tiles), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 2, 2, BETA, S, AADJ, BADJ>), dim3(n_tiles), dim3(256), 0, s, a);
       break;
   }
 }
 
+template <typename T, bool BETA, typename S>
+static void launch_tiles_ops(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
+  switch (ops & 3) {
+    case 0: launch_tiles<T, BETA, S, false, false>(a, cfg, n_tiles, s); break;
+    case 1: launch_tiles<T, BETA, S, true, false>(a, cfg, n_tiles, s); break;
+    case 2: launch_tiles<T, BETA, S, false, true>(a, cfg, n_tiles, s); break;
+    default: launch_tiles<T, BETA, S, true, true>(a, cfg, n_tiles, s); break;
+  }
+}
+
+// ops: the op pair (GJob::ops) shared by every tile of the launch; the
+// narrow-output and one-wave kernels read each job's own ops.
 template <typename T, typename S>
-static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
+static void launch_cfg(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
   if (n_tiles == 0) return;
   switch (cfg) {
     case TILE_SKINNY:
@@ -626,26 +681,26 @@ static void launch_cfg(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t
         hipLaunchKernelGGL((gemm_small_kernel<T, false, S>), dim3((n_tiles + 3) / 4), dim3(256), 0, s, a, n_tiles);
       break;
     default:
-      if (a.beta != 0.0) launch_tiles<T, true, S>(a, cfg, n_tiles, s);
-      else launch_tiles<T, false, S>(a, cfg, n_tiles, s);
+      if (a.beta != 0.0) launch_tiles_ops<T, true, S>(a, cfg, ops, n_tiles, s);
+      else launch_tiles_ops<T, false, S>(a, cfg, ops, n_tiles, s);
       break;
   }
 }
 
-void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<double, double>(a, cfg, n_tiles, s);
+void launch_gemm_f64(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<double, double>(a, cfg, ops, n_tiles, s);
 }
 
-void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<float, double>(a, cfg, n_tiles, s);
+void launch_gemm_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<float, double>(a, cfg, ops, n_tiles, s);
 }
 
-void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<double, float>(a, cfg, n_tiles, s);
+void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<double, float>(a, cfg, ops, n_tiles, s);
 }
 
-void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s) {
-  launch_cfg<float, float>(a, cfg, n_tiles, s);
+void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s) {
+  launch_cfg<float, float>(a, cfg, ops, n_tiles, s);
 }
 
 }  // namespace tnt_dev

@@ -196,12 +196,13 @@ __host__ __device__ inline int tile_cols(int cfg) {
 }
 
 // Launch one grouped GEMM over the tile table.  All tiles of one launch must
-// share `cfg`; the caller splits the table by configuration.
-void launch_gemm_f64(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
-void launch_gemm_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+// share `cfg` and the op pair `ops` (GJob::ops); the caller splits the table
+// by both.
+void launch_gemm_f64(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s);
+void launch_gemm_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s);
 // The same kernels over FP32 payloads (FP64 or FP32 arithmetic).
-void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
-void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, uint32_t n_tiles, hipStream_t s);
+void launch_gemm_f64_on_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s);
+void launch_gemm_f32_on_f32(const GemmArgs& a, int cfg, int ops, uint32_t n_tiles, hipStream_t s);
 
 // Split-BF16 (Ozaki) GEMM: operands are pre-split into `slices` BF16 planes
 // with per-row (lhs) / per-column (rhs) power-of-two scales.
