@@ -169,7 +169,7 @@ def cpu_port_baseline(workload, wl, budget_s=30.0):
     return times[len(times) // 2], wl.flops, P.threads(), sample, len(times), out
 
 
-def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap", min_runs=3):
+def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap", min_runs=3, su2_count="dimension"):
     """Time the oracle (numpy FP64, host cores) on the same workload.  With the
     device workload `wl` given, the oracle's operands are its downloaded
     payloads, so the parity check compares results of identical inputs."""
@@ -187,7 +187,7 @@ def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap", min_runs=3):
     if workload in ("u1_env", "su2_env"):
         rname = "U1" if workload == "u1_env" else "SU2"
         orule = make_rule(rname)
-        _, V, P, M = mps_env_spaces(rname, D)
+        _, V, P, M = mps_env_spaces(rname, D, su2_count)
 
         def leg(g):
             return OL(list(zip(g.ids, g.degs)), g.is_dual)
@@ -203,7 +203,7 @@ def cpu_baseline(workload, D, budget_s=20.0, wl=None, fmove="swap", min_runs=3):
             t1 = L.contract([2], A, [0], [0, 1, 2, 3], 2)
             t2 = t1.contract([1, 2], W, [3, 1], [0, 2, 3, 1], 2)
             return Ad.contract([2, 0], t2, [0, 2], [0, 1, 2], 2)
-        sample = f"full {rname} environment update at D={D}"
+        sample = f"full {rname} environment update at D={D}" + (" multiplets" if su2_count == "multiplets" else "")
     elif workload == "batched":
         from tenet_amd.workloads import batched_specs
         orule = make_rule("U1xZ2")
@@ -559,10 +559,12 @@ def run_gemm_vs_vendor(args):
 # The other configurations, measured after the headline one in the default
 # run (each in a child process with its own time limit, so a failure there
 # leaves the headline record intact): the SU(2) environment update at
-# D = 4096, the SU(2) F-move (FP64 and split-INT8), the batched merges, the
+# D = 4096 (D counted as the leg dimension, and as multiplets), the SU(2)
+# F-move (FP64 and split-INT8), the batched merges, the
 # served H_eff trace (U(1) and SU(2)) and the reference's microbench.
 SECONDARY = (  # (name, arguments, time limit in seconds; each takes 5-25 s)
     ("su2_env_f64", ["--workload", "su2_env"], 90),
+    ("su2_env_multiplets_f64", ["--workload", "su2_env", "--su2-count", "multiplets"], 90),
     ("su2_fmove_f64", ["--workload", "su2_fmove", "--steps", "10", "--warmup", "2"], 150),
     ("su2_fmove_split_i8", ["--workload", "su2_fmove", "--precision", "split_i8", "--steps", "10",
                             "--warmup", "2", "--cpu-baseline", "off"], 90),
@@ -752,6 +754,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["f64", "f32", "split_bf16", "split_i8"],
                     help="GEMM precision (default: split_i8 for serve, f64 otherwise)")
     ap.add_argument("--D", type=int, default=4096)
+    ap.add_argument("--su2-count", default="dimension", choices=["dimension", "multiplets"],
+                    help="su2_env: D is the leg dimension, or the number of SU(2) multiplets")
     ap.add_argument("--mult", type=int, default=64, help="su2_fmove multiplicity scale")
     ap.add_argument("--fmove", default="swap", choices=["swap", "cross"],
                     help="su2_fmove form: codomain swap of a 3<-1 tensor then contract, or A[1,3].B[2,0]")
@@ -835,8 +839,12 @@ def main():
                "degeneracies": "gaussian sigma=4 (largest remainder)", "phys": "U1 {0,1}",
                "mpo_bond": "U1 {0:3,+1:1,-1:1}", "steps_per_update": "L.A, (.)W, Adag.(.)"}
     elif args.workload == "su2_env":
-        wl = MPSEnvUpdate(rt, "SU2", args.D, args.precision)
-        cfg = {"workload": "su2_mps_env_update", "bond_dim": args.D, "spins": "2j=0..20",
+        wl = MPSEnvUpdate(rt, "SU2", args.D, args.precision, su2_count=args.su2_count)
+        cfg = {"workload": "su2_mps_env_update" + ("_multiplets" if args.su2_count == "multiplets" else ""),
+               "bond_dim": args.D,
+               "bond_dim_counts": ("multiplets: sum_j m_j = D (leg dimension %d)" % wl.V.dim()
+                                   if args.su2_count == "multiplets" else "leg dimension: sum_j m_j (2j+1) = D"),
+               "spins": "2j=0..20", "multiplicities": "gaussian in j, sigma=3 (largest remainder)",
                "phys": "spin-1/2", "mpo_bond": "SU2 {0:2, 1:1}"}
     elif args.workload == "batched":
         import numpy as np
@@ -1023,7 +1031,8 @@ def main():
         # the parity checker; timed once beside the port
         t_cpu, cores, sample, out, reps = cpu_baseline(args.workload, args.D,
                                                        wl=wl if args.workload in ("u1_env", "su2_env") else None,
-                                                       fmove=args.fmove, budget_s=0.0, min_runs=1)
+                                                       fmove=args.fmove, budget_s=0.0, min_runs=1,
+                                                       su2_count=args.su2_count)
         if args.workload == "batched":
             nsample = len(out)
             flops_sample = sum(c.flops for *_, c in wl.pairs[:nsample])

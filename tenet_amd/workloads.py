@@ -8,7 +8,8 @@
   U(1): V charges -10..10 with Gaussian degeneracies (sigma 4, largest
   remainder), P = {0, 1}, M = {0:3, +1:1, -1:1} (Heisenberg S^z / S^+- channels).
   SU(2): V spins with multiplicities from the same Gaussian rule so that
-  sum_j m_j (2j+1) = D, P = spin 1/2, M = {0, 1, 0} (dimension 5).
+  sum_j m_j (2j+1) = D (or, counting multiplets, sum_j m_j = D), P = spin 1/2,
+  M = {0, 1, 0} (dimension 5).
 * ``su2_fmove`` (spins 2j = 0..8, multiplicity floor(64 / (2j+1))):
   - swap form: A in V (x) V (x) V <- V; A' = permute(A, (0, 2, 1), (3,))
     braids codomain legs 1 and 2, which recouples the inner line j01 (Multi
@@ -65,21 +66,30 @@ def u1_bond(D=4096, qmax=10, sigma=4.0):
     return gaussian_degeneracies(D, labels, sigma)
 
 
-def su2_bond(D=4096, twoj_max=20, sigma=3.0):
-    """Spins 0..twoj_max/2 (integer and half-integer); m_j Gaussian in j; sum m_j(2j+1) = D."""
+def su2_bond(D=4096, twoj_max=20, sigma=3.0, count="dimension"):
+    """Spins 0..twoj_max/2 (integer and half-integer); m_j Gaussian in j.
+
+    count="dimension": D is the dimension of the leg, sum_j m_j (2j+1) = D;
+    count="multiplets": D counts multiplets, sum_j m_j = D (the bond dimension
+    SU(2) DMRG codes quote; the leg's dimension is then several times D)."""
     labels = list(range(0, twoj_max + 1))  # doubled spins
-    deg = gaussian_degeneracies(D, [l / 2.0 for l in labels], sigma, weight=lambda j: int(2 * j + 1))
+    if count == "multiplets":
+        deg = gaussian_degeneracies(D, [l / 2.0 for l in labels], sigma)
+    elif count == "dimension":
+        deg = gaussian_degeneracies(D, [l / 2.0 for l in labels], sigma, weight=lambda j: int(2 * j + 1))
+    else:
+        raise ValueError(count)
     return {l: deg[l / 2.0] for l in labels}
 
 
-def mps_env_spaces(rule_name="U1", D=4096):
+def mps_env_spaces(rule_name="U1", D=4096, su2_count="dimension"):
     rule = Rule(rule_name)
     if rule_name == "U1":
         V = GradedSpace(rule, sorted(u1_bond(D).items()))
         P = GradedSpace(rule, [(0, 1), (1, 1)])
         M = GradedSpace(rule, [(0, 3), (1, 1), (-1, 1)])
     elif rule_name == "SU2":
-        V = GradedSpace(rule, sorted(su2_bond(D).items()))
+        V = GradedSpace(rule, sorted(su2_bond(D, count=su2_count).items()))
         P = GradedSpace(rule, [(1, 1)])
         M = GradedSpace(rule, [(0, 2), (2, 1)])
     else:
@@ -151,9 +161,9 @@ class GraphStep:
 class MPSEnvUpdate(GraphStep):
     """Left-environment update L' = A^dagger . ((L . A) . W) on one runtime."""
 
-    def __init__(self, rt, rule_name="U1", D=4096, precision="f64", seed=3):
+    def __init__(self, rt, rule_name="U1", D=4096, precision="f64", seed=3, su2_count="dimension"):
         self.rt = rt
-        self.rule, V, P, M = mps_env_spaces(rule_name, D)
+        self.rule, V, P, M = mps_env_spaces(rule_name, D, su2_count)
         self.V, self.P, self.M = V, P, M
         self.precision = precision
         self.L = TensorMap.rand(rt, [V, M], [V], seed=seed, normal=True)
