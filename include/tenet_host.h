@@ -113,9 +113,11 @@ double tnth_contract_flops(const tnth_contract* c);
  * per-sector GEMM, transform).  The backend plans the same route inside
  * tnt_contract_upload; this entry point exposes the plan for inspection and
  * CPU checks: a transform from the lhs payload to the output payload made of
- * Multi groups whose coefficients are U[t] = scale[t] * rhs[idx[t]]
- * (idx[t] < 0: U[t] = 0).  Returns TNT_E_UNSUPPORTED, with the failed
- * condition in tnth_last_error(), when the contraction is not eligible. */
+ * Multi groups whose coefficients are sums of rhs terms,
+ * U[t] = sum over ptr[t] <= p < ptr[t+1] of scale[p] * rhs[idx[p]]
+ * (tnth_fusion_gather returns the number of coefficients; ptr has one more
+ * entry).  Returns TNT_E_UNSUPPORTED, with the failed condition in
+ * tnth_last_error(), when the contraction is not eligible. */
 /* Stacked (batched) contraction: `copies` independent instances of the
  * contraction d in one descriptor, for one tnt_contract call that runs them
  * all with the launches of one (the serving loop's cross-request batching;
@@ -133,7 +135,7 @@ typedef struct tnth_fusion tnth_fusion;
 int tnth_narrow_fusion(const tnt_contract_desc* d, tnth_fusion** out);
 void tnth_fusion_destroy(tnth_fusion* f);
 const tnt_transform_desc* tnth_fusion_desc(const tnth_fusion* f);
-size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** idx, const double** scale);
+size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** ptr, const int64_t** idx, const double** scale);
 
 /* ---- snapshots: TeNeT's v1 semantic wire format ----------------------------
  * Replace TensorMap::to_bytes_with / from_bytes_with and GradedSpace's pair

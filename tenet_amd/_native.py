@@ -186,7 +186,7 @@ HOST_API = {
     "tnth_batch_destroy": (None, [VP]),
     "tnth_fusion_destroy": (None, [VP]),
     "tnth_fusion_desc": (C.POINTER(TransformDesc), [VP]),
-    "tnth_fusion_gather": (SZ, [VP, C.POINTER(I64P), C.POINTER(C.POINTER(C.c_double))]),
+    "tnth_fusion_gather": (SZ, [VP, C.POINTER(I64P), C.POINTER(I64P), C.POINTER(C.POINTER(C.c_double))]),
     "tnth_bytes_data": (C.POINTER(C.c_uint8), [VP]),
     "tnth_bytes_size": (C.c_size_t, [VP]),
     "tnth_bytes_free": (None, [VP]),

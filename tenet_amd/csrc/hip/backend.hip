@@ -230,9 +230,10 @@ struct tnt_artifact {
   // fused narrow route: one recoupling pass lhs -> dst with coefficients
   // gathered from the rhs payload (tnt_fuse::plan_narrow_fusion)
   tnt_plan* fused = nullptr;
+  int64_t* g_ptr = nullptr;  // coefficient t: gather terms [g_ptr[t], g_ptr[t + 1])
   int64_t* g_idx = nullptr;
   double* g_scale = nullptr;
-  uint32_t n_gather = 0;
+  uint32_t n_gather = 0;     // coefficients
   // gathered small-GEMM route of tiny contractions (FP64 calls): element maps
   // of the Single-only transforms and one tile list (32 x 32) over every job
   bool tiny = false;
@@ -1686,7 +1687,9 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
         a->fused = nullptr;  // not expressible as a plan: keep the general route
         ctx->err.clear();
       } else {
-        a->n_gather = (uint32_t)nf.gather_idx.size();
+        if (nf.gather_ptr.size() - 1 > UINT32_MAX) return fail((ctx->err = "fused plan has too many coefficients", TNT_E_UNSUPPORTED));
+        a->n_gather = (uint32_t)(nf.gather_ptr.size() - 1);
+        if (int rc = upload_vec(ctx, nf.gather_ptr, &a->g_ptr, a->allocs)) return fail(rc);
         if (int rc = upload_vec(ctx, nf.gather_idx, &a->g_idx, a->allocs)) return fail(rc);
         if (int rc = upload_vec(ctx, nf.gather_scale, &a->g_scale, a->allocs)) return fail(rc);
         *out = a;  // the fused route needs no scratch payloads
@@ -1810,7 +1813,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   const double* L = static_cast<const double*>(lhs->ptr);
   const double* R = static_cast<const double*>(rhs->ptr);
   if (a->fused) {
-    launch_coeff_gather(a->fused->d.coeffs, R, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
+    launch_coeff_gather(a->fused->d.coeffs, R, a->g_ptr, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
     run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
     return launch_status(ctx);
   }

@@ -117,10 +117,11 @@ constexpr double kSmallPlanBytes = 16.0 * (1 << 20);
 constexpr double kTinyMaxFlops = 16.0 * (1 << 20);
 constexpr int64_t kTinyMaxElems = int64_t(1) << 20;
 constexpr int kTinyKBlocks = 8;  // jobs with K <= 8 * this load all of K at once
-// coeffs[t] = idx[t] >= 0 ? scale[t] * src[idx[t]] : 0 (coefficients of a fused
-// narrow contraction gathered from its small operand).
-void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,
-                         uint32_t n, hipStream_t s);
+// coeffs[t] = sum over ptr[t] <= p < ptr[t + 1] of scale[p] * src[idx[p]]
+// (coefficients of a fused narrow contraction gathered from its small
+// operand; t < n).
+void launch_coeff_gather(double* coeffs, const double* src, const int64_t* ptr, const int64_t* idx,
+                         const double* scale, uint32_t n, hipStream_t s);
 
 // Device GEMM job: element offsets from the kernel's base pointers.
 struct GJob {

@@ -448,20 +448,21 @@ __global__ __launch_bounds__(256) void xform_any_kernel(XformArgs a, uint32_t n)
   }
 }
 
-__global__ __launch_bounds__(256) void coeff_gather_kernel(double* coeffs, const double* src,
-                                                          const int64_t* idx, const double* scale,
-                                                          uint32_t n) {
+__global__ __launch_bounds__(256) void coeff_gather_kernel(double* coeffs, const double* src, const int64_t* ptr,
+                                                          const int64_t* idx, const double* scale, uint32_t n) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t < n) {
-    const int64_t i = idx[t];
-    coeffs[t] = i >= 0 ? scale[t] * src[i] : 0.0;
+    const int64_t p0 = ptr[t], p1 = ptr[t + 1];
+    double u = p0 < p1 ? scale[p0] * src[idx[p0]] : 0.0;  // one term: the plain product
+    for (int64_t p = p0 + 1; p < p1; ++p) u = fma(scale[p], src[idx[p]], u);
+    coeffs[t] = u;
   }
 }
 
-void launch_coeff_gather(double* coeffs, const double* src, const int64_t* idx, const double* scale,
-                         uint32_t n, hipStream_t s) {
+void launch_coeff_gather(double* coeffs, const double* src, const int64_t* ptr, const int64_t* idx,
+                         const double* scale, uint32_t n, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(coeff_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, coeffs, src, idx, scale, n);
+  hipLaunchKernelGGL(coeff_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, coeffs, src, ptr, idx, scale, n);
 }
 
 void launch_xform_any(const XformArgs& a, uint32_t n_items, hipStream_t s) {

@@ -268,10 +268,11 @@ int tnth_narrow_fusion(const tnt_contract_desc* d, tnth_fusion** out) {
 }
 void tnth_fusion_destroy(tnth_fusion* f) { delete f; }
 const tnt_transform_desc* tnth_fusion_desc(const tnth_fusion* f) { return &f->f.desc; }
-size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** idx, const double** scale) {
+size_t tnth_fusion_gather(const tnth_fusion* f, const int64_t** ptr, const int64_t** idx, const double** scale) {
+  *ptr = f->f.gather_ptr.data();
   *idx = f->f.gather_idx.data();
   *scale = f->f.gather_scale.data();
-  return f->f.gather_idx.size();
+  return f->f.gather_ptr.empty() ? 0 : f->f.gather_ptr.size() - 1;
 }
 
 struct tnth_bytes {

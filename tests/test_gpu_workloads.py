@@ -186,6 +186,42 @@ def test_fused_mpo_step_equals_general_route(rt, prec):
         rt1.close()
 
 
+@pytest.mark.parametrize("count,D", [("dimension", 256), ("multiplets", 64)])
+def test_su2_fused_mpo_step_matches_general_route_and_oracle(rt, count, D):
+    """The SU(2) MPO step (lhs, rhs and output transforms with Multi blocks)
+    runs as one fused pass of Multi groups whose coefficients are sums of
+    gathered MPO terms: its T2 agrees with the general route
+    (TENET_AMD_FUSE_NARROW=0 at upload) to rounding, and the whole update
+    with the oracle."""
+    import os
+    from tenet_amd.tensor import Runtime
+    os.environ["TENET_AMD_TINY_FLOPS"] = "0"
+    rt2 = None
+    wl = wl2 = None
+    try:
+        wl = MPSEnvUpdate(rt, "SU2", D, "f64", su2_count=count)
+        assert wl.contractions[1].route == "fused_narrow"
+        out = wl.run().to_numpy().copy()
+        t2 = wl._tmp["T2"].to_numpy().copy()
+        os.environ["TENET_AMD_FUSE_NARROW"] = "0"
+        rt2 = Runtime(0)
+        wl2 = MPSEnvUpdate(rt2, "SU2", D, "f64", su2_count=count)
+        assert wl2.contractions[1].route == "transform_gemm"
+        assert rel(wl2._tmp["T2"].to_numpy(), t2) < 1e-14
+        assert rel(wl2.run().to_numpy(), out) < 1e-14
+        L, A, W = _o(wl.L), _o(wl.A), _o(wl.W)
+        t1 = L.contract([2], A, [0], [0, 1, 2, 3], 2)
+        t2o = t1.contract(MPSEnvUpdate.T2_AXES[0], W, *MPSEnvUpdate.T2_AXES[1:])
+        want = A.adjoint().contract(MPSEnvUpdate.LP_AXES[0], t2o, *MPSEnvUpdate.LP_AXES[1:])
+        assert rel(out, want.data) < 1e-13
+    finally:
+        os.environ.pop("TENET_AMD_FUSE_NARROW", None)
+        del os.environ["TENET_AMD_TINY_FLOPS"]
+        del wl, wl2
+        if rt2 is not None:
+            rt2.close()
+
+
 def test_fused_route_accumulates_and_reads_adjoint_mpo(rt):
     """Fused narrow route on the device: out = alpha X.op(W) + beta out, with W
     as stored and as a lazy adjoint read with op = adjoint, against the oracle."""

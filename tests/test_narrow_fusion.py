@@ -3,7 +3,8 @@
 A narrow contraction (every core job k <= 32, n <= 16) runs on the device as
 one recoupling pass lhs -> dst whose Multi coefficients are gathered from the
 rhs payload.  Checked on the CPU: the planned transform, replayed by the
-oracle's replay with U[t] = scale[t] * rhs[idx[t]], equals the general route
+oracle's replay with U[t] = sum of scale[p] * rhs[idx[p]] over the gather
+terms of coefficient t, equals the general route
 (lhs transform, per-job GEMM, output transform, replayed the same way) and the
 oracle contraction; ineligible contractions are refused with the failed
 condition named.
@@ -29,14 +30,23 @@ def fused(cdesc):
     return f, None
 
 
-def fused_replay(f, lhs, rhs):
+def gather_terms(f):
+    """(ptr, idx, scale) of the plan's coefficient gather: coefficient t is
+    the sum over ptr[t] <= p < ptr[t+1] of scale[p] * rhs[idx[p]]."""
     h = N.host()
-    idx, scale = N.I64P(), C.POINTER(C.c_double)()
-    n = h.tnth_fusion_gather(f, C.byref(idx), C.byref(scale))
-    ix = np.ctypeslib.as_array(idx, (n,)).copy() if n else np.zeros(0, np.int64)
-    sc = np.ctypeslib.as_array(scale, (n,)).copy() if n else np.zeros(0)
-    U = np.where(ix >= 0, sc * rhs[np.maximum(ix, 0)], 0.0)
-    return replay_desc(h.tnth_fusion_desc(f), lhs, coeffs=list(U)), ix, sc
+    ptr, idx, scale = N.I64P(), N.I64P(), C.POINTER(C.c_double)()
+    n = h.tnth_fusion_gather(f, C.byref(ptr), C.byref(idx), C.byref(scale))
+    pt = np.ctypeslib.as_array(ptr, (n + 1,)).copy() if n else np.zeros(1, np.int64)
+    nt = int(pt[-1])
+    ix = np.ctypeslib.as_array(idx, (nt,)).copy() if nt else np.zeros(0, np.int64)
+    sc = np.ctypeslib.as_array(scale, (nt,)).copy() if nt else np.zeros(0)
+    return pt, ix, sc
+
+
+def fused_replay(f, lhs, rhs):
+    pt, ix, sc = gather_terms(f)
+    U = [sum(sc[p] * rhs[ix[p]] for p in range(pt[t], pt[t + 1])) for t in range(len(pt) - 1)]
+    return replay_desc(N.host().tnth_fusion_desc(f), lhs, coeffs=U), ix, sc
 
 
 def env_spaces(rule, D):
@@ -103,23 +113,35 @@ def test_wide_contractions_keep_the_general_route():
     assert f is None and "not narrow" in why
 
 
-@pytest.mark.parametrize("D", [32, 64])
-def test_su2_mpo_step_fused_or_refused_correctly(D):
-    """SU(2) recouplings of the MPO step may need Multi blocks: then the
-    planner refuses; whatever it plans must equal the general route."""
-    r, L, A, W = env_spaces("SU2", D)
+@pytest.mark.parametrize("count,D", [("dimension", 32), ("dimension", 64), ("multiplets", 12)])
+def test_su2_mpo_step_fuses_and_matches(count, D):
+    """The SU(2) MPO step bends legs across the codomain/domain boundary, so
+    its lhs, rhs and output transforms have Multi blocks (2 x 2, mixing row
+    trees of both core jobs).  It fuses into groups that are connected
+    components of row trees under those blocks, with U entries that are sums
+    of rhs terms; the pass equals the general route and the oracle."""
+    r, V, P, M = mps_env_spaces("SU2", D, count)
+    L, A, W = HomSpace(r, [V, M], [V]), HomSpace(r, [V], [P, V]), HomSpace(r, [M, P], [P, M])
     T1 = _HostContraction(L, A, [2], [0], [0, 1, 2, 3], 2, False, False).dst_space
     c = _HostContraction(T1, W, [1, 2], [3, 1], [0, 3, 1, 2], 2, False, False)
+    d = c.desc.contents
+    assert any(d.lhs.contents.blocks[i].kind == 1 for i in range(d.lhs.contents.n_blocks))
+    assert any(d.out.contents.blocks[i].kind == 1 for i in range(d.out.contents.n_blocks))
     f, why = fused(c.desc)
-    if f is None:
-        assert why
-        return
+    assert f is not None, why
     try:
+        fd = N.host().tnth_fusion_desc(f).contents
+        assert max(fd.blocks[i].src_count for i in range(fd.n_blocks)) > 1 < max(
+            fd.blocks[i].dst_count for i in range(fd.n_blocks))
+        pt, _, _ = gather_terms(f)
+        assert np.any(np.diff(pt) > 1)  # some coefficients are sums of rhs terms
         rng = np.random.default_rng(3)
         lhs, rhs = rng.normal(size=T1.len), rng.normal(size=W.len)
         got, _, _ = fused_replay(f, lhs, rhs)
         want = contract_desc_replay(c.desc, lhs, rhs, c.desc.contents.core_len)
         assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+        ref = OTensor(ospace(T1), lhs).contract([1, 2], OTensor(ospace(W), rhs), [3, 1], [0, 3, 1, 2], 2)
+        assert np.max(np.abs(got - ref.data)) <= 1e-13 * np.max(np.abs(ref.data))
     finally:
         N.host().tnth_fusion_destroy(f)
 
