@@ -14,18 +14,6 @@ struct Axis {
   int64_t other;  // stride in the payload on the other side of the transform
 };
 
-struct Piece {
-  bool set = false;
-  int64_t offset = 0;
-  std::vector<int64_t> strides;  // one per row-box axis
-  double coeff = 1.0;
-};
-
-struct Group {
-  std::vector<std::pair<int64_t, int64_t>> box;  // row box: (extent, core stride), by core stride
-  std::vector<Piece> src, dst;                   // k pieces, n pieces
-};
-
 struct Job {
   int64_t d_off, a_off, b_off, m, k, n, ldb;
 };
@@ -250,11 +238,6 @@ bool plan_narrow_fusion(const tnt_contract_desc* d, NarrowFusion& f) {
       return -2;
     }
     return it->second;
-  };
-  auto strides_of = [](const std::vector<Axis>& rows) {
-    std::vector<int64_t> st;
-    for (auto& r : rows) st.push_back(r.other);
-    return st;
   };
   // the other-side layout of a block, with its axes split like the core layout's
   struct Split {
