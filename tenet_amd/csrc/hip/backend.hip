@@ -205,6 +205,10 @@ struct GemmTable {
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
   std::vector<GJob> split_jobs;  // host_jobs with the narrow jobs emptied
+  // narrow outputs (n <= 16) on 64 x 64 workgroup tiles instead of the
+  // narrow-output kernel (artifacts whose output transform is folded into
+  // the tile epilogue, where those jobs are a small share of the flops)
+  bool no_skinny = false;
 };
 
 // A job with a dimension of 16 or less is bound by HBM, not by the matrix
@@ -218,6 +222,10 @@ struct tnt_artifact {
   tnt_plan* rhs = nullptr;
   tnt_plan* out = nullptr;
   GemmTable gemm;
+  // output transform folded into the tile GEMM's epilogue (build_colmap):
+  // FP64 / FP32 calls on workgroup tiles write the destination directly
+  int64_t* cmap_start = nullptr;
+  ColMapEntry* cmap = nullptr;
   std::vector<void*> allocs;
   int64_t lhs_len = 0, rhs_len = 0, core_len = 0;
   double* scratch_l = nullptr;
@@ -847,8 +855,10 @@ static int job_cfg(const GJob& j, int family, const TileKnobs& kn) {
   return cfg;
 }
 
-static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const TileKnobs& kn, int forced = -1) {
+static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const TileKnobs& kn, int forced = -1,
+                             bool no_skinny = false) {
   t.host_jobs = jobs;
+  t.no_skinny = no_skinny;
   std::vector<std::vector<GTile>> per(TILE_CFG_COUNT);
   // larger jobs first for load balance
   std::vector<size_t> order(jobs.size());
@@ -867,7 +877,7 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const 
     // forced one-wave tiles also take narrow jobs of few rows (one launch for
     // a batch of tiny GEMMs instead of two)
     if (ovr >= 0 && (cfg != TILE_SKINNY || (ovr == TILE_SMALL && j.m <= 256))) cfg = ovr;
-    if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
+    if (cfg == TILE_SKINNY && (j.n > 16 || no_skinny)) cfg = TILE_64x64;
     // nothing contracted: D = beta D, written by the workgroup-tile epilogue
     // (that kernel loads no operand element when k == 0)
     if (j.k == 0) cfg = TILE_64x64;
@@ -926,7 +936,8 @@ static double table_flops(const GemmTable& t) {
 // storage: payload type of d, a and b (TNT_F64, or TNT_F32 for FP64 / FP32
 // arithmetic only; the caller has checked the combination).
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
-                          double alpha, double beta, int precision, int storage = TNT_F64) {
+                          double alpha, double beta, int precision, int storage = TNT_F64,
+                          const int64_t* cmap_start = nullptr, const ColMapEntry* cmap = nullptr) {
   TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
   if (storage == TNT_F32) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
@@ -964,7 +975,7 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
     TimedScope tk(ctx, timing_kind_kernel(precision, c), ctx->timing ? t.flops_cfg[c] : 0.0);
     for (int o = 0; o < 4; ++o) {
       if (!t.part_n[c][o]) continue;
-      GemmArgs g{t.jobs, t.tiles + t.part_start[c][o], d, a, b, alpha, beta};
+      GemmArgs g{t.jobs, t.tiles + t.part_start[c][o], d, a, b, alpha, beta, cmap_start, cmap};
       if (precision == TNT_PREC_F32) launch_gemm_f32(g, c, o, t.part_n[c][o], ctx->stream);
       else launch_gemm_f64(g, c, o, t.part_n[c][o], ctx->stream);
     }
@@ -1573,6 +1584,98 @@ static bool tiny_scatter_map(const tnt_transform_desc* d, std::vector<TinyEntry>
 // Decides and builds the tiny route of an uploaded artifact (after the general
 // route, which stays for the other precisions).  Returns non-zero only on an
 // allocation or copy failure.
+// The output transform folded into the tile GEMM's epilogue: eligible when
+// it is Single-only with no inactive layouts, the core has no uncovered
+// region, and every block maps whole core columns -- its core-side rows are
+// all rows of its job, contiguous (first row axis stride 1, each next axis
+// the product of the previous extents), and its destination-side row
+// strides follow the same nesting -- so that core element (row, col) goes to
+// dst[off(col) + row * rs(col)] scaled by the block coefficient, and the
+// blocks together write every destination element once.  Not eligible:
+// nothing is built (the contraction keeps transform-after-GEMM).
+// TENET_AMD_OUT_EPILOGUE=0 at upload keeps every contraction on the
+// separate output pass (A/B runs and tests).
+static bool out_epilogue_enabled() {
+  const char* e = std::getenv("TENET_AMD_OUT_EPILOGUE");
+  return !(e && e[0] == '0');
+}
+
+static int build_colmap(tnt_ctx* ctx, const tnt_contract_desc* d, const std::vector<GJob>& gj, tnt_artifact* a) {
+  const tnt_transform_desc* t = d->out;
+  if (!t || !out_epilogue_enabled() || t->n_inactive || d->n_uncovered) return TNT_OK;
+  std::vector<int64_t> start(gj.size() + 1, 0);
+  for (size_t j = 0; j < gj.size(); ++j) start[j + 1] = start[j] + std::max(gj[j].n, 0);
+  std::vector<ColMapEntry> map((size_t)start.back(), ColMapEntry{-1, 0, 0.0});
+  std::vector<std::pair<int64_t, size_t>> starts;  // core offset -> job
+  for (size_t j = 0; j < gj.size(); ++j)
+    if (gj[j].m > 0 && gj[j].n > 0) starts.push_back({gj[j].d_off, j});
+  std::sort(starts.begin(), starts.end());
+  int64_t written = 0;
+  for (size_t bi = 0; bi < t->n_blocks; ++bi) {
+    const tnt_transform_block& b = t->blocks[bi];
+    if (b.kind != TNT_BLOCK_SINGLE || b.dst_count != 1 || b.src_count != 1 || b.coeff >= t->n_coeffs ||
+        b.dst_layout >= t->n_layouts || b.src_layout >= t->n_layouts)
+      return TNT_OK;
+    const tnt_layout& Od = t->layouts[b.dst_layout];
+    const tnt_layout& Oc = t->layouts[b.src_layout];
+    if (Od.rank != Oc.rank || Oc.rank < 0 || Oc.rank > TNT_MAX_RANK) return TNT_OK;
+    int64_t cnt = 1, dhi = Od.offset;
+    for (int i = 0; i < Oc.rank; ++i) {
+      if (Od.shape[i] != Oc.shape[i] || Oc.shape[i] < 0 || Od.strides[i] < 0) return TNT_OK;
+      cnt *= Oc.shape[i];
+      if (Oc.shape[i] > 0) dhi += (Oc.shape[i] - 1) * Od.strides[i];
+    }
+    if (cnt == 0) continue;
+    if (Od.offset < 0 || dhi >= t->dst_len) return TNT_OK;
+    auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(Oc.offset, SIZE_MAX));
+    if (it == starts.begin()) return TNT_OK;
+    --it;
+    const GJob& J = gj[it->second];
+    const int64_t rel = Oc.offset - J.d_off, m = J.m;
+    if (rel < 0 || rel >= m * (int64_t)J.n || rel % m) return TNT_OK;  // must start at row 0
+    const int64_t c0 = rel / m;
+    std::vector<int> rows, cols;
+    for (int i = 0; i < Oc.rank; ++i) {
+      if (Oc.shape[i] == 1) continue;
+      if (Oc.strides[i] < m) rows.push_back(i);
+      else if (Oc.strides[i] % m) return TNT_OK;
+      else cols.push_back(i);
+    }
+    std::sort(rows.begin(), rows.end(), [&](int x, int y) { return Oc.strides[x] < Oc.strides[y]; });
+    int64_t prod = 1;
+    const int64_t rs = rows.empty() ? 0 : Od.strides[rows[0]];
+    for (int i : rows) {
+      if (Oc.strides[i] != prod || Od.strides[i] != rs * prod) return TNT_OK;
+      prod *= Oc.shape[i];
+    }
+    if (prod != m) return TNT_OK;  // the block covers every row of its job
+    const double c = t->coeffs[b.coeff];
+    std::vector<int64_t> idx(cols.size(), 0);
+    while (true) {
+      int64_t col = c0, off = Od.offset;
+      for (size_t u = 0; u < cols.size(); ++u) {
+        col += idx[u] * (Oc.strides[cols[u]] / m);
+        off += idx[u] * Od.strides[cols[u]];
+      }
+      if (col >= J.n) return TNT_OK;
+      ColMapEntry& e = map[(size_t)(start[it->second] + col)];
+      if (e.off >= 0) return TNT_OK;  // a core column mapped twice
+      e = ColMapEntry{off, rs, c};
+      size_t u = 0;
+      for (; u < cols.size(); ++u) {
+        if (++idx[u] < Oc.shape[cols[u]]) break;
+        idx[u] = 0;
+      }
+      if (u == cols.size()) break;
+    }
+    written += cnt;
+  }
+  if (written != t->dst_len) return TNT_OK;  // some destination element would be left unwritten
+  start.pop_back();
+  if (int rc = upload_vec(ctx, start, &a->cmap_start, a->allocs)) return rc;
+  return upload_vec(ctx, map, &a->cmap, a->allocs);
+}
+
 static int build_tiny_route(tnt_ctx* ctx, const tnt_contract_desc* d, const std::vector<GJob>& gj,
                             tnt_artifact* a) {
   const double max_flops = tiny_max_flops();
@@ -1663,7 +1766,18 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     if (int rc = build_plan(ctx, d->out, &a->out)) return fail(rc);
     if (a->out->src_len != d->core_len) return fail((ctx->err = "output transform length mismatch", TNT_E_SHAPE));
   }
-  build_gemm_table(a->gemm, gj, ctx->knobs);
+  if (int rc = build_colmap(ctx, d, gj, a)) return fail(rc);
+  bool no_skinny = false;
+  if (a->cmap) {  // narrow jobs to workgroup tiles when they are a small share of the work
+    double all = 0.0, narrow = 0.0;
+    for (const GJob& j : gj) {
+      const double f = 2.0 * double(std::max(j.m, 0)) * double(std::max(j.n, 0)) * double(std::max(j.k, 0));
+      all += f;
+      if (j.n <= 16) narrow += f;
+    }
+    no_skinny = narrow <= 0.02 * all;
+  }
+  build_gemm_table(a->gemm, gj, ctx->knobs, -1, no_skinny);
   if (int rc = upload_gemm_table(ctx, a->gemm, a->allocs)) return fail(rc);
   std::vector<int64_t> uo, ul;
   for (size_t i = 0; i < d->n_uncovered; ++i) {
@@ -1726,7 +1840,7 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* a, int cfg) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<GJob> jobs = a->gemm.host_jobs;
   GemmTable t;
-  build_gemm_table(t, jobs, ctx->knobs, cfg);
+  build_gemm_table(t, jobs, ctx->knobs, cfg, a->gemm.no_skinny);
   std::vector<void*> allocs;
   if (int rc = upload_gemm_table(ctx, t, allocs)) {
     for (auto p : allocs) pool_free(ctx, p);
@@ -1848,6 +1962,16 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
       launch_shadow_regions(a->unc_off, a->unc_len, a->n_unc, cnt, ctx->stream);
     });
   };
+  // the output transform folded into the epilogue: FP64 / FP32 tables on
+  // workgroup tiles only (the narrow-output and one-wave kernels write the
+  // core layout), not under the shadow check (it counts core writes)
+  if (a->out && a->cmap && (precision == TNT_PREC_F64 || precision == TNT_PREC_F32) && !ctx->shadow &&
+      !a->gemm.n_tiles_cfg[TILE_SKINNY] && !a->gemm.n_tiles_cfg[TILE_SMALL]) {
+    rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision, TNT_F64,
+                        a->cmap_start, a->cmap);
+    if (rc) return rc;
+    return launch_status(ctx);
+  }
   if (a->out) {
     launch_scale_regions(a->scratch_c, a->unc_off, a->unc_len, a->n_unc, 0.0, ctx->stream);
     rc = run_gemm_table(ctx, a->gemm, (uintptr_t)a->scratch_c, (uintptr_t)L, (uintptr_t)R, 1.0, 0.0, precision);

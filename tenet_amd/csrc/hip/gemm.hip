@@ -235,6 +235,38 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
     if (live && t + 1 < nk) frags(cur ^ 1, 0, 0);
   }
   if (!live) return;
+  const double alpha = g.alpha, beta = g.beta;
+  if (g.cmap) {
+    // Output transform folded in: core column col -> dst[off + row * rs],
+    // scaled by c, in the same operation order as the recoupling pass
+    // ((alpha c) x + beta y), so the result is that of transform-after-GEMM.
+    const ColMapEntry* __restrict__ cm = g.cmap + g.cmap_start[tile.job];
+    const gptr<S> D0 = gmem<S>(g.dbase);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int col = n0 + wn + j * 16 + (lane & 15);
+      if (col < n0o || col >= n) continue;
+      const ColMapEntry ce = cm[col];
+      if (ce.off < 0) continue;
+      const double ac = alpha * ce.c;
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + i * 16 + MfmaOp<T>::row(lane, r);
+          if (row >= m0o && row < m) {
+            const gptr<S> p = D0 + ce.off + (int64_t)row * ce.rs;
+            if constexpr (BETA) {
+              const double w = (double)*p;
+              *p = (S)(ac * (double)acc[i][j][r] + beta * w);
+            } else {
+              *p = (S)(ac * (double)acc[i][j][r]);
+            }
+          }
+        }
+    }
+    return;
+  }
 
   // Epilogue: D = alpha * acc (+ beta * D in the BETA instance, launched for
   // beta != 0) over rows [m0o, m) and columns [n0o, n) of the tile.  There
@@ -242,7 +274,6 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   // clamped into the job) before any is stored, so the wave waits once per
   // fragment row rather than once per element; the beta == 0 instance keeps
   // its registers for the main loop.
-  const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
     double old[BETA ? NT : 1][4];

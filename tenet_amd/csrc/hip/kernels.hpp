@@ -139,11 +139,25 @@ struct GTile {
   int32_t cfg; // tile-shape configuration of this tile's job
 };
 
+// Output transform folded into the workgroup-tile epilogue (Single-only
+// output transforms whose blocks each map whole core columns): core column
+// `col` of a job goes to dst[off + row * rs] scaled by c; off < 0: the
+// column is not part of the output.
+struct ColMapEntry {
+  int64_t off;
+  int64_t rs;
+  double c;
+};
+
 struct GemmArgs {
   const GJob* jobs;
   const GTile* tiles;
   uintptr_t dbase, abase, bbase;  // byte addresses; job offsets are in elements
   double alpha, beta;
+  // column-mapped destination (gemm_kernel only; null: D at dbase + d_off in
+  // the job's column-major layout): job j's columns at cmap + cmap_start[j]
+  const int64_t* cmap_start = nullptr;
+  const ColMapEntry* cmap = nullptr;
 };
 
 // Gathered small-GEMM route of tiny contractions (one launch per call): the

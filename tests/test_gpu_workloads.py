@@ -222,6 +222,56 @@ def test_su2_fused_mpo_step_matches_general_route_and_oracle(rt, count, D):
             rt2.close()
 
 
+@pytest.mark.parametrize("rule_name,D,count", [("U1", 512, "dimension"), ("SU2", 512, "dimension"),
+                                              ("SU2", 96, "multiplets")])
+def test_output_transform_in_gemm_epilogue_equals_separate_pass(rt, rule_name, D, count):
+    """A Single-only output transform that maps whole core columns (the last
+    contraction of the environment update) is folded into the tile GEMM's
+    epilogue for FP64 / FP32 calls: the same result as GEMM-then-transform
+    (TENET_AMD_OUT_EPILOGUE=0 at upload) to rounding -- such an artifact also
+    runs its few narrow-output jobs on workgroup tiles, which sum in another
+    order than the narrow-output kernel -- also accumulating with alpha and
+    beta, and the one-wave tiles (which keep the separate pass) agree."""
+    import os
+    from tenet_amd import _native as N
+    os.environ["TENET_AMD_TINY_FLOPS"] = "0"
+    rt2 = None
+    wl = wl2 = None
+    try:
+        wl = MPSEnvUpdate(rt, rule_name, D, "f64", su2_count=count)
+        os.environ["TENET_AMD_OUT_EPILOGUE"] = "0"
+        rt2 = Runtime(0)
+        wl2 = MPSEnvUpdate(rt2, rule_name, D, "f64", seed=3, su2_count=count)
+        for prec in ("f64", "f32"):
+            wl.precision = wl2.precision = prec
+            a = wl.run().to_numpy().copy()
+            b = wl2.run().to_numpy().copy()
+            assert rel(a, b) < (1e-14 if prec == "f64" else 1e-6), prec
+        from tenet_amd.tensor import TensorMap
+        c3 = wl.contractions[2]
+        wl.precision = wl2.precision = "f64"
+        prod = wl.run().to_numpy().copy()  # T2 of this run is the FP64 one
+        assert rel(wl2.run().to_numpy(), prod) < 1e-14
+        X = TensorMap.rand(rt, wl.out.space, seed=77, normal=True)
+        before = X.to_numpy().copy()
+        wl.A.adjoint().contract(MPSEnvUpdate.LP_AXES[0], wl._tmp["T2"], *MPSEnvUpdate.LP_AXES[1:], out=X,
+                                alpha=2.0, beta=-0.5)
+        assert rel(X.to_numpy(), 2.0 * prod - 0.5 * before) < 1e-14
+        # forced one-wave tiles keep the separate output pass: same result
+        h = N.hip()
+        rt._check(h.tnt_contract_set_tile_config(rt._ctx, c3.art, 6))
+        wl.precision = "f64"
+        small = wl.run().to_numpy()
+        rt._check(h.tnt_contract_set_tile_config(rt._ctx, c3.art, -1))
+        assert rel(small, prod) < 1e-14
+    finally:
+        os.environ.pop("TENET_AMD_OUT_EPILOGUE", None)
+        del os.environ["TENET_AMD_TINY_FLOPS"]
+        del wl, wl2
+        if rt2 is not None:
+            rt2.close()
+
+
 def test_fused_route_accumulates_and_reads_adjoint_mpo(rt):
     """Fused narrow route on the device: out = alpha X.op(W) + beta out, with W
     as stored and as a lazy adjoint read with op = adjoint, against the oracle."""
