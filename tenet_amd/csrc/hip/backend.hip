@@ -879,8 +879,9 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const 
     if (ovr >= 0 && (cfg != TILE_SKINNY || (ovr == TILE_SMALL && j.m <= 256))) cfg = ovr;
     if (cfg == TILE_SKINNY && (j.n > 16 || no_skinny)) cfg = TILE_64x64;
     // nothing contracted: D = beta D, written by the workgroup-tile epilogue
-    // (that kernel loads no operand element when k == 0)
-    if (j.k == 0) cfg = TILE_64x64;
+    // (that kernel loads no operand element when k == 0), on the table's
+    // workgroup shape so that such jobs join its launch
+    if (j.k == 0) cfg = ovr >= 0 && ovr != TILE_SKINNY && ovr != TILE_SMALL ? ovr : family;
     int bm = tile_rows(cfg), bn = tile_cols(cfg);
     int tm = (j.m + bm - 1) / bm, tn = (j.n + bn - 1) / bn;
     for (int a = 0; a < tm; ++a)
@@ -1075,6 +1076,37 @@ static int make_seam_job(tnt_ctx* ctx, const tnt_gemm_job& j, int lhs_op, int rh
   g.ops = (la ? 1 : 0) | (rb ? 2 : 0);
   *out = g;
   return TNT_OK;
+}
+
+// Core regions that no job writes as GEMM jobs with nothing contracted (the
+// workgroup-tile epilogue gives dst = beta dst): region [off, off + len) is
+// a 64-row column-major block of len / 64 columns (at most 2^30 columns per
+// job) plus one column of len % 64 rows.  ops (the contraction's op pair)
+// keeps them in the launch of its other tiles; no operand element is read.
+static void region_jobs(const tnt_region* r, size_t n, int32_t ops, std::vector<GJob>& out) {
+  constexpr int64_t kRows = 64, kMaxCols = int64_t(1) << 30;
+  auto push = [&](int64_t off, int64_t m, int64_t cols) {
+    GJob g{};
+    g.d_off = off;
+    g.m = (int32_t)m;
+    g.n = (int32_t)cols;
+    g.k = 0;
+    g.lda = (int32_t)m;
+    g.ldb = (int32_t)cols;
+    g.ldd = (int32_t)m;
+    g.ops = ops;
+    out.push_back(g);
+  };
+  for (size_t i = 0; i < n; ++i) {
+    int64_t off = r[i].offset, cols = r[i].len / kRows;
+    while (cols > 0) {
+      const int64_t c = std::min(cols, kMaxCols);
+      push(off, kRows, c);
+      off += kRows * c;
+      cols -= c;
+    }
+    if (r[i].len % kRows) push(off, r[i].len % kRows, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1600,6 +1632,13 @@ static bool out_epilogue_enabled() {
   return !(e && e[0] == '0');
 }
 
+// Uncovered core regions as GEMM jobs with nothing contracted unless
+// TENET_AMD_REGION_JOBS=0 (then a region pass of their own; for A/B runs).
+static bool region_jobs_enabled() {
+  const char* e = std::getenv("TENET_AMD_REGION_JOBS");
+  return !(e && e[0] == '0');
+}
+
 static int build_colmap(tnt_ctx* ctx, const tnt_contract_desc* d, const std::vector<GJob>& gj, tnt_artifact* a) {
   const tnt_transform_desc* t = d->out;
   if (!t || !out_epilogue_enabled() || t->n_inactive || d->n_uncovered) return TNT_OK;
@@ -1777,20 +1816,31 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     }
     no_skinny = narrow <= 0.02 * all;
   }
-  build_gemm_table(a->gemm, gj, ctx->knobs, -1, no_skinny);
-  if (int rc = upload_gemm_table(ctx, a->gemm, a->allocs)) return fail(rc);
-  std::vector<int64_t> uo, ul;
-  for (size_t i = 0; i < d->n_uncovered; ++i) {
+  for (size_t i = 0; i < d->n_uncovered; ++i)
     if (d->uncovered[i].offset < 0 || d->uncovered[i].len < 0 ||
         d->uncovered[i].offset + d->uncovered[i].len > d->core_len) {
       ctx->err = "uncovered region out of range";
       return fail(TNT_E_OOB);
     }
-    uo.push_back(d->uncovered[i].offset);
-    ul.push_back(d->uncovered[i].len);
+  // Without an output transform the uncovered regions (dst = beta dst) are
+  // jobs of the GEMM table with nothing contracted, in the launch of the
+  // contraction's own tiles (they are a few hundred elements, and a launch of
+  // their own costs more than their bytes); with one they stay regions of
+  // the core scratch, zeroed before the GEMM.
+  std::vector<GJob> table_jobs = gj;
+  std::vector<int64_t> uo, ul;
+  if (!d->out && region_jobs_enabled()) {
+    region_jobs(d->uncovered, d->n_uncovered, gj.empty() ? 0 : gj[0].ops, table_jobs);
+  } else {
+    for (size_t i = 0; i < d->n_uncovered; ++i) {
+      uo.push_back(d->uncovered[i].offset);
+      ul.push_back(d->uncovered[i].len);
+    }
   }
+  build_gemm_table(a->gemm, table_jobs, ctx->knobs, -1, no_skinny);
+  if (int rc = upload_gemm_table(ctx, a->gemm, a->allocs)) return fail(rc);
   a->n_unc = (uint32_t)uo.size();
-  for (const GJob& j : gj) a->core_writes += (uint64_t)std::max(j.m, 0) * (uint64_t)std::max(j.n, 0);
+  for (const GJob& j : table_jobs) a->core_writes += (uint64_t)std::max(j.m, 0) * (uint64_t)std::max(j.n, 0);
   for (int64_t l : ul) a->core_writes += (uint64_t)l;
   if (int rc = upload_vec(ctx, uo, &a->unc_off, a->allocs)) return fail(rc);
   if (int rc = upload_vec(ctx, ul, &a->unc_len, a->allocs)) return fail(rc);
