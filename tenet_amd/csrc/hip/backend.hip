@@ -205,9 +205,8 @@ struct GemmTable {
   std::vector<GJob> host_jobs;
   std::vector<GTile> host_tiles;
   std::vector<GJob> split_jobs;  // host_jobs with the narrow jobs emptied
-  // narrow outputs (n <= 16) on 64 x 64 workgroup tiles instead of the
-  // narrow-output kernel (artifacts whose output transform is folded into
-  // the tile epilogue, where those jobs are a small share of the flops)
+  // narrow outputs (n <= 16) on workgroup tiles instead of the narrow-output
+  // kernel (tables where those jobs are a small share of the flops)
   bool no_skinny = false;
 };
 
@@ -877,7 +876,8 @@ static void build_gemm_table(GemmTable& t, const std::vector<GJob>& jobs, const 
     // forced one-wave tiles also take narrow jobs of few rows (one launch for
     // a batch of tiny GEMMs instead of two)
     if (ovr >= 0 && (cfg != TILE_SKINNY || (ovr == TILE_SMALL && j.m <= 256))) cfg = ovr;
-    if (cfg == TILE_SKINNY && (j.n > 16 || no_skinny)) cfg = TILE_64x64;
+    if (cfg == TILE_SKINNY && j.n > 16) cfg = TILE_64x64;
+    if (cfg == TILE_SKINNY && no_skinny) cfg = ovr >= 0 && ovr != TILE_SKINNY && ovr != TILE_SMALL ? ovr : TILE_64x64;
     // nothing contracted: D = beta D, written by the workgroup-tile epilogue
     // (that kernel loads no operand element when k == 0), on the table's
     // workgroup shape so that such jobs join its launch
@@ -1806,8 +1806,15 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
     if (a->out->src_len != d->core_len) return fail((ctx->err = "output transform length mismatch", TNT_E_SHAPE));
   }
   if (int rc = build_colmap(ctx, d, gj, a)) return fail(rc);
+  // Narrow outputs (n <= 16) go to workgroup tiles, in the launch of the
+  // other tiles, when they are at most 2 % of the flops: their padded tiles
+  // cost less than a launch of the narrow-output kernel of their own (and an
+  // output transform folded into the epilogue needs workgroup tiles).
+  // TENET_AMD_NARROW_ON_TILES=0 keeps them on that kernel unless the output
+  // transform is folded (for A/B runs).
   bool no_skinny = false;
-  if (a->cmap) {  // narrow jobs to workgroup tiles when they are a small share of the work
+  const char* nt_env = std::getenv("TENET_AMD_NARROW_ON_TILES");
+  if (a->cmap || !(nt_env && nt_env[0] == '0')) {
     double all = 0.0, narrow = 0.0;
     for (const GJob& j : gj) {
       const double f = 2.0 * double(std::max(j.m, 0)) * double(std::max(j.n, 0)) * double(std::max(j.k, 0));
