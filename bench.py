@@ -913,7 +913,8 @@ def main():
         from tenet_amd.autotune import autotune
         settle(wl.run, rt.synchronize, args.settle_ms / 2)
         conts = list(wl.contractions) if hasattr(wl, "contractions") else [wl.contraction]
-        tuned = autotune(rt, conts, wl.run, reps=max(2, min(5, args.steps // 4)), log=log)
+        tuned = autotune(rt, conts, wl.run, reps=max(2, min(5, args.steps // 4)), log=log,
+                         graph_step=wl if hasattr(wl, "capture") else None)
     # Timed region 1: K steps as a caller runs them -> value, ms_per_step.  The
     # step's launches are replayed from a hipGraph (captured after tuning) where
     # the workload supports it; each replay runs every kernel of the step.

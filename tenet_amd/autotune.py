@@ -3,7 +3,10 @@
 For each contraction artifact of a workload the tuner times the workload
 step with every GEMM tile configuration of that artifact (the other
 artifacts held fixed) and keeps the fastest -- coordinate descent over the
-artifacts, in the order they run.  Results are keyed by the artifact's
+artifacts, in the order they run.  A workload that can record its step as a
+hipGraph is timed on graph replays: eager steps of a few tens of
+microseconds are bound by host issue, and their timings would rank the
+configurations by noise.  Results are keyed by the artifact's
 job-shape signature (the sorted (m, k, n) list), kept on the Runtime and,
 when TENET_AMD_TILE_CACHE names a file, persisted as JSON so later runs
 start tuned.
@@ -60,8 +63,29 @@ def _time(rt, step, reps):
     return (time.perf_counter() - t) / reps
 
 
-def autotune(rt, contractions, step, reps=5, log=None):
-    """Pick a tile configuration per contraction artifact; returns {signature: cfg}."""
+def _time_graph(rt, graph_step, reps):
+    """Seconds per hipGraph replay of graph_step's step (recorded afresh:
+    the tile tables of the configuration under test)."""
+    graph_step.capture()
+    try:
+        for _ in range(2):
+            graph_step.replay()
+        rt.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            graph_step.replay()
+        rt.synchronize()
+        return (time.perf_counter() - t) / reps
+    finally:
+        rt._check(N.hip().tnt_graph_destroy(rt._ctx, graph_step.graph))
+        graph_step.graph = None
+
+
+def autotune(rt, contractions, step, reps=5, log=None, graph_step=None):
+    """Pick a tile configuration per contraction artifact; returns {signature: cfg}.
+
+    graph_step: an object with capture() / replay() / graph (a GraphStep)
+    whose step is `step`; when given, configurations are timed on replays."""
     table = getattr(rt, "tile_table", None)
     if table is None:
         table = rt.tile_table = load_cache()
@@ -73,7 +97,7 @@ def autotune(rt, contractions, step, reps=5, log=None):
         best, best_t = -1, None
         for cfg in (-1,) + CONFIGS:
             apply(rt, c, cfg)
-            t = _time(rt, step, reps)
+            t = _time_graph(rt, graph_step, max(reps, 10)) if graph_step is not None else _time(rt, step, reps)
             if log:
                 log(f"[autotune] {sig} cfg {cfg}: {t * 1e3:.3f} ms")
             # leave the per-job default only for a clear win (timing noise is ~1-2%)

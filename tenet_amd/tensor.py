@@ -168,6 +168,8 @@ class HomSpace:
         self.nin = len(self.domain)
         self.len = int(h.tnth_space_len(self._h))
         self._sectors = None
+        self._key = None
+        self._adj = None
 
     def _leg(self, axis):
         h = N.host()
@@ -197,7 +199,10 @@ class HomSpace:
         return self.nout + self.nin
 
     def key(self):
-        return (self.rule.name, tuple(l.key() for l in self.codomain), tuple(l.key() for l in self.domain))
+        # spaces are immutable: the key (and the adjoint below) are built once
+        if self._key is None:
+            self._key = (self.rule.name, tuple(l.key() for l in self.codomain), tuple(l.key() for l in self.domain))
+        return self._key
 
     def __eq__(self, o):
         return isinstance(o, HomSpace) and o.key() == self.key()
@@ -245,7 +250,9 @@ class HomSpace:
         return sp
 
     def adjoint(self):
-        return HomSpace(self.rule, self.domain, self.codomain)
+        if self._adj is None:
+            self._adj = HomSpace(self.rule, self.domain, self.codomain)
+        return self._adj
 
     def __repr__(self):
         return f"HomSpace({self.rule.name}, {self.nout}<-{self.nin}, len={self.len})"
@@ -582,7 +589,7 @@ class TensorMap:
         if out is None:
             out = TensorMap(self.rt, c.dst_space)
             beta = 0.0
-        elif out.space != c.dst_space:
+        elif out.space is not c.dst_space and out.space != c.dst_space:
             raise TenetError(-3, "destination space does not match the contraction")
         prec = PRECISIONS[precision or self.rt.precision]
         self.rt._check(N.hip().tnt_contract(self.rt._ctx, c.art, out._buf, a._buf, b._buf, alpha, beta, prec))
