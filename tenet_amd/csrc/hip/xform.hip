@@ -22,6 +22,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace tnt_dev {
 
@@ -230,13 +231,15 @@ __device__ __forceinline__ void flat_item(const XformArgs& a, const XFlat& f, do
 // Multi items with at most NS source layouts; dst_j = sum_i U[j, i] src_i is
 // accumulated with fma in source order.  Few sources (NS <= 4, e.g. the fused
 // narrow contractions): the source layouts of the current outer index are
-// held in registers and each lane recouples two plane elements (e, e + 64)
-// per step, so 2 * nsrc loads are in flight before the stores.  More sources:
-// one element per lane per step, layouts read from the (scalar-cached) table,
-// which keeps the register budget -- and the occupancy -- of the wide groups.
-template <int NS>
+// held in registers and each lane recouples UN plane elements (e, e + 64,
+// ..., e + 64 (UN - 1)) per step, so UN * nsrc loads are in flight before the
+// stores.  More sources: one element per lane per step, layouts read from the
+// (scalar-cached) table, which keeps the register budget -- and the
+// occupancy -- of the wide groups.
+template <int NS, int UN = 2>
 __device__ void multi_item(const XformArgs& a, const XItem it) {
-  constexpr bool kTwo = NS <= 4;
+  constexpr bool kFew = NS <= 4;
+  static_assert(UN >= 1 && UN <= 8, "elements per lane and step");
   const XBlock& b = a.blocks[it.block];
   const int rank = b.rank;
   const int64_t n0 = b.dims[0];
@@ -257,7 +260,7 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
     outer_index(b, o, idx);
     uint32_t e = w.e_lo + lane;
     int64_t x = e % n0, y = e / n0;
-    if constexpr (kTwo) {
+    if constexpr (kFew) {
       int64_t sb[NS], sx[NS], sy[NS];
 #pragma unroll
       for (int i = 0; i < NS; ++i) {
@@ -271,42 +274,54 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
         }
       }
       while (e < w.e_hi) {
-        const bool two = e + 64 < w.e_hi;
-        int64_t x2 = x + dx, y2 = y + dy;
-        if (x2 >= n0) { x2 -= n0; y2 += 1; }
-        double v0[NS], v1[NS];
+        // plane coordinates of elements e + 64 u; past the end of the range a
+        // load repeats element e (unconditional loads) and nothing is stored
+        int64_t xs[UN], ys[UN];
+        bool ok[UN];
+        xs[0] = x;
+        ys[0] = y;
+        ok[0] = true;
 #pragma unroll
-        for (int i = 0; i < NS; ++i) {
-          v0[i] = v1[i] = 0.0;
-          if (i < ns) {
-            // unconditional: past the end the second load repeats the first
-            v0[i] = src[sb[i] + x * sx[i] + y * sy[i]];
-            v1[i] = src[two ? sb[i] + x2 * sx[i] + y2 * sy[i] : sb[i] + x * sx[i] + y * sy[i]];
-          }
+        for (int u = 1; u < UN; ++u) {
+          xs[u] = xs[u - 1] + dx;
+          ys[u] = ys[u - 1] + dy;
+          if (xs[u] >= n0) { xs[u] -= n0; ys[u] += 1; }
+          ok[u] = e + 64u * u < w.e_hi;
         }
+        double v[UN][NS];
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            v[u][i] = 0.0;
+            if (i < ns) v[u][i] = src[ok[u] ? sb[i] + xs[u] * sx[i] + ys[u] * sy[i] : sb[i] + x * sx[i] + y * sy[i]];
+          }
         for (int j = 0; j < nd; ++j) {
           const XLayout& Ld = L[j];
           const int64_t db = base_offset(Ld, idx, rank);
           const int64_t d0 = Ld.strides[0], d1 = rank > 1 ? Ld.strides[1] : 0;
-          double acc0 = 0.0, acc1 = 0.0;
+          double acc[UN];
+#pragma unroll
+          for (int u = 0; u < UN; ++u) acc[u] = 0.0;
 #pragma unroll
           for (int i = 0; i < NS; ++i) {
             if (i < ns) {
-              const double u = U[j * ns + i];
-              acc0 = fma(u, v0[i], acc0);
-              acc1 = fma(u, v1[i], acc1);
+              const double c = U[j * ns + i];
+#pragma unroll
+              for (int u = 0; u < UN; ++u) acc[u] = fma(c, v[u][i], acc[u]);
             }
           }
-          double* p0 = dst + db + x * d0 + y * d1;
-          *p0 = readd ? alpha * acc0 + beta * *p0 : alpha * acc0;
-          if (two) {
-            double* p1 = dst + db + x2 * d0 + y2 * d1;
-            *p1 = readd ? alpha * acc1 + beta * *p1 : alpha * acc1;
+#pragma unroll
+          for (int u = 0; u < UN; ++u) {
+            if (ok[u]) {
+              double* p = dst + db + xs[u] * d0 + ys[u] * d1;
+              *p = readd ? alpha * acc[u] + beta * *p : alpha * acc[u];
+            }
           }
         }
-        e += 128;
-        x = x2 + dx;
-        y = y2 + dy;
+        e += 64u * UN;
+        x = xs[UN - 1] + dx;
+        y = ys[UN - 1] + dy;
         if (x >= n0) { x -= n0; y += 1; }
       }
     } else {
@@ -415,9 +430,19 @@ __global__ __launch_bounds__(256) void xform_transpose_kernel(XformArgs a, uint3
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) transpose_item(a, a.items[begin + i], tile);
 }
 
-template <int NS>
+template <int NS, int UN = 2>
 __global__ __launch_bounds__(256) void xform_multi_kernel(XformArgs a, uint32_t begin, uint32_t n) {
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) multi_item<NS>(a, a.items[begin + i]);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) multi_item<NS, UN>(a, a.items[begin + i]);
+}
+
+// Plane elements per lane and step of the few-source Multi launches
+// (TENET_AMD_MULTI_UNROLL = 2 or 4, read once; for A/B runs).
+static int multi_unroll() {
+  static const int u = [] {
+    const char* e = std::getenv("TENET_AMD_MULTI_UNROLL");
+    return e && std::atoi(e) == 2 ? 2 : 4;
+  }();
+  return u;
 }
 
 // Every class in one launch (small plans: one launch costs more than the
@@ -486,7 +511,8 @@ void launch_xform(const XformArgs& a, int cls, uint32_t begin, uint32_t n, hipSt
       hipLaunchKernelGGL(xform_single_kernel, dim3(grid), dim3(256), 0, s, a, begin, n);
       break;
     case XC_MULTI4:
-      hipLaunchKernelGGL((xform_multi_kernel<4>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      if (multi_unroll() == 4) hipLaunchKernelGGL((xform_multi_kernel<4, 4>), dim3(grid), dim3(256), 0, s, a, begin, n);
+      else hipLaunchKernelGGL((xform_multi_kernel<4, 2>), dim3(grid), dim3(256), 0, s, a, begin, n);
       break;
     case XC_MULTI16:
       hipLaunchKernelGGL((xform_multi_kernel<16>), dim3(grid), dim3(256), 0, s, a, begin, n);
