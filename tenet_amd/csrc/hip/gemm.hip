@@ -60,7 +60,8 @@ constexpr int BK = 16;
 // Workgroup = GM x GN waves (wave w at row w % GM, column w / GM of the tile).
 // Register budget: 4-wave tiles at most 256 VGPRs+AGPRs per lane, so two
 // waves share each SIMD (one wave per SIMD leaves LDS and global latency
-// exposed); 8-wave tiles at most 128, four waves per SIMD.
+// exposed); 8-wave tiles and 64 x 64 tiles at most 128, four waves per SIMD
+// (for 64 x 64 tiles four workgroups per CU, as many as their LDS allows).
 //
 // Every global load of the main loop is unconditional, so the loop has no
 // branch but its trip test and the compiler keeps each K slab's loads in
@@ -79,7 +80,7 @@ constexpr int BK = 16;
 // k-step ks issue.
 template <typename T, int BM, int BN, int GM = 2, int GN = 2, bool BETA = false, typename S = double,
           bool AADJ = false, bool BADJ = false>
-__global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 ? 4 : 2))) void gemm_kernel(
+__global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM * GN == 8 || (BM == 64 && BN == 64) ? 4 : 2))) void gemm_kernel(
     GemmArgs g) {
   constexpr int NTH = 64 * GM * GN;
   constexpr int WM = BM / GM, WN = BN / GN;
@@ -148,18 +149,30 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   auto aoff = [&](int kk) { return AADJ ? (int64_t)kk : (int64_t)kk * lda; };
   auto boff = [&](int kk) { return BADJ ? (int64_t)kk * ldb : (int64_t)kk; };
 
-  S ra[A_LOADS], rb[B_LOADS];
-  auto store = [&](int buf) {
+  // Slab registers: one set, or two for the two-slab prefetch of 64 x 64
+  // tiles (their slab is 1024 MFMA cycles per wave, four workgroups per CU:
+  // one slab ahead left global latency exposed; 85 against 81 % of peak on
+  // 4096^3, 65 against 62 % on the U(1) environment's A^dagger.T2 jobs).
+  constexpr bool PF2 = BM == 64 && BN == 64;
+  S ra[A_LOADS], rb[B_LOADS], ra1[PF2 ? A_LOADS : 1], rb1[PF2 ? B_LOADS : 1];
+  auto store = [&](int buf, const S (&xa)[A_LOADS], const S (&xb)[B_LOADS]) {
 #pragma unroll
     for (int r = 0; r < A_LOADS; ++r) {
-      if (!AADJ) As[buf][akk(r)][a_c] = (T)ra[r];
-      else As[buf][a_c][a_o + a_step * r] = (T)ra[r];
+      if (!AADJ) As[buf][akk(r)][a_c] = (T)xa[r];
+      else As[buf][a_c][a_o + a_step * r] = (T)xa[r];
     }
 #pragma unroll
     for (int r = 0; r < B_LOADS; ++r) {
-      if (!BADJ) Bs[buf][b_c][b_o + b_step * r] = (T)rb[r];
-      else Bs[buf][bkk(r)][b_c] = (T)rb[r];
+      if (!BADJ) Bs[buf][b_c][b_o + b_step * r] = (T)xb[r];
+      else Bs[buf][bkk(r)][b_c] = (T)xb[r];
     }
+  };
+  // the slab at k offsets (ak_, bk_) from each load's own line start
+  auto load = [&](S (&xa)[A_LOADS], S (&xb)[B_LOADS], int64_t ak_, int64_t bk_) {
+#pragma unroll
+    for (int r = 0; r < A_LOADS; ++r) xa[r] = aptr(r)[ak_];
+#pragma unroll
+    for (int r = 0; r < B_LOADS; ++r) xb[r] = bptr(r)[bk_];
   };
   T fa[2][MT], fb[2][NT];
   auto frags = [&](int buf, int ks, int s) {
@@ -194,12 +207,17 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
       const S v = bptr(r)[boff(min(kk, kf - 1) - kk)];
       rb[r] = kk < kf ? v : S(0);
     }
-    store(0);
+    store(0, ra, rb);
   }
-  __syncthreads();
   // Slab t >= 1 covers k in [kf + (t - 1) BK, kf + t BK).
   int64_t ak = aoff(kf), bk = boff(kf);
   const int64_t a_kadv = aoff(BK), b_kadv = boff(BK);
+  if constexpr (PF2) {
+    if (nk > 1) load(ra1, rb1, ak, bk);  // slab 1
+    ak += a_kadv;
+    bk += b_kadv;
+  }
+  __syncthreads();
   // A wave whose part of the tile stores nothing (wholly outside the job, or
   // inside the rows / columns a moved edge tile only recomputes) skips its
   // LDS reads and MFMAs -- the waves of the other workgroups on its SIMD get
@@ -208,19 +226,7 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
   const bool live = __builtin_amdgcn_readfirstlane((m0 + wm + WM > m0o) && (m0 + wm < m) &&
                                                    (n0 + wn + WN > n0o) && (n0 + wn < n));
   if (live && nk > 0) frags(0, 0, 0);
-  // Double-buffered LDS: slab t is read from buffer t&1 while slab t+1 is
-  // fetched into registers and then written to the other buffer; one
-  // barrier per slab.
-  for (int t = 0; t < nk; ++t) {
-    const int cur = t & 1;
-    if (t + 1 < nk) {
-#pragma unroll
-      for (int r = 0; r < A_LOADS; ++r) ra[r] = aptr(r)[ak];
-#pragma unroll
-      for (int r = 0; r < B_LOADS; ++r) rb[r] = bptr(r)[bk];
-      ak += a_kadv;
-      bk += b_kadv;
-    }
+  auto compute = [&](int cur) {
     if (live) {
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
@@ -230,9 +236,48 @@ __global__ __launch_bounds__(64 * GM * GN) __attribute__((amdgpu_waves_per_eu(GM
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    if (t + 1 < nk) store(cur ^ 1);
-    __syncthreads();
-    if (live && t + 1 < nk) frags(cur ^ 1, 0, 0);
+  };
+  if constexpr (!PF2) {
+    // Double-buffered LDS: slab t is read from buffer t&1 while slab t+1 is
+    // fetched into registers and then written to the other buffer; one
+    // barrier per slab.
+    for (int t = 0; t < nk; ++t) {
+      const int cur = t & 1;
+      if (t + 1 < nk) {
+        load(ra, rb, ak, bk);
+        ak += a_kadv;
+        bk += b_kadv;
+      }
+      compute(cur);
+      if (t + 1 < nk) store(cur ^ 1, ra, rb);
+      __syncthreads();
+      if (live && t + 1 < nk) frags(cur ^ 1, 0, 0);
+    }
+  } else if (nk == 1) {
+    compute(0);  // the remainder slab only: no further load, no LDS write
+  } else if (nk > 1) {
+    // Two slabs in flight: while slab t is multiplied from LDS buffer t&1,
+    // slab t+1 waits in one register set (loaded an iteration earlier) and
+    // slab t+2 is fetched into the other; after the MFMAs slab t+1 goes to
+    // the other LDS buffer.  Past the last slab the loads re-read slab
+    // nk - 1 (unconditional; the values are not used).
+    const int64_t a_last = aoff(kf) + (int64_t)(nk - 2) * a_kadv, b_last = boff(kf) + (int64_t)(nk - 2) * b_kadv;
+    auto step = [&](int t, int cur, const S (&nxa)[A_LOADS], const S (&nxb)[B_LOADS], S (&isa)[A_LOADS],
+                    S (&isb)[B_LOADS]) {
+      const bool more = t + 2 < nk;
+      load(isa, isb, more ? ak : a_last, more ? bk : b_last);
+      ak += a_kadv;
+      bk += b_kadv;
+      compute(cur);
+      if (t + 1 < nk) store(cur ^ 1, nxa, nxb);
+      __syncthreads();
+      if (live && t + 1 < nk) frags(cur ^ 1, 0, 0);
+    };
+    for (int t = 0; t < nk; t += 2) {
+      step(t, 0, ra1, rb1, ra, rb);
+      if (t + 1 >= nk) break;
+      step(t + 1, 1, ra, rb, ra1, rb1);
+    }
   }
   if (!live) return;
   const double alpha = g.alpha, beta = g.beta;
