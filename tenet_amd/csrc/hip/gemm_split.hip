@@ -44,6 +44,7 @@
 
 #include "../../../include/tenet_hip.h"
 #include "split.hpp"
+#include "../common/i8_digits.hpp"
 
 namespace tnt_dev {
 
@@ -474,40 +475,45 @@ __global__ __launch_bounds__(256) void split_i8_chunk_exp(PrepArgs x) {
 
 // The S INT8 digit planes of one 64-k step: lane l holds 16 k of line l % 16
 // (v) and writes 16 bytes of each plane frag + p; e is the line's exponent
-// over all of K.  a = v 2^(7-e) is cut into d0 = rint(a) and S-1 base-256
-// digits by round-to-nearest; a digit of 128 becomes -128 with a carry into
-// the digit above, from the last digit up, so every digit fits INT8.
+// over all of K.  a = v 2^(7-e) is cut into d0 = rint(a) and S-1 balanced
+// base-256 digits (tnt_digits: for S <= 5 the remainder is rounded once to
+// 8 (S-1) bits and its bytes, biased by 128, are the digits; a top digit of
+// 128 carries into d0), so every digit fits INT8.  Byte j of plane word w
+// is element 4 w + j of the lane's 16.
 template <int S>
 __device__ __forceinline__ void i8_step_digits(const double (&v)[16], int e, void* planes, int64_t frag, int lane) {
   uint32_t packed[S][4];
+  if constexpr (S <= 5) {
 #pragma unroll
-  for (int p = 0; p < S; ++p)
+    for (int w = 0; w < 4; ++w) {
+      uint32_t fr[4], lead = 0u;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
+      for (int j = 0; j < 4; ++j) {
+        const int d0 = tnt_digits::i8_digits_fixed<S>(ldexp(v[4 * w + j], 7 - e), &fr[j]);  // |a| < 126.5
+        lead |= (uint32_t)(d0 & 0xff) << (8 * j);
+      }
+      packed[0][w] = lead;
+      // digit p is byte S - 1 - p of each fr[j]: gather it from the four
+      // elements into one word
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    double a = ldexp(v[q], 7 - e);  // |a| < 126.5
-    int d[S];
-    double r = rint(a);
-    d[0] = (int)r;
-    a -= r;                         // |a| <= 1/2, exact
-#pragma unroll
-    for (int p = 1; p < S; ++p) {
-      a *= 256.0;
-      r = rint(a);
-      d[p] = (int)r;                // in [-128, 128]
-      a -= r;
+      for (int p = 1; p < S; ++p) {
+        const int b = 8 * (S - 1 - p);
+        packed[p][w] = ((fr[0] >> b) & 0xffu) | (((fr[1] >> b) & 0xffu) << 8) | (((fr[2] >> b) & 0xffu) << 16) |
+                       (((fr[3] >> b) & 0xffu) << 24);
+      }
     }
-    int carry = 0;
+  } else {
 #pragma unroll
-    for (int p = S - 1; p >= 1; --p) {
-      d[p] += carry;
-      carry = d[p] > 127 ? 1 : 0;
-      d[p] -= carry << 8;
+    for (int p = 0; p < S; ++p)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) packed[p][w] = 0u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int d[S];
+      tnt_digits::i8_digits_stepwise<S>(ldexp(v[q], 7 - e), d);
+#pragma unroll
+      for (int p = 0; p < S; ++p) packed[p][q >> 2] |= (uint32_t)(d[p] & 0xff) << (8 * (q & 3));
     }
-    d[0] += carry;
-#pragma unroll
-    for (int p = 0; p < S; ++p) packed[p][q >> 2] |= (uint32_t)(d[p] & 0xff) << (8 * (q & 3));
   }
 #pragma unroll
   for (int p = 0; p < S; ++p) {
