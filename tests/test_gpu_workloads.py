@@ -343,3 +343,29 @@ def test_runtime_pool_leases_from_threads(rt):
         assert pool.quarantined == 1 and bad._ctx is None
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "split_i8"])
+def test_fmove_operand_transforms_on_two_streams_equal_one_stream(rt, monkeypatch, prec):
+    """Operand transforms that each move >= 64 MiB run on two streams (the rhs
+    one on the context's side stream, joined before the GEMM).  Same kernels,
+    so the result equals the one-stream result bit for bit; two contractions
+    issued back to back on the same artifact (the second reuses the transform
+    scratch payloads the first one's GEMM reads) must each keep their result."""
+    from tenet_amd.tensor import TensorMap
+    wl = SU2FMoveContract(rt, mult=24, twoj_max=8, precision=prec)
+    c = wl.contraction.desc.contents
+    assert c.lhs and c.rhs
+    A2 = TensorMap.rand(rt, wl.A.space, seed=11)
+    B2 = TensorMap.rand(rt, wl.B.space, seed=12)
+
+    def pair():
+        o1 = wl.A.contract([1, 3], wl.B, [2, 0], [0, 1, 2, 3], 2, precision=prec)
+        o2 = A2.contract([1, 3], B2, [2, 0], [0, 1, 2, 3], 2, precision=prec)
+        return o1.to_numpy(), o2.to_numpy()
+
+    forked = pair()
+    monkeypatch.setenv("TENET_AMD_FORK_TRANSFORMS", "0")
+    serial = pair()
+    assert np.array_equal(forked[0], serial[0]) and np.array_equal(forked[1], serial[1])
+    assert not np.array_equal(serial[0], serial[1])
