@@ -940,9 +940,18 @@ static double table_flops(const GemmTable& t) {
 
 // storage: payload type of d, a and b (TNT_F64, or TNT_F32 for FP64 / FP32
 // arithmetic only; the caller has checked the combination).
+// side / side_done: the rhs operand is still being written on `side` (the
+// operand transforms ran on two streams); `side_done` is recorded there after
+// that work.  The split-INT8 table overlaps its rhs pre-passes with it; every
+// other table waits for `side_done` first.
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
                           double alpha, double beta, int precision, int storage = TNT_F64,
-                          const int64_t* cmap_start = nullptr, const ColMapEntry* cmap = nullptr) {
+                          const int64_t* cmap_start = nullptr, const ColMapEntry* cmap = nullptr,
+                          hipStream_t side = nullptr, hipEvent_t side_done = nullptr) {
+  if (side && precision != TNT_PREC_SPLIT_I8) {
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, side_done, 0));
+    side = nullptr;
+  }
   TimedScope ts(ctx, 1 + precision, ctx->timing ? table_flops(t) : 0.0);
   if (storage == TNT_F32) {
     for (int c = 0; c < TILE_CFG_COUNT; ++c) {
@@ -958,20 +967,30 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
     return launch_status(ctx);
   }
   if (precision == TNT_PREC_SPLIT_BF16 || precision == TNT_PREC_SPLIT_I8) {
-    for (int c = 0; c < TILE_CFG_COUNT; ++c) {
-      if (!t.n_narrow_cfg[c]) continue;
-      TimedScope tk(ctx, timing_kind_kernel(TNT_PREC_F64, c), ctx->timing ? t.flops_narrow_cfg[c] : 0.0);
-      for (int o = 0; o < 4; ++o) {
-        if (!t.part_narrow[c][o]) continue;
-        const uint32_t first = t.part_start[c][o] + t.part_n[c][o] - t.part_narrow[c][o];
-        GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
-        launch_gemm_f64(g, c, o, t.part_narrow[c][o], ctx->stream);
+    // FP64 kernels for the narrow jobs (after the split run when the rhs is
+    // still being written on `side`: that run joins it)
+    auto narrow = [&]() {
+      for (int c = 0; c < TILE_CFG_COUNT; ++c) {
+        if (!t.n_narrow_cfg[c]) continue;
+        TimedScope tk(ctx, timing_kind_kernel(TNT_PREC_F64, c), ctx->timing ? t.flops_narrow_cfg[c] : 0.0);
+        for (int o = 0; o < 4; ++o) {
+          if (!t.part_narrow[c][o]) continue;
+          const uint32_t first = t.part_start[c][o] + t.part_n[c][o] - t.part_narrow[c][o];
+          GemmArgs g{t.jobs, t.tiles + first, d, a, b, alpha, beta};
+          launch_gemm_f64(g, c, o, t.part_narrow[c][o], ctx->stream);
+        }
       }
-    }
+    };
+    if (!side) narrow();
     if (!ctx->split) ctx->split.reset(split_workspace_create());
     const int scheme = precision == TNT_PREC_SPLIT_I8 ? SPLIT_SCHEME_I8 : SPLIT_SCHEME_BF16;
     int rc = split_gemm_run(ctx->split.get(), scheme, ctx->split_slices, t.split_jobs, t.jobs, d, a, b, alpha, beta,
-                            ctx->stream, ctx->capturing, ctx->live_graphs > 0, ctx->err);
+                            ctx->stream, ctx->capturing, ctx->live_graphs > 0, ctx->err, side, side_done);
+    if (side) {
+      // joined here too: the run may have returned before joining
+      HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, side_done, 0));
+      if (!rc) narrow();
+    }
     if (rc) return rc;
     return launch_status(ctx);
   }
@@ -2044,9 +2063,12 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
     L = a->scratch_l;
   }
-  if (fork) {
+  // the split-INT8 table overlaps its rhs pre-passes with the side stream
+  // and joins it itself (run_gemm_table); every other path joins here
+  const bool join_in_table = fork && precision == TNT_PREC_SPLIT_I8 && !a->out;
+  if (fork && !join_in_table) {
     HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
-  } else if (a->rhs) {
+  } else if (!fork && a->rhs) {
     run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
     R = a->scratch_r;
   }
@@ -2079,7 +2101,9 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     run_plan(ctx, a->out, static_cast<double*>(dst->ptr), a->scratch_c, alpha, beta, 0);
   } else {
     launch_scale_regions(static_cast<double*>(dst->ptr), a->unc_off, a->unc_len, a->n_unc, beta, ctx->stream);
-    rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision);
+    rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision,
+                        TNT_F64, nullptr, nullptr, join_in_table ? ctx->side : nullptr,
+                        join_in_table ? ctx->join_ev : nullptr);
     if (rc) return rc;
     if ((rc = shadow_core())) return rc;
   }

@@ -1062,7 +1062,7 @@ static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
 
 int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
                    uintptr_t d, uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s, bool capturing,
-                   bool graphs_alive, std::string& err) {
+                   bool graphs_alive, std::string& err, hipStream_t side, hipEvent_t side_done) {
   if (S < 2 || S > 6) {
     err = "split slices outside 2..6";
     return TNT_E_INVALID;
@@ -1226,15 +1226,39 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob
     px.items_a = t.exp_items_a;
     px.side[0].prefix = t.prefix + 2 * (n_jobs + 1);
     px.side[1].prefix = t.prefix + 3 * (n_jobs + 1);
-    if (t.exp_items_a + t.exp_items_b)
-      hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a + t.exp_items_b), dim3(256), 0, s, px);
+    auto digits = [&](const PrepArgs& q, uint32_t n, hipStream_t st) {
+      switch (S) {
+        case 2: launch_digits_i8_s<2>(q, n, st); break;
+        case 3: launch_digits_i8_s<3>(q, n, st); break;
+        case 4: launch_digits_i8_s<4>(q, n, st); break;
+        case 6: launch_digits_i8_s<6>(q, n, st); break;
+        default: launch_digits_i8_s<5>(q, n, st); break;
+      }
+    };
+    if (side) {
+      // rhs side on `side` (a grid of side-1 items only: items_a = 0), lhs side
+      // here (a grid of the side-0 items only), joined before the GEMM
+      PrepArgs pxb = px, pab = pa;
+      pxb.items_a = 0;
+      pab.items_a = 0;
+      if (t.exp_items_b) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_b), dim3(256), 0, side, pxb);
+      digits(pab, t.items_b, side);
+      SPLIT_HIP(hipEventRecord(side_done, side));
+      if (t.exp_items_a) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a), dim3(256), 0, s, px);
+      digits(pa, t.items_a, s);
+      SPLIT_HIP(hipStreamWaitEvent(s, side_done, 0));
+    } else {
+      if (t.exp_items_a + t.exp_items_b)
+        hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a + t.exp_items_b), dim3(256), 0, s, px);
+      digits(pa, items, s);
+    }
     SplitGemmArgs g{t.jobs, t.tiles, d, ap, bp, ar, br, alpha, beta};
     switch (S) {
-      case 2: launch_digits_i8_s<2>(pa, items, s); launch_split_i8_s<2>(t, g, s); break;
-      case 3: launch_digits_i8_s<3>(pa, items, s); launch_split_i8_s<3>(t, g, s); break;
-      case 4: launch_digits_i8_s<4>(pa, items, s); launch_split_i8_s<4>(t, g, s); break;
-      case 6: launch_digits_i8_s<6>(pa, items, s); launch_split_i8_s<6>(t, g, s); break;
-      default: launch_digits_i8_s<5>(pa, items, s); launch_split_i8_s<5>(t, g, s); break;
+      case 2: launch_split_i8_s<2>(t, g, s); break;
+      case 3: launch_split_i8_s<3>(t, g, s); break;
+      case 4: launch_split_i8_s<4>(t, g, s); break;
+      case 6: launch_split_i8_s<6>(t, g, s); break;
+      default: launch_split_i8_s<5>(t, g, s); break;
     }
     SPLIT_HIP(hipGetLastError());
     return TNT_OK;
