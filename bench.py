@@ -568,6 +568,8 @@ SECONDARY = (  # (name, arguments, time limit in seconds; each takes 5-25 s)
     ("su2_fmove_f64", ["--workload", "su2_fmove", "--steps", "10", "--warmup", "2"], 150),
     ("su2_fmove_split_i8", ["--workload", "su2_fmove", "--precision", "split_i8", "--steps", "10",
                             "--warmup", "2", "--cpu-baseline", "off"], 90),
+    ("su2_fmove_cross_split_i8", ["--workload", "su2_fmove", "--fmove", "cross", "--precision", "split_i8",
+                                  "--steps", "10", "--warmup", "2", "--cpu-baseline", "off"], 90),
     ("batched_f32", ["--workload", "batched", "--precision", "f32"], 90),
     ("serve_u1", ["--workload", "serve", "--rule", "U1", "--cpu-baseline", "off"], 90),
     ("serve_su2", ["--workload", "serve", "--rule", "SU2", "--cpu-baseline", "off"], 90),
