@@ -940,14 +940,14 @@ static double table_flops(const GemmTable& t) {
 
 // storage: payload type of d, a and b (TNT_F64, or TNT_F32 for FP64 / FP32
 // arithmetic only; the caller has checked the combination).
-// side / side_done: the rhs operand is still being written on `side` (the
-// operand transforms ran on two streams); `side_done` is recorded there after
-// that work.  The split-INT8 table overlaps its rhs pre-passes with it; every
-// other table waits for `side_done` first.
+// side / side_done: operand side_op (0 lhs, 1 rhs) is still being written on
+// `side` (its transform was forked there); `side_done` is recorded there
+// after that work.  The split-INT8 table runs that operand's pre-passes on
+// `side` and the other's here; every other table waits for `side_done` first.
 static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr_t a, uintptr_t b,
                           double alpha, double beta, int precision, int storage = TNT_F64,
                           const int64_t* cmap_start = nullptr, const ColMapEntry* cmap = nullptr,
-                          hipStream_t side = nullptr, hipEvent_t side_done = nullptr) {
+                          hipStream_t side = nullptr, hipEvent_t side_done = nullptr, int side_op = 1) {
   if (side && precision != TNT_PREC_SPLIT_I8) {
     HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, side_done, 0));
     side = nullptr;
@@ -985,7 +985,7 @@ static int run_gemm_table(tnt_ctx* ctx, const GemmTable& t, uintptr_t d, uintptr
     if (!ctx->split) ctx->split.reset(split_workspace_create());
     const int scheme = precision == TNT_PREC_SPLIT_I8 ? SPLIT_SCHEME_I8 : SPLIT_SCHEME_BF16;
     int rc = split_gemm_run(ctx->split.get(), scheme, ctx->split_slices, t.split_jobs, t.jobs, d, a, b, alpha, beta,
-                            ctx->stream, ctx->capturing, ctx->live_graphs > 0, ctx->err, side, side_done);
+                            ctx->stream, ctx->capturing, ctx->live_graphs > 0, ctx->err, side, side_done, side_op);
     if (side) {
       // joined here too: the run may have returned before joining
       HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, side_done, 0));
@@ -2038,14 +2038,25 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
       return rc;
     return launch_status(ctx);
   }
-  // Both operand transforms large: the rhs one runs on the side stream while
-  // the lhs one runs here (two HBM-bound passes of short runs side by side
-  // move more bytes per second than one after the other: 3.61 -> 3.32 ms for
-  // the SU(2) F-move's pair); joined before the GEMM.  Serial under graph
-  // capture, per-kernel timing and the shadow check.
-  const bool fork = a->lhs && a->rhs && !ctx->capturing && !ctx->timing && !ctx->shadow &&
-                    std::min(a->lhs->bytes_moved, a->rhs->bytes_moved) >= kForkPlanBytes && fork_enabled();
-  if (fork) {
+  // Operand transforms on two streams.  Both large: the rhs one runs on the
+  // side stream while the lhs one runs here (two HBM-bound passes of short
+  // runs side by side move more bytes per second than one after the other:
+  // 3.61 -> 3.32 ms for the SU(2) F-move's pair).  One large transform and a
+  // split-INT8 table (no output transform): that transform runs on the side
+  // stream and the other operand's exponent and digit passes run here beside
+  // it.  The split-INT8 table joins the side stream before its GEMM itself
+  // (run_gemm_table); every other path joins before the GEMM here.  Serial
+  // under graph capture, per-kernel timing and the shadow check.
+  const bool big_l = a->lhs && a->lhs->bytes_moved >= kForkPlanBytes;
+  const bool big_r = a->rhs && a->rhs->bytes_moved >= kForkPlanBytes;
+  const bool split_table = precision == TNT_PREC_SPLIT_I8 && !a->out;
+  int fork_op = -1;  // operand (0 lhs, 1 rhs) whose transform runs on the side stream
+  if (!ctx->capturing && !ctx->timing && !ctx->shadow && fork_enabled()) {
+    if (big_l && big_r) fork_op = 1;
+    else if (split_table && big_r && !a->lhs) fork_op = 1;
+    else if (split_table && big_l && !a->rhs) fork_op = 0;
+  }
+  if (fork_op >= 0) {
     if (!ctx->side) {
       HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
       HIPCHK(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
@@ -2054,24 +2065,21 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     HIPCHK(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
     std::swap(ctx->stream, ctx->side);
-    run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
+    if (fork_op == 1) run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
+    else run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
     std::swap(ctx->stream, ctx->side);
-    R = a->scratch_r;
     HIPCHK(ctx, hipEventRecord(ctx->join_ev, ctx->side));
   }
   if (a->lhs) {
-    run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
+    if (fork_op != 0) run_plan(ctx, a->lhs, a->scratch_l, L, 1.0, 0.0, 1);
     L = a->scratch_l;
   }
-  // the split-INT8 table overlaps its rhs pre-passes with the side stream
-  // and joins it itself (run_gemm_table); every other path joins here
-  const bool join_in_table = fork && precision == TNT_PREC_SPLIT_I8 && !a->out;
-  if (fork && !join_in_table) {
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
-  } else if (!fork && a->rhs) {
-    run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
+  if (a->rhs) {
+    if (fork_op != 1) run_plan(ctx, a->rhs, a->scratch_r, R, 1.0, 0.0, 1);
     R = a->scratch_r;
   }
+  const bool join_in_table = fork_op >= 0 && split_table;
+  if (fork_op >= 0 && !join_in_table) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   int rc;
   // shadow check of the core destination: the GEMM table's tiles (FP64 / FP32
   // tables; the split precisions run their own tiles) and the uncovered regions
@@ -2103,7 +2111,7 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     launch_scale_regions(static_cast<double*>(dst->ptr), a->unc_off, a->unc_len, a->n_unc, beta, ctx->stream);
     rc = run_gemm_table(ctx, a->gemm, (uintptr_t)dst->ptr, (uintptr_t)L, (uintptr_t)R, alpha, beta, precision,
                         TNT_F64, nullptr, nullptr, join_in_table ? ctx->side : nullptr,
-                        join_in_table ? ctx->join_ev : nullptr);
+                        join_in_table ? ctx->join_ev : nullptr, join_in_table ? fork_op : 1);
     if (rc) return rc;
     if ((rc = shadow_core())) return rc;
   }

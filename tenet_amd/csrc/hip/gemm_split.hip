@@ -1062,7 +1062,7 @@ static int split_shape_for(const GJob& j, int64_t total_64x64_tiles, int S) {
 
 int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob>& host_jobs, const GJob* dev_jobs,
                    uintptr_t d, uintptr_t a, uintptr_t b, double alpha, double beta, hipStream_t s, bool capturing,
-                   bool graphs_alive, std::string& err, hipStream_t side, hipEvent_t side_done) {
+                   bool graphs_alive, std::string& err, hipStream_t side, hipEvent_t side_done, int side_op) {
   if (S < 2 || S > 6) {
     err = "split slices outside 2..6";
     return TNT_E_INVALID;
@@ -1236,16 +1236,18 @@ int split_gemm_run(SplitWorkspace* ws, int scheme, int S, const std::vector<GJob
       }
     };
     if (side) {
-      // rhs side on `side` (a grid of side-1 items only: items_a = 0), lhs side
-      // here (a grid of the side-0 items only), joined before the GEMM
+      // operand b's pre-passes as grids of its items only (items_a = 0),
+      // operand a's as grids of the first items_a items; operand side_op's on
+      // `side`, the other's here; joined before the GEMM
       PrepArgs pxb = px, pab = pa;
       pxb.items_a = 0;
       pab.items_a = 0;
-      if (t.exp_items_b) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_b), dim3(256), 0, side, pxb);
-      digits(pab, t.items_b, side);
+      hipStream_t sa = side_op == 0 ? side : s, sb = side_op == 0 ? s : side;
+      if (t.exp_items_b) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_b), dim3(256), 0, sb, pxb);
+      digits(pab, t.items_b, sb);
+      if (t.exp_items_a) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a), dim3(256), 0, sa, px);
+      digits(pa, t.items_a, sa);
       SPLIT_HIP(hipEventRecord(side_done, side));
-      if (t.exp_items_a) hipLaunchKernelGGL(split_i8_chunk_exp, dim3(t.exp_items_a), dim3(256), 0, s, px);
-      digits(pa, t.items_a, s);
       SPLIT_HIP(hipStreamWaitEvent(s, side_done, 0));
     } else {
       if (t.exp_items_a + t.exp_items_b)

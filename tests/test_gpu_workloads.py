@@ -369,3 +369,27 @@ def test_fmove_operand_transforms_on_two_streams_equal_one_stream(rt, monkeypatc
     serial = pair()
     assert np.array_equal(forked[0], serial[0]) and np.array_equal(forked[1], serial[1])
     assert not np.array_equal(serial[0], serial[1])
+
+
+def test_split_i8_single_transform_on_side_stream_equals_one_stream(rt, monkeypatch):
+    """A contraction with one large operand transform (B's legs contracted in
+    reversed order, A's domain as it is) in split-INT8: B's transform and its
+    exponent and digit passes run on the side stream while A's passes run on
+    the main stream; the result equals the one-stream result bit for bit and
+    the FP64 one within 1e-10."""
+    from tenet_amd.tensor import TensorMap
+    from tenet_amd.workloads import su2_fmove_space
+    _, V = su2_fmove_space(24, 8)
+    A = TensorMap.rand(rt, [V, V], [V, V], seed=21)
+    B = TensorMap.rand(rt, [V, V], [V, V], seed=22)
+    args = ([2, 3], B, [1, 0], [0, 1, 2, 3], 2)
+    c = rt.contraction(A.space, B.space, [2, 3], [1, 0], [0, 1, 2, 3], 2).desc.contents
+    assert c.rhs and not c.lhs and not c.out
+    assert c.rhs.contents.src_len * 16 >= 64 * 2 ** 20  # the transform moves >= 64 MiB
+    forked = A.contract(*args, precision="split_i8").to_numpy()
+    again = A.contract(*args, precision="split_i8").to_numpy()
+    monkeypatch.setenv("TENET_AMD_FORK_TRANSFORMS", "0")
+    serial = A.contract(*args, precision="split_i8").to_numpy()
+    assert np.array_equal(forked, serial) and np.array_equal(again, serial)
+    ref = A.contract(*args, precision="f64").to_numpy()
+    assert rel(serial, ref) < 1e-10
