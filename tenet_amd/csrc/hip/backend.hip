@@ -2057,10 +2057,20 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
     else if (split_table && big_l && !a->rhs) fork_op = 0;
   }
   if (fork_op >= 0) {
-    if (!ctx->side) {
-      HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-      HIPCHK(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-      HIPCHK(ctx, hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    if (!ctx->side) {  // the stream and both events, or none of them
+      hipStream_t st = nullptr;
+      hipEvent_t ef = nullptr, ej = nullptr;
+      const hipError_t e1 = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      const hipError_t e2 = e1 == hipSuccess ? hipEventCreateWithFlags(&ef, hipEventDisableTiming) : e1;
+      const hipError_t e3 = e2 == hipSuccess ? hipEventCreateWithFlags(&ej, hipEventDisableTiming) : e2;
+      if (e3 != hipSuccess) {
+        if (ef) (void)hipEventDestroy(ef);
+        if (st) (void)hipStreamDestroy(st);
+        FAIL(ctx, TNT_E_HIP, std::string("side stream for the operand transforms: ") + hipGetErrorString(e3));
+      }
+      ctx->side = st;
+      ctx->fork_ev = ef;
+      ctx->join_ev = ej;
     }
     HIPCHK(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
