@@ -273,6 +273,14 @@ int tnt_contract_set_tile_config(tnt_ctx* ctx, tnt_artifact* art, int cfg);
  * the general route. */
 enum tnt_route { TNT_ROUTE_TRANSFORM_GEMM = 0, TNT_ROUTE_FUSED_NARROW = 1, TNT_ROUTE_TINY_GATHERED = 2 };
 int tnt_contract_route(const tnt_artifact* art, int* route);
+/* Stream order: all work of the call is ordered on the context stream
+ * (tnt_ctx_stream).  When both operand transforms move at least 64 MiB (or,
+ * for TNT_PREC_SPLIT_I8 without an output transform, one of them does), one
+ * transform runs on a second stream of the context, forked from and joined
+ * back into the context stream by events inside the call, so work queued on
+ * the context stream after the call still sees the whole result.  Under
+ * graph capture, per-kernel timing and the shadow check it runs on the
+ * context stream alone (TENET_AMD_FORK_TRANSFORMS=0: always). */
 int tnt_contract(tnt_ctx* ctx, const tnt_artifact* art, tnt_buf* dst, const tnt_buf* lhs,
                  const tnt_buf* rhs, double alpha, double beta, int precision);
 
