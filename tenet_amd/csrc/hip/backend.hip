@@ -245,6 +245,9 @@ struct tnt_artifact {
   int64_t* g_idx = nullptr;
   double* g_scale = nullptr;
   uint32_t n_gather = 0;     // coefficients
+  // the fused plan is all few-source Multi items of small blocks: the
+  // coefficients are gathered inside its recoupling launch
+  bool gather_fold = false;
   // gathered small-GEMM route of tiny contractions (FP64 calls): element maps
   // of the Single-only transforms and one tile list (32 x 32) over every job
   bool tiny = false;
@@ -1661,6 +1664,14 @@ static bool out_epilogue_enabled() {
   return !(e && e[0] == '0');
 }
 
+// The fused narrow route gathers its coefficients inside the recoupling
+// launch where it can (TENET_AMD_GATHER_FOLD=0: a gather launch first, for
+// A/B runs).
+static bool gather_fold_enabled() {
+  const char* e = std::getenv("TENET_AMD_GATHER_FOLD");
+  return !(e && e[0] == '0');
+}
+
 // Operand transforms of a contraction run on two streams when each moves at
 // least this many bytes (TENET_AMD_FORK_TRANSFORMS=0: always one stream, for
 // A/B runs).
@@ -1901,6 +1912,13 @@ int tnt_contract_upload(tnt_ctx* ctx, const tnt_contract_desc* d, tnt_artifact**
         if (int rc = upload_vec(ctx, nf.gather_ptr, &a->g_ptr, a->allocs)) return fail(rc);
         if (int rc = upload_vec(ctx, nf.gather_idx, &a->g_idx, a->allocs)) return fail(rc);
         if (int rc = upload_vec(ctx, nf.gather_scale, &a->g_scale, a->allocs)) return fail(rc);
+        const auto& cc = a->fused->d;
+        bool fold = cc.cls_count[XC_MULTI4] > 0;
+        for (int c = 0; c < XC_COUNT; ++c)
+          if (c != XC_MULTI4 && cc.cls_count[c]) fold = false;
+        for (size_t bi = 0; fold && bi < nf.desc.n_blocks; ++bi)
+          if ((size_t)nf.desc.blocks[bi].dst_count * nf.desc.blocks[bi].src_count > (size_t)kMaxGatherCoef) fold = false;
+        a->gather_fold = fold;
         *out = a;  // the fused route needs no scratch payloads
         return TNT_OK;
       }
@@ -2022,8 +2040,21 @@ int tnt_contract(tnt_ctx* ctx, const tnt_artifact* a, tnt_buf* dst, const tnt_bu
   const double* L = static_cast<const double*>(lhs->ptr);
   const double* R = static_cast<const double*>(rhs->ptr);
   if (a->fused) {
-    launch_coeff_gather(a->fused->d.coeffs, R, a->g_ptr, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
-    run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
+    if (a->gather_fold && !ctx->shadow && gather_fold_enabled()) {
+      // one launch: coefficients gathered per item, then the recoupling
+      const tnt_plan* p = a->fused;
+      XformArgs x{p->d.blocks, p->d.layouts, p->d.items, p->d.flat, p->d.coeffs, static_cast<double*>(dst->ptr),
+                  L, alpha, beta, 0};
+      x.gptr = a->g_ptr;
+      x.gidx = a->g_idx;
+      x.gscale = a->g_scale;
+      x.gsrc = R;
+      TimedScope ts(ctx, 0, p->bytes_moved);
+      launch_xform_multi_gather(x, p->d.cls_begin[XC_MULTI4], p->d.cls_count[XC_MULTI4], ctx->stream);
+    } else {
+      launch_coeff_gather(a->fused->d.coeffs, R, a->g_ptr, a->g_idx, a->g_scale, a->n_gather, ctx->stream);
+      run_plan(ctx, a->fused, static_cast<double*>(dst->ptr), L, alpha, beta, 0);
+    }
     return launch_status(ctx);
   }
   if (a->tiny && precision == TNT_PREC_F64) {

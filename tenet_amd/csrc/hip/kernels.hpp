@@ -82,6 +82,13 @@ struct XformArgs {
   double alpha;
   double beta;
   int overwrite;
+  // coefficients gathered per item instead of read from coeffs (the fused
+  // narrow route): coefficient c = sum over gptr[c] <= p < gptr[c + 1] of
+  // gscale[p] * gsrc[gidx[p]] (launch_xform_multi_gather only)
+  const int64_t* gptr = nullptr;
+  const int64_t* gidx = nullptr;
+  const double* gscale = nullptr;
+  const double* gsrc = nullptr;
 };
 
 // Work-item classes of a plan: Single and Scale items, Multi items by source
@@ -122,6 +129,13 @@ constexpr int kTinyKBlocks = 8;  // jobs with K <= 8 * this load all of K at onc
 // operand; t < n).
 void launch_coeff_gather(double* coeffs, const double* src, const int64_t* ptr, const int64_t* idx,
                          const double* scale, uint32_t n, hipStream_t s);
+// Multi items [begin, begin + n) of at most 4 sources whose coefficients are
+// gathered per item (XformArgs::gptr...) into LDS, in one launch with the
+// recoupling: the same sums in the same order as launch_coeff_gather, so the
+// result equals gather-then-launch_xform bit for bit.  Every block of these
+// items must have ndst * nsrc <= kMaxGatherCoef.
+constexpr int kMaxGatherCoef = 1024;
+void launch_xform_multi_gather(const XformArgs& a, uint32_t begin, uint32_t n, hipStream_t s);
 
 // Device GEMM job: element offsets from the kernel's base pointers.
 struct GJob {

@@ -237,7 +237,7 @@ __device__ __forceinline__ void flat_item(const XformArgs& a, const XFlat& f, do
 // (scalar-cached) table, which keeps the register budget -- and the
 // occupancy -- of the wide groups.
 template <int NS, int UN = 2>
-__device__ void multi_item(const XformArgs& a, const XItem it) {
+__device__ void multi_item(const XformArgs& a, const XItem it, const double* Ublock = nullptr) {
   constexpr bool kFew = NS <= 4;
   static_assert(UN >= 1 && UN <= 8, "elements per lane and step");
   const XBlock& b = a.blocks[it.block];
@@ -247,7 +247,7 @@ __device__ void multi_item(const XformArgs& a, const XItem it) {
   const int wave = threadIdx.x >> 6;
   const int nd = b.ndst, ns = b.nsrc;
   const XLayout* __restrict__ L = a.layouts + b.lay;
-  const double* __restrict__ U = a.coeffs + b.coeff;
+  const double* __restrict__ U = Ublock ? Ublock : a.coeffs + b.coeff;
   const double* __restrict__ src = a.src;
   double* __restrict__ dst = a.dst;
   const double alpha = a.alpha, beta = a.beta;
@@ -433,6 +433,34 @@ __global__ __launch_bounds__(256) void xform_transpose_kernel(XformArgs a, uint3
 template <int NS, int UN = 2>
 __global__ __launch_bounds__(256) void xform_multi_kernel(XformArgs a, uint32_t begin, uint32_t n) {
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) multi_item<NS, UN>(a, a.items[begin + i]);
+}
+
+// Few-source Multi items with their coefficients gathered per item: the
+// workgroup sums the block's ndst x nsrc coefficients into LDS (each as
+// coeff_gather_kernel forms it), then recouples the item from there.
+__global__ __launch_bounds__(256) void xform_multi_gather_kernel(XformArgs a, uint32_t begin, uint32_t n) {
+  __shared__ double su[kMaxGatherCoef];
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const XItem it = a.items[begin + i];
+    const XBlock& b = a.blocks[it.block];
+    const int nc = b.ndst * b.nsrc;
+    for (int t = threadIdx.x; t < nc; t += 256) {
+      const int64_t c = (int64_t)b.coeff + t;
+      const int64_t p0 = a.gptr[c], p1 = a.gptr[c + 1];
+      double u = p0 < p1 ? a.gscale[p0] * a.gsrc[a.gidx[p0]] : 0.0;  // one term: the plain product
+      for (int64_t p = p0 + 1; p < p1; ++p) u = fma(a.gscale[p], a.gsrc[a.gidx[p]], u);
+      su[t] = u;
+    }
+    __syncthreads();
+    multi_item<4, 4>(a, it, su);
+    __syncthreads();  // su is refilled by the next item
+  }
+}
+
+void launch_xform_multi_gather(const XformArgs& a, uint32_t begin, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t grid = n < 4096u ? n : 4096u;
+  hipLaunchKernelGGL(xform_multi_gather_kernel, dim3(grid), dim3(256), 0, s, a, begin, n);
 }
 
 // Plane elements per lane and step of the few-source Multi launches

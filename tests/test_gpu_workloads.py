@@ -393,3 +393,27 @@ def test_split_i8_single_transform_on_side_stream_equals_one_stream(rt, monkeypa
     assert np.array_equal(forked, serial) and np.array_equal(again, serial)
     ref = A.contract(*args, precision="f64").to_numpy()
     assert rel(serial, ref) < 1e-10
+
+
+@pytest.mark.parametrize("rule_name,D,count", [("U1", 256, "dimension"), ("SU2", 256, "dimension"),
+                                               ("SU2", 64, "multiplets")])
+def test_fused_mpo_step_gather_in_launch_equals_gather_launch(rt, monkeypatch, rule_name, D, count):
+    """The fused MPO step gathers its coefficients inside the recoupling launch
+    (per item, into LDS) when its plan is all few-source Multi items; the sums
+    are formed as the separate gather launch forms them, so the result is the
+    same bit for bit, with beta != 0 too."""
+    from tenet_amd.tensor import TensorMap
+    wl = MPSEnvUpdate(rt, rule_name, D, "f64", su2_count=count)
+    assert wl.contractions[1].route == "fused_narrow"
+    T1 = wl._tmp["T1"]
+    X0 = TensorMap.rand(rt, wl._tmp["T2"].space, seed=5)
+
+    def step():
+        X = TensorMap.from_numpy(rt, X0.space, X0.to_numpy())
+        T1.contract(MPSEnvUpdate.T2_AXES[0], wl.W, *MPSEnvUpdate.T2_AXES[1:], out=X, alpha=0.75, beta=-1.5)
+        return X.to_numpy()
+
+    folded = step()
+    monkeypatch.setenv("TENET_AMD_GATHER_FOLD", "0")
+    separate = step()
+    assert np.array_equal(folded, separate)
