@@ -1,8 +1,9 @@
 """The C ABI's struct layouts and enum values against the Rust binding.
 
-INTEGRATION.md holds the `#[repr(C)]` structs and constants a TeNeT
-maintainer would add (tenet-dense/src/hip_ffi.rs).  This test parses that
-block, computes each struct's C layout from the Rust field types (size,
+The repository's integration guide holds, in a `hip_ffi.rs` code block, the
+`#[repr(C)]` structs and constants a TeNeT maintainer would add
+(tenet-dense/src/hip_ffi.rs).  This test parses that block, computes each
+struct's C layout from the Rust field types (size,
 alignment, field offsets under the C rules `#[repr(C)]` follows), and
 compares it with what gcc reports for include/tenet_hip.h.  The ctypes
 mirror in tenet_amd/_native.py is checked against the same numbers, and the
