@@ -560,6 +560,14 @@ static int64_t min_plan_items() {
   return e ? std::max<int64_t>(0, std::atoll(e)) : kMinPlanItems;
 }
 
+// Element updates per Multi work item (TENET_AMD_MULTI_ITEM_WORK overrides at
+// upload, for A/B runs).
+static int64_t multi_item_work() {
+  const char* e = std::getenv("TENET_AMD_MULTI_ITEM_WORK");
+  const int64_t v = e ? std::atoll(e) : 0;
+  return v >= 256 ? v : kMultiItemWork;
+}
+
 static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out) {
   if (!d) FAIL(ctx, TNT_E_INVALID, "null transform descriptor");
   std::vector<XBlock> blocks;
@@ -572,7 +580,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
   // item sizes: Single kSingleItem elements, Multi ~kMultiItemWork element
   // updates (elements x layouts / 2), both shrunk for small plans so that each
   // class has about min_plan_items() items (never below 256 elements)
-  int64_t single_target = kSingleItem, multi_work_target = kMultiItemWork;
+  int64_t single_target = kSingleItem, multi_work_target = multi_item_work();
   {
     int64_t single_elems = 0, multi_work = 0;
     for (size_t bi = 0; bi < d->n_blocks; ++bi) {
@@ -588,7 +596,7 @@ static int build_plan(tnt_ctx* ctx, const tnt_transform_desc* d, tnt_plan** out)
     const int64_t want = min_plan_items();
     if (want > 0 && single_elems < want * kSingleItem)
       single_target = std::max<int64_t>(256, ((single_elems / want + 63) / 64) * 64);
-    if (want > 0 && multi_work < want * kMultiItemWork)
+    if (want > 0 && multi_work < want * multi_work_target)
       multi_work_target = std::max<int64_t>(256, multi_work / want);
   }
   auto add_block = [&](int kind, const std::vector<uint32_t>& dl, const std::vector<uint32_t>& sl,
